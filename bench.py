@@ -1,0 +1,143 @@
+"""Throughput of the spectral_rs hot path on one MI355X.
+
+Workload (N=1): the plain FP32 unmagnetized shock on a 4096 x 1024 grid,
+32 particles per cell per species (ions m=16, electrons m=1; delta = 16 gives
+2 x 133,169,152 = 266,338,304 particles), gamma0 = 10, c = 10 cells/omega_pe
+with dt = 0.045 (0.45 cells/step, 10-cell skin depth), 4 binomial passes.
+Particles are generated on the GPU (positions exactly as the reference,
+momenta from a seeded counter-based RNG); they are resident in HBM before the
+timed region.  One step = one iteration of the reference's run() loop:
+zero, push+move+deposit of both species, ghost fold, filter, 7 forward and
+6 inverse 2D FFTs with the spectral Maxwell update.
+
+Prints one JSON line (rank 0) with the metric, a roofline object for the
+dominant kernel (the fused particle pass) and a CPU baseline: the oracle's
+C++ restatement of the reference CPU path, timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+C2 = dict(size_x=4096, size_y=1024, delta=16, dens=32, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0)
+SPECIES = [(1.0, 16.0, 0.0025), (-1.0, 1.0, 0.01)]  # (q, m, vth): ions then electrons (lib.rs:124-126)
+
+
+def particles(cfg):
+    return 2 * cfg["dens"] * (cfg["size_x"] - 2 * cfg["delta"]) * cfg["size_y"]
+
+
+def cpu_baseline(seconds_hint=15.0):
+    """The oracle (restated reference CPU path, f32, OpenMP like rayon) on a
+    512 x 128 crop of the same physics with 32 ppc per species."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from tests.helpers import shock_species
+
+    cfg = dict(C2, size_x=512, size_y=128)
+    r = O.RefSim(prec=4, **cfg)
+    for k, (q, m, vth) in enumerate(SPECIES):
+        r.add_species(q, m, **shock_species(cfg["size_x"], cfg["size_y"], cfg["delta"], cfg["dens"],
+                                            cfg["c"], cfg["gamma_inj"], vth, seed=k, ft=np.float32))
+    r.prestep()
+    t0 = time.perf_counter()
+    r.step(1)
+    one = time.perf_counter() - t0
+    steps = max(2, int(seconds_hint / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    r.step(steps)
+    el = time.perf_counter() - t0
+    n = particles(cfg)
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    thr = int(os.environ.get("OMP_NUM_THREADS", cores))
+    return {"value": n * steps / el, "unit": "particle-steps/s", "cores": thr, "kind": "port",
+            "sample": f"oracle f32, 512x128 crop, 32 ppc/species ({n} particles), {steps} steps, "
+                      f"{el:.1f} s; serial mover/deposit, threaded push/filter/FFT as the reference"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--size", default=None, help="override grid, e.g. 1024x256 (not the metric workload)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    from spectral_rs_amd import capi
+
+    cfg = dict(C2)
+    if args.size:
+        sx, sy = (int(v) for v in args.size.lower().split("x"))
+        cfg.update(size_x=sx, size_y=sy)
+    n = particles(cfg)
+    cells = (cfg["size_x"] + 2) * (cfg["size_y"] + 2)
+    print(f"bench: {cfg['size_x']}x{cfg['size_y']}, {n} particles, warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+    sim = capi.GpuSim(prec=4, **cfg)
+    for k, (q, m, vth) in enumerate(SPECIES):
+        sim.init_species(q, m, vth, seed=0)
+    sim.prestep()
+    sim.step(args.warmup)
+    sim.sync()
+    sim.set_timing(True)
+    sim.timers(reset=True)
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    el = time.perf_counter() - t0
+    tm = sim.timers(reset=True)
+    value = n * args.steps / el
+    print(f"bench: {el:.3f} s, {value:.4g} particle-steps/s, phases ms {tm}", file=sys.stderr, flush=True)
+
+    # Fused particle pass (push + move + deposit, both species): algorithmic
+    # bytes = 48 B per particle (24 B state read + 24 B written) + 44 B per
+    # cell (6 field grids gathered, 5 deposit grids written).
+    part_ms = tm["particles"] / args.steps
+    traffic_alg = 48.0 * n + 44.0 * cells
+    achieved = traffic_alg / (part_ms * 1e-3) / 1e9
+    out = {
+        "metric": "particle-steps/s on 1 MI355X",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (GPU-generated shock initial state, seed 0)",
+        "config": {"workload": f"plain FP32 unmagnetized shock {cfg['size_x']}x{cfg['size_y']}, 32 ppc/species, "
+                               f"delta 16, gamma0 10, mass ratio 16, 4 binomial passes",
+                   "particles": n, "grid": [cfg["size_x"], cfg["size_y"]]},
+        "phases_ms_per_step": {k: v / args.steps for k, v in tm.items()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_push_move_deposit (both species per step)",
+                     "bytes_per_launch_algorithmic": traffic_alg, "ms_per_launch": part_ms},
+    }
+    sim.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

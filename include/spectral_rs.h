@@ -1,0 +1,168 @@
+/*
+ * spectral_rs.h -- C ABI of the MI355X (gfx950) implementation of the
+ * spectral_rs per-timestep hot path: particle move + current/density deposit,
+ * binomial current filter, spectral Maxwell solve through a 2D FFT, and the
+ * relativistic Boris push.
+ *
+ * The reference (pcrumley/spectral_rs, Rust) has no FFI; its seam is the
+ * method set that `run()` calls.  Each entry point below replaces one of those
+ * methods (file:line relative to the reference root) and keeps its meaning:
+ * same grids (padded (Ny+2)x(Nx+2) row-major, 1-cell ghost ring), same cell
+ * convention (ghost-offset ix in 1..=Nx, iy in 1..=Ny, dx,dy in [-0.5,0.5)),
+ * same normalisations.  Where the reference panics through assert!, the call
+ * returns SP_EINVARIANT and sp_last_error() names the assert it mirrors.
+ *
+ * Threading: one host thread per context.  Calls are queued on the context's
+ * HIP stream and return before the GPU finishes, except the getters/setters
+ * and sp_sync(), which wait.  Invariant violations raised by a kernel are
+ * reported by the next call that waits.  Nothing is allocated after
+ * sp_create()/sp_species_*() (like the reference, which allocates only in
+ * Flds::new and Prtl::new).
+ *
+ * Precision: SP_F32 is the reference's default build (Float = f32); SP_F64 is
+ * `--features dprec` (Cargo.toml:27-31, src/lib.rs:15-19).  Real arrays
+ * passed through `void*` hold float (SP_F32) or double (SP_F64) elements.
+ */
+#ifndef SPECTRAL_RS_H
+#define SPECTRAL_RS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sp_status {
+  SP_OK = 0,
+  SP_EINVAL = -1,      /* bad argument (config validation, src/lib.rs:75-113) */
+  SP_EHIP = -2,        /* HIP runtime error */
+  SP_EINVARIANT = -3,  /* a reference assert! would have fired */
+  SP_ENOMEM = -4
+};
+
+enum sp_precision { SP_F32 = 0, SP_F64 = 1 };
+
+/* Behaviour when a particle reaches ix <= delta.  The reference has no left
+ * boundary and asserts (src/prtls.rs:163-170); SP_LEFT_REFLECT mirrors the
+ * right wall (src/prtls.rs:172-188) at cell delta+1. */
+enum sp_left_edge { SP_LEFT_ASSERT = 0, SP_LEFT_REFLECT = 1 };
+
+/* Grid ids.  0..10 name the padded real grids of Flds (src/flds/mod.rs:15-37);
+ * the same ids name their spectra for sp_get_spectrum/sp_set_spectrum/
+ * sp_fft2d, and SP_BXW..SP_BZW name the persistent half-step spectra
+ * b_*_wrk (src/flds/mod.rs:31-33). */
+enum sp_grid {
+  SP_EX = 0, SP_EY, SP_EZ, SP_BX, SP_BY, SP_BZ, SP_JX, SP_JY, SP_JZ,
+  SP_RHO /* Flds.dsty */, SP_DENS, SP_BXW, SP_BYW, SP_BZW
+};
+
+/* Mirrors Params + Setup of config.toml (src/lib.rs:34-66) plus the cargo
+ * features as run-time flags. */
+typedef struct sp_params {
+  int32_t size_x, size_y; /* even (src/lib.rs:76-87) */
+  int32_t delta;          /* 2*delta < size_x (src/lib.rs:88-92) */
+  int32_t dens;           /* particles per species per cell */
+  int32_t n_pass;         /* binomial filter passes */
+  double dt, c;           /* reference units: 1/omega_pe and c/omega_pe cells */
+  double gamma_inj;       /* > 1 (src/lib.rs:93-95) */
+  int32_t precision;      /* enum sp_precision */
+  int32_t periodic_x;     /* cargo feature `periodic` (src/prtls.rs:136-143) */
+  int32_t checked;        /* 1 = the asserts of the default build are checked */
+  int32_t left_edge;      /* enum sp_left_edge */
+  int32_t sort_interval;  /* steps between cell sorts of the particles, 0 = never */
+  int32_t reserved[3];
+} sp_params;
+
+typedef struct sp_diag {
+  double energy_e;      /* sum |E|^2 / 2 over the interior (FP64) */
+  double energy_b;      /* sum |B|^2 / 2 */
+  double energy_kin[8]; /* per species sum (gamma - 1) m c^2 */
+  double momentum[3];   /* sum m p over all species */
+  double gauss_pre;     /* pre-correction Gauss residual of the last solve */
+  double gauss_post;    /* post-correction residual (round-off level) */
+  int64_t steps;        /* loop iterations run so far */
+} sp_diag;
+
+typedef struct sp_ctx sp_ctx;
+
+/* Flds::new + Sim::new (src/flds/mod.rs:40-65, src/lib.rs:247-264): validates
+ * the parameters like Config::new (src/lib.rs:68-117) and allocates every grid,
+ * spectrum and FFT table on the current HIP device. */
+int sp_create(const sp_params* params, sp_ctx** out);
+int sp_destroy(sp_ctx* ctx);
+/* Text of the last error on this context (or of the last failed sp_create
+ * when ctx is NULL). */
+const char* sp_last_error(const sp_ctx* ctx);
+
+/* Prtl::new with caller-provided state (src/prtls.rs:60-84): charge, mass as
+ * there (alpha, beta derived the same way); ix, iy are the reference's usize
+ * cells as int32; dx..pz are Float arrays of length n.  gamma (psa) is
+ * sqrt(1 + |p|^2/c^2), the expression of src/prtls.rs:248-249 and :440.
+ * The arrays are copied; the caller keeps them. */
+int sp_species_add(sp_ctx* ctx, double charge, double mass, int64_t n, const int32_t* ix,
+                   const int32_t* iy, const void* dx, const void* dy, const void* px,
+                   const void* py, const void* pz, int* id);
+/* Prtl::new generated on the GPU (src/prtls.rs:191-261): positions exactly as
+ * initialize_positions, momenta by the initialize_velocities recipe (Gaussian
+ * of width vth*c, Zenitani flip, boost by gamma_inj) from a counter-based RNG
+ * keyed by (seed, species id, particle index) instead of thread_rng. */
+int sp_species_init(sp_ctx* ctx, double charge, double mass, double vth, uint64_t seed, int* id);
+int sp_species_count(const sp_ctx* ctx, int id, int64_t* n);
+/* Copies a species back in the order it was added; psa may be NULL. */
+int sp_get_species(sp_ctx* ctx, int id, int64_t n, int32_t* ix, int32_t* iy, void* dx, void* dy,
+                   void* px, void* py, void* pz, void* psa);
+
+/* Padded real grids: count = (size_x+2)*(size_y+2) elements. */
+int sp_get_grid(sp_ctx* ctx, int grid, void* out, size_t count);
+int sp_set_grid(sp_ctx* ctx, int grid, const void* in, size_t count);
+/* Spectra: interleaved (re, im), count = 2*size_x*size_y elements. */
+int sp_get_spectrum(sp_ctx* ctx, int spec, void* out, size_t count);
+int sp_set_spectrum(sp_ctx* ctx, int spec, const void* in, size_t count);
+
+/* Zero J, rho and dens including ghosts (src/lib.rs:157-167). */
+int sp_zero_currents(sp_ctx* ctx);
+/* Sim::move_and_deposit (src/lib.rs:478-489) for one species. */
+int sp_move_and_deposit(sp_ctx* ctx, int id);
+/* Field::deposit_ghosts on Jx, Jy, Jz, rho and dens (src/lib.rs:174-182,
+ * src/flds/field.rs:259-373). */
+int sp_deposit_ghosts(sp_ctx* ctx);
+/* Flds::update (src/flds/mod.rs:87-309). */
+int sp_fields_update(sp_ctx* ctx);
+/* Prtl::boris_push (src/prtls.rs:262-443) for one species. */
+int sp_boris_push(sp_ctx* ctx, int id);
+/* `passes` calls of Field::binomial_filter_2_d on one grid
+ * (src/flds/field.rs:375-426). */
+int sp_filter(sp_ctx* ctx, int grid, int passes);
+/* Fft2D::fft / Fft2D::inv_fft on one spectrum (src/flds/fft_2d.rs:91-172). */
+int sp_fft2d(sp_ctx* ctx, int spec, int inverse);
+/* The pointwise spectral part of Flds::update (src/flds/mod.rs:105-278) on
+ * spectra already holding the forward transforms. */
+int sp_spectral_update(sp_ctx* ctx);
+
+/* The untimed pre-loop of run(): move_and_deposit of every species, then the
+ * ghost fold (src/lib.rs:130-142). */
+int sp_prestep(sp_ctx* ctx);
+/* n iterations of the loop body of run() (src/lib.rs:155-199): zero, move &
+ * deposit every species, fold ghosts, Flds::update, Boris push every species.
+ * Adjacent push / move&deposit pairs run as one fused particle pass. */
+int sp_step(sp_ctx* ctx, int n);
+/* Wait for the context's stream and report deferred invariant violations. */
+int sp_sync(sp_ctx* ctx);
+/* Phase timers (no reference counterpart beyond its stdout timers).  With
+ * timing on, every phase is bracketed by HIP events on the context stream:
+ * ms[0] "dep" (zero + move&deposit + ghost fold, src/lib.rs:169-184),
+ * ms[1] "solve" (Flds::update, :187-189), ms[2] "push" (stand-alone Boris
+ * pushes, :193-197) and ms[3] the particle passes alone (inside "dep"; a
+ * fused push+move&deposit pass counts there).  Milliseconds accumulated since
+ * the last reset; reset != 0 clears them. */
+int sp_set_timing(sp_ctx* ctx, int on);
+int sp_get_timers(sp_ctx* ctx, double* ms4, int reset);
+/* FP64 diagnostics (no reference counterpart): energies, momentum, Gauss
+ * residuals. */
+int sp_get_diag(sp_ctx* ctx, sp_diag* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPECTRAL_RS_H */

@@ -1,0 +1,201 @@
+"""ctypes binding of the C ABI in include/spectral_rs.h (libspectral_rs_hip.so).
+
+``GpuSim`` is a thin object over one ``sp_ctx``: one method per entry point,
+numpy arrays in and out, errors raised as ``SpError`` with the library's
+message.  There is no CPU fallback: if the HIP library is not built, or no
+GPU is visible, constructing a ``GpuSim`` raises.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libspectral_rs_hip.so")
+
+SP_OK, SP_EINVAL, SP_EHIP, SP_EINVARIANT, SP_ENOMEM = 0, -1, -2, -3, -4
+SP_F32, SP_F64 = 0, 1
+SP_LEFT_ASSERT, SP_LEFT_REFLECT = 0, 1
+GRIDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5, "jx": 6, "jy": 7, "jz": 8,
+         "rho": 9, "dsty": 9, "dens": 10, "bxw": 11, "byw": 12, "bzw": 13}
+
+# exported symbols, in the order include/spectral_rs.h declares them
+SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_species_init",
+           "sp_species_count", "sp_get_species", "sp_get_grid", "sp_set_grid", "sp_get_spectrum",
+           "sp_set_spectrum", "sp_zero_currents", "sp_move_and_deposit", "sp_deposit_ghosts",
+           "sp_fields_update", "sp_boris_push", "sp_filter", "sp_fft2d", "sp_spectral_update",
+           "sp_prestep", "sp_step", "sp_sync", "sp_set_timing", "sp_get_timers", "sp_get_diag"]
+
+
+class SpParams(ct.Structure):
+    _fields_ = [("size_x", ct.c_int32), ("size_y", ct.c_int32), ("delta", ct.c_int32),
+                ("dens", ct.c_int32), ("n_pass", ct.c_int32), ("dt", ct.c_double), ("c", ct.c_double),
+                ("gamma_inj", ct.c_double), ("precision", ct.c_int32), ("periodic_x", ct.c_int32),
+                ("checked", ct.c_int32), ("left_edge", ct.c_int32), ("sort_interval", ct.c_int32),
+                ("reserved", ct.c_int32 * 3)]
+
+
+class SpDiag(ct.Structure):
+    _fields_ = [("energy_e", ct.c_double), ("energy_b", ct.c_double), ("energy_kin", ct.c_double * 8),
+                ("momentum", ct.c_double * 3), ("gauss_pre", ct.c_double), ("gauss_post", ct.c_double),
+                ("steps", ct.c_int64)]
+
+
+class SpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run python -m spectral_rs_amd.build)")
+    L = ct.CDLL(path)
+    vp, i32, i64, d = ct.c_void_p, ct.c_int, ct.c_int64, ct.c_double
+    L.sp_create.argtypes = [ct.POINTER(SpParams), ct.POINTER(vp)]
+    L.sp_destroy.argtypes = [vp]
+    L.sp_last_error.restype = ct.c_char_p
+    L.sp_last_error.argtypes = [vp]
+    L.sp_species_add.argtypes = [vp, d, d, i64] + [vp] * 7 + [ct.POINTER(i32)]
+    L.sp_species_init.argtypes = [vp, d, d, d, ct.c_uint64, ct.POINTER(i32)]
+    L.sp_species_count.argtypes = [vp, i32, ct.POINTER(i64)]
+    L.sp_get_species.argtypes = [vp, i32, i64] + [vp] * 8
+    for n in ["sp_get_grid", "sp_set_grid", "sp_get_spectrum", "sp_set_spectrum"]:
+        getattr(L, n).argtypes = [vp, i32, vp, ct.c_size_t]
+    for n in ["sp_zero_currents", "sp_deposit_ghosts", "sp_fields_update", "sp_spectral_update",
+              "sp_prestep", "sp_sync"]:
+        getattr(L, n).argtypes = [vp]
+    for n in ["sp_move_and_deposit", "sp_boris_push", "sp_step", "sp_set_timing"]:
+        getattr(L, n).argtypes = [vp, i32]
+    L.sp_filter.argtypes = [vp, i32, i32]
+    L.sp_fft2d.argtypes = [vp, i32, i32]
+    L.sp_get_timers.argtypes = [vp, vp, i32]
+    L.sp_get_diag.argtypes = [vp, ct.POINTER(SpDiag)]
+    for n in SYMBOLS:
+        if n not in ("sp_last_error",):
+            getattr(L, n).restype = ct.c_int
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+class GpuSim:
+    """One simulation context on the current GPU (Sim + Flds + Vec<Prtl>)."""
+
+    def __init__(self, size_x, size_y, delta, dens, n_pass=4, dt=0.1, c=3.0, gamma_inj=5.0,
+                 prec=4, periodic_x=False, checked=True, left_edge=SP_LEFT_ASSERT, sort_interval=0):
+        L = load()
+        self.nx, self.ny = size_x, size_y
+        self.prec = prec
+        self.ft = np.float64 if prec == 8 else np.float32
+        self.ct = np.complex128 if prec == 8 else np.complex64
+        p = SpParams(size_x, size_y, delta, dens, n_pass, dt, c, gamma_inj,
+                     SP_F64 if prec == 8 else SP_F32, int(periodic_x), int(checked), int(left_edge),
+                     int(sort_interval))
+        h = ct.c_void_p()
+        rc = L.sp_create(ct.byref(p), ct.byref(h))
+        if rc != SP_OK:
+            raise SpError(rc, L.sp_last_error(None).decode())
+        self.h = h
+        self.nspecies = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().sp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != SP_OK:
+            raise SpError(rc, load().sp_last_error(self.h).decode())
+        return rc
+
+    # ---- species ----
+    def add_species(self, charge, mass, ix, iy, dx, dy, px, py, pz, psa=None):
+        ft = self.ft
+        ix = np.ascontiguousarray(ix, np.int32)
+        iy = np.ascontiguousarray(iy, np.int32)
+        fl = [np.ascontiguousarray(a, ft) for a in (dx, dy, px, py, pz)]
+        sid = ct.c_int(-1)
+        self._chk(load().sp_species_add(self.h, charge, mass, ix.size, _p(ix), _p(iy),
+                                        *[_p(a) for a in fl], ct.byref(sid)))
+        self.nspecies += 1
+        return sid.value
+
+    def init_species(self, charge, mass, vth, seed):
+        sid = ct.c_int(-1)
+        self._chk(load().sp_species_init(self.h, charge, mass, vth, seed, ct.byref(sid)))
+        self.nspecies += 1
+        return sid.value
+
+    def count(self, sid):
+        n = ct.c_int64()
+        self._chk(load().sp_species_count(self.h, sid, ct.byref(n)))
+        return n.value
+
+    def species(self, sid):
+        n = self.count(sid)
+        ix = np.empty(n, np.int32)
+        iy = np.empty(n, np.int32)
+        fl = [np.empty(n, self.ft) for _ in range(6)]
+        self._chk(load().sp_get_species(self.h, sid, n, _p(ix), _p(iy), *[_p(a) for a in fl]))
+        return dict(ix=ix.astype(np.int64), iy=iy.astype(np.int64), dx=fl[0], dy=fl[1], px=fl[2],
+                    py=fl[3], pz=fl[4], psa=fl[5])
+
+    # ---- grids ----
+    def grid(self, name):
+        a = np.empty((self.ny + 2, self.nx + 2), self.ft)
+        self._chk(load().sp_get_grid(self.h, GRIDS[name], _p(a), a.size))
+        return a
+
+    def set_grid(self, name, a):
+        a = np.ascontiguousarray(a, self.ft).reshape(self.ny + 2, self.nx + 2)
+        self._chk(load().sp_set_grid(self.h, GRIDS[name], _p(a), a.size))
+
+    def spectrum(self, name):
+        a = np.empty((self.ny, self.nx), self.ct)
+        self._chk(load().sp_get_spectrum(self.h, GRIDS[name], _p(a), 2 * a.size))
+        return a
+
+    def set_spectrum(self, name, a):
+        a = np.ascontiguousarray(a, self.ct).reshape(self.ny, self.nx)
+        self._chk(load().sp_set_spectrum(self.h, GRIDS[name], _p(a), 2 * a.size))
+
+    # ---- reference seams ----
+    def zero_currents(self): self._chk(load().sp_zero_currents(self.h))
+    def move_and_deposit(self, sid): self._chk(load().sp_move_and_deposit(self.h, sid))
+    def fold_all(self): self._chk(load().sp_deposit_ghosts(self.h))
+    def fields_update(self): self._chk(load().sp_fields_update(self.h))
+    def boris_push(self, sid): self._chk(load().sp_boris_push(self.h, sid))
+    def filter(self, name, passes): self._chk(load().sp_filter(self.h, GRIDS[name], passes))
+    def fft2d(self, name, inverse=False): self._chk(load().sp_fft2d(self.h, GRIDS[name], int(inverse)))
+    def spectral_update(self): self._chk(load().sp_spectral_update(self.h))
+    def prestep(self): self._chk(load().sp_prestep(self.h))
+    def step(self, n=1): self._chk(load().sp_step(self.h, n))
+    def sync(self): self._chk(load().sp_sync(self.h))
+    def set_timing(self, on=True): self._chk(load().sp_set_timing(self.h, int(on)))
+
+    def timers(self, reset=False):
+        t = np.zeros(4, np.float64)
+        self._chk(load().sp_get_timers(self.h, _p(t), int(reset)))
+        return dict(dep=t[0], solve=t[1], push=t[2], particles=t[3])
+
+    def diag(self):
+        d = SpDiag()
+        self._chk(load().sp_get_diag(self.h, ct.byref(d)))
+        return d

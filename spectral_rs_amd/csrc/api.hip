@@ -1,0 +1,694 @@
+// api.hip -- the C ABI of include/spectral_rs.h: context lifetime, buffers,
+// host<->device copies, and the order of kernel launches that reproduces the
+// reference's run() loop (src/lib.rs:119-202).
+//
+// Particle passes: run() calls boris_push at the end of iteration t and
+// move_and_deposit at the start of iteration t+1 with nothing in between that
+// touches particles (src/lib.rs:171-196).  sp_step therefore leaves the push
+// of its last iteration pending and performs it fused into the next
+// iteration's move&deposit pass; any other entry point (and sp_sync) first
+// runs a pending push on its own, so every observable state matches the
+// reference's.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sp_internal.h"
+
+using namespace sp;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct SpeciesBuf {
+  int64_t n = 0;
+  double charge = 0, mass = 0;
+  int32_t* cell = nullptr;
+  void *dx = nullptr, *dy = nullptr, *px = nullptr, *py = nullptr, *pz = nullptr;
+  int32_t* tag = nullptr;
+};
+
+struct TimerPair {
+  hipEvent_t a, b;
+  int phase;
+};
+
+}  // namespace
+
+struct sp_ctx {
+  sp_params p{};
+  bool f64 = false;
+  size_t esz = 4;
+  hipStream_t st = nullptr;
+  int device = 0;
+  void* grid[11] = {};
+  void* work[4] = {};
+  void* spec[14] = {};
+  FftDim fx{}, fy{};
+  int* flags_d = nullptr;
+  int* flags_h = nullptr;
+  std::vector<SpeciesBuf> species;
+  std::string err;
+  bool push_pending = false;
+  int64_t steps = 0;
+  // phase timers (dep, solve, push, particle pass)
+  bool timing = false;
+  std::vector<TimerPair> open_pairs;
+  std::vector<hipEvent_t> free_events;
+  double t_ms[4] = {0, 0, 0, 0};
+  size_t ngrid_elems() const { return (size_t)(p.size_x + 2) * (p.size_y + 2); }
+  size_t nspec_elems() const { return (size_t)p.size_x * p.size_y; }
+};
+
+namespace {
+
+int fail(sp_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(ctx, SP_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+template <class T> Geo<T> geo(const sp_ctx* c) {
+  Geo<T> g;
+  g.nx = c->p.size_x;
+  g.ny = c->p.size_y;
+  g.pitch = c->p.size_x + 2;
+  g.delta = c->p.delta;
+  g.wall = c->p.size_x + 1 - c->p.delta;
+  g.lwall = c->p.delta + 1;
+  g.periodic_x = c->p.periodic_x;
+  g.checked = c->p.checked;
+  g.left_edge = c->p.left_edge;
+  g.dt = (T)c->p.dt;
+  g.c = (T)c->p.c;
+  g.csqinv = T(1.) / (g.c * g.c);
+  g.half_dt = T(0.5) * g.dt;
+  return g;
+}
+
+template <class T> SpeciesDev<T> sdev(const sp_ctx* c, const SpeciesBuf& s) {
+  SpeciesDev<T> d;
+  d.n = s.n;
+  d.cell = s.cell;
+  d.dx = (T*)s.dx;
+  d.dy = (T*)s.dy;
+  d.px = (T*)s.px;
+  d.py = (T*)s.py;
+  d.pz = (T*)s.pz;
+  d.tag = s.tag;
+  const T q = (T)s.charge, m = (T)s.mass;
+  const T dt = (T)c->p.dt, cc = (T)c->p.c;
+  d.charge = q;
+  d.beta = (q / m) * T(0.5) * dt;          // src/prtls.rs:61
+  d.alpha = (q / m) * T(0.5) * dt / cc;    // src/prtls.rs:62
+  return d;
+}
+
+GridSet gset(const sp_ctx* c) {
+  GridSet g;
+  for (int i = 0; i < 11; ++i) g.g[i] = c->grid[i];
+  return g;
+}
+
+bool plan_dim(FftDim& d, int n) {
+  d.n = n;
+  d.nrad = 0;
+  d.radix_code = 0;
+  int m = n;
+  auto push = [&](int r) {
+    if (d.nrad < 16) d.radix_code |= (uint64_t)r << (4 * d.nrad);
+    d.nrad++;
+    m /= r;
+  };
+  while (m % 8 == 0) push(8);
+  while (m % 4 == 0) push(4);
+  while (m % 2 == 0) push(2);
+  while (m % 3 == 0) push(3);
+  return m == 1 && d.nrad <= 16;
+}
+
+// ---- phase timers ----
+hipEvent_t take_event(sp_ctx* c) {
+  if (!c->free_events.empty()) {
+    hipEvent_t e = c->free_events.back();
+    c->free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+struct PhaseScope {
+  sp_ctx* c;
+  int phase;
+  hipEvent_t a = nullptr;
+  PhaseScope(sp_ctx* c_, int ph) : c(c_), phase(ph) {
+    if (c->timing) {
+      a = take_event(c);
+      if (a) hipEventRecord(a, c->st);
+    }
+  }
+  ~PhaseScope() {
+    if (c->timing && a) {
+      hipEvent_t b = take_event(c);
+      if (b) {
+        hipEventRecord(b, c->st);
+        c->open_pairs.push_back({a, b, phase});
+      }
+    }
+  }
+};
+void harvest_timers(sp_ctx* c) {  // after a stream sync
+  for (auto& tp : c->open_pairs) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, tp.a, tp.b) == hipSuccess) c->t_ms[tp.phase] += ms;
+    c->free_events.push_back(tp.a);
+    c->free_events.push_back(tp.b);
+  }
+  c->open_pairs.clear();
+}
+
+// ---- launch sequences, templated on Float ----
+template <class T> void zero_currents(sp_ctx* c) {
+  const int ids[5] = {SP_JX, SP_JY, SP_JZ, SP_RHO, SP_DENS};
+  for (int id : ids) fill<T>((T*)c->grid[id], T(0), (int64_t)c->ngrid_elems(), c->st);
+}
+
+template <class T> void fold(sp_ctx* c) {
+  T* g[5] = {(T*)c->grid[SP_JX], (T*)c->grid[SP_JY], (T*)c->grid[SP_JZ], (T*)c->grid[SP_RHO],
+             (T*)c->grid[SP_DENS]};
+  launch_fold_ghosts<T>(g, 5, c->p.size_x, c->p.size_y, c->st);
+}
+
+template <class T> void flush_push(sp_ctx* c) {
+  if (!c->push_pending) return;
+  PhaseScope ps(c, 2);
+  const Geo<T> g = geo<T>(c);
+  const GridSet gs = gset(c);
+  for (auto& s : c->species) launch_push<T>(sdev<T>(c, s), g, gs, c->flags_d, c->st);
+  c->push_pending = false;
+}
+
+template <class T> void fields_update(sp_ctx* c) {
+  const int nx = c->p.size_x, ny = c->p.size_y;
+  {  // mod.rs:90-94
+    T* g[4] = {(T*)c->grid[SP_JX], (T*)c->grid[SP_JY], (T*)c->grid[SP_JZ], (T*)c->grid[SP_RHO]};
+    T* w[4] = {(T*)c->work[0], (T*)c->work[1], (T*)c->work[2], (T*)c->work[3]};
+    launch_binomial<T>(g, w, 4, nx, ny, c->p.n_pass, c->st);
+  }
+  {  // mod.rs:97-103, 122-124: 7 forward transforms, spatial->spectral fused
+    FftBatch b{};
+    const int ids[7] = {SP_JX, SP_JY, SP_JZ, SP_EX, SP_EY, SP_EZ, SP_RHO};
+    b.count = 7;
+    for (int i = 0; i < 7; ++i) {
+      b.spec[i] = c->spec[ids[i]];
+      b.grid[i] = c->grid[ids[i]];
+      b.scale[i] = 1.0;
+    }
+    b.scale[6] = (double)(T(1.0) / (T)c->p.dens);  // mod.rs:80-84
+    launch_fft_rows<T>(b, c->fx, ny, 0, LOAD_REAL_PADDED, STORE_COMPLEX, 1.0, c->st);
+    launch_fft_cols<T>(b, c->fy, nx, 0, 1.0, c->st);
+  }
+  launch_spectral_update<T>(c->spec, geo<T>(c), c->p.dens, c->st);  // mod.rs:105-278
+  {  // mod.rs:280-307: 6 inverse transforms, spectral->spatial + ghosts fused
+    FftBatch b{};
+    const int ids[6] = {SP_BX, SP_BY, SP_BZ, SP_EX, SP_EY, SP_EZ};
+    b.count = 6;
+    for (int i = 0; i < 6; ++i) {
+      b.spec[i] = c->spec[ids[i]];
+      b.grid[i] = c->grid[ids[i]];
+      b.scale[i] = 1.0;
+    }
+    const T norm = T(1) / (T)((int64_t)nx * ny);  // fft_2d.rs:170
+    launch_fft_cols<T>(b, c->fy, nx, 1, 1.0, c->st);
+    launch_fft_rows<T>(b, c->fx, ny, 1, LOAD_COMPLEX, STORE_REAL_PADDED, (double)norm, c->st);
+  }
+}
+
+// One iteration of the loop body (src/lib.rs:155-199), with the push left
+// pending (see the file comment).
+template <class T> void one_step(sp_ctx* c) {
+  const Geo<T> g = geo<T>(c);
+  const GridSet gs = gset(c);
+  {
+    PhaseScope ps(c, 0);
+    zero_currents<T>(c);
+    {
+      PhaseScope pp(c, 3);
+      for (auto& s : c->species) {
+        if (c->push_pending) launch_push_move_deposit<T>(sdev<T>(c, s), g, gs, c->flags_d, c->st);
+        else launch_move_deposit<T>(sdev<T>(c, s), g, gs, c->flags_d, c->st);
+      }
+    }
+    c->push_pending = false;
+    fold<T>(c);
+  }
+  {
+    PhaseScope ps(c, 1);
+    fields_update<T>(c);
+  }
+  c->push_pending = true;
+  c->steps += 1;
+}
+
+const char* flag_text(int f) {
+  if (f & F_LEFT) return "assertion failed: min ix > delta (src/prtls.rs:163-170): a particle reached the left edge";
+  if (f & F_WALL) return "assertion failed: max ix <= wall_loc + 1 (src/prtls.rs:155-162)";
+  if (f & F_CELL) return "assertion failed: ix > 0 && iy > 0 && ijp1 + ix + 1 < len (src/prtls.rs:100-101, src/lib.rs:324-353)";
+  return "non-finite particle momentum after the Boris push";
+}
+
+// Wait for the stream; report and clear deferred invariant flags.
+int sync_check(sp_ctx* c) {
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  harvest_timers(c);
+  HIPCHK(c, hipMemcpy(c->flags_h, c->flags_d, sizeof(int), hipMemcpyDeviceToHost));
+  if (*c->flags_h) {
+    const int f = *c->flags_h;
+    *c->flags_h = 0;
+    HIPCHK(c, hipMemcpy(c->flags_d, c->flags_h, sizeof(int), hipMemcpyHostToDevice));
+    return fail(c, SP_EINVARIANT, flag_text(f));
+  }
+  return SP_OK;
+}
+
+int flush(sp_ctx* c) {
+  if (c->f64) flush_push<double>(c);
+  else flush_push<float>(c);
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int valid_species(sp_ctx* c, int id) {
+  if (!c) return SP_EINVAL;
+  if (id < 0 || id >= (int)c->species.size()) return fail(c, SP_EINVAL, "unknown species id");
+  return SP_OK;
+}
+
+int alloc_species(sp_ctx* c, SpeciesBuf& s, int64_t n) {
+  s.n = n;
+  const size_t nb = (size_t)(n > 0 ? n : 1);
+  HIPCHK(c, hipMalloc(&s.cell, nb * 4));
+  HIPCHK(c, hipMalloc(&s.tag, nb * 4));
+  void** f[5] = {&s.dx, &s.dy, &s.px, &s.py, &s.pz};
+  for (auto* p : f) HIPCHK(c, hipMalloc(p, nb * c->esz));
+  return SP_OK;
+}
+
+void free_species(SpeciesBuf& s) {
+  hipFree(s.cell);
+  hipFree(s.tag);
+  void* f[5] = {s.dx, s.dy, s.px, s.py, s.pz};
+  for (auto* p : f) hipFree(p);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sp_create(const sp_params* params, sp_ctx** out) {
+  if (!params || !out) return fail(nullptr, SP_EINVAL, "null argument");
+  const sp_params& p = *params;
+  // Config::new validation (src/lib.rs:75-113)
+  if (p.size_x <= 0 || p.size_y <= 0) return fail(nullptr, SP_EINVAL, "grid sizes must be positive");
+  if (p.size_x % 2 != 0 || p.size_y % 2 != 0) {
+    const char* w = (p.size_x % 2 != 0 && p.size_y % 2 != 0) ? "x & y" : (p.size_x % 2 != 0 ? "x" : "y");
+    return fail(nullptr, SP_EINVAL, std::string("Number of cells in ") + w + " direction must be even");
+  }
+  if (p.delta < 0 || p.delta * 2 >= p.size_x)
+    return fail(nullptr, SP_EINVAL, "Delta must be less than 1/2 the size of size_x");
+  if (!(p.gamma_inj > 1.0)) return fail(nullptr, SP_EINVAL, "Lorentz factor must be greater than 1");
+  if (p.dens < 1 || p.n_pass < 0) return fail(nullptr, SP_EINVAL, "dens must be >= 1 and n_pass >= 0");
+  if (!(p.dt > 0) || !(p.c > 0)) return fail(nullptr, SP_EINVAL, "dt and c must be positive");
+  if (p.precision != SP_F32 && p.precision != SP_F64) return fail(nullptr, SP_EINVAL, "unknown precision");
+  if (p.left_edge != SP_LEFT_ASSERT && p.left_edge != SP_LEFT_REFLECT)
+    return fail(nullptr, SP_EINVAL, "unknown left_edge rule");
+  // this build's limits: packed cells, LDS-resident 1D transforms
+  if (p.size_x + 2 > 65535 || p.size_y + 2 > 32767)
+    return fail(nullptr, SP_EINVAL, "grid too large for the packed (iy << 16 | ix) cell index");
+  const int maxn = p.precision == SP_F64 ? fft_max_len<double>() : fft_max_len<float>();
+  FftDim fx{}, fy{};
+  if (!plan_dim(fx, p.size_x) || !plan_dim(fy, p.size_y) || p.size_x > maxn || p.size_y > maxn)
+    return fail(nullptr, SP_EINVAL,
+                "size_x and size_y must be 2^a 3^b and at most " + std::to_string(maxn) + " for the GPU FFT");
+
+  sp_ctx* c = new sp_ctx();
+  c->p = p;
+  c->f64 = p.precision == SP_F64;
+  c->esz = c->f64 ? 8 : 4;
+  c->fx = fx;
+  c->fy = fy;
+  auto bail = [&](int rc) {
+    g_create_error = c->err;
+    sp_destroy(c);
+    return rc;
+  };
+#define CK(call)                                                                     \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      c->err = std::string(#call) + ": " + hipGetErrorString(e_);                    \
+      return bail(SP_EHIP);                                                          \
+    }                                                                                \
+  } while (0)
+  CK(hipGetDevice(&c->device));
+  CK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  const size_t gb = c->ngrid_elems() * c->esz, sb = c->nspec_elems() * 2 * c->esz;
+  for (auto& g : c->grid) CK(hipMalloc(&g, gb));
+  for (auto& g : c->work) CK(hipMalloc(&g, gb));
+  for (auto& s : c->spec) CK(hipMalloc(&s, sb));
+  for (auto& g : c->grid) CK(hipMemsetAsync(g, 0, gb, c->st));
+  for (auto& g : c->work) CK(hipMemsetAsync(g, 0, gb, c->st));
+  for (auto& s : c->spec) CK(hipMemsetAsync(s, 0, sb, c->st));
+  CK(hipMalloc(&c->fx.tw, (size_t)fx.n * 2 * c->esz));
+  CK(hipMalloc(&c->fy.tw, (size_t)fy.n * 2 * c->esz));
+  if (c->f64) {
+    fft_make_twiddles<double>(c->fx, c->st);
+    fft_make_twiddles<double>(c->fy, c->st);
+  } else {
+    fft_make_twiddles<float>(c->fx, c->st);
+    fft_make_twiddles<float>(c->fy, c->st);
+  }
+  CK(hipMalloc(&c->flags_d, sizeof(int)));
+  CK(hipHostMalloc((void**)&c->flags_h, sizeof(int), hipHostMallocDefault));
+  CK(hipMemsetAsync(c->flags_d, 0, sizeof(int), c->st));
+  CK(hipStreamSynchronize(c->st));
+  *c->flags_h = 0;
+#undef CK
+  *out = c;
+  return SP_OK;
+}
+
+int sp_destroy(sp_ctx* c) {
+  if (!c) return SP_OK;
+  if (c->st) hipStreamSynchronize(c->st);
+  for (auto& s : c->species) free_species(s);
+  for (auto g : c->grid) hipFree(g);
+  for (auto g : c->work) hipFree(g);
+  for (auto s : c->spec) hipFree(s);
+  hipFree(c->fx.tw);
+  hipFree(c->fy.tw);
+  hipFree(c->flags_d);
+  if (c->flags_h) hipHostFree(c->flags_h);
+  for (auto& tp : c->open_pairs) { hipEventDestroy(tp.a); hipEventDestroy(tp.b); }
+  for (auto e : c->free_events) hipEventDestroy(e);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+  return SP_OK;
+}
+
+const char* sp_last_error(const sp_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int sp_species_add(sp_ctx* c, double charge, double mass, int64_t n, const int32_t* ix, const int32_t* iy,
+                   const void* dx, const void* dy, const void* px, const void* py, const void* pz, int* id) {
+  if (!c) return SP_EINVAL;
+  if (n < 0 || (n > 0 && (!ix || !iy || !dx || !dy || !px || !py || !pz)))
+    return fail(c, SP_EINVAL, "bad species arrays");
+  if (!(mass != 0.0)) return fail(c, SP_EINVAL, "mass must be non-zero");
+  if (n > INT32_MAX) return fail(c, SP_EINVAL, "species too large for int32 tags");
+  int rc = flush(c);
+  if (rc) return rc;
+  std::vector<int32_t> cell(n), tag(n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (ix[i] < 0 || ix[i] > c->p.size_x + 1 || iy[i] < 0 || iy[i] > c->p.size_y + 1)
+      return fail(c, SP_EINVAL, "particle cell outside the padded grid");
+    cell[i] = make_cell(ix[i], iy[i]);
+    tag[i] = (int32_t)i;
+  }
+  SpeciesBuf s;
+  s.charge = charge;
+  s.mass = mass;
+  rc = alloc_species(c, s, n);
+  if (rc) { free_species(s); return rc; }
+  if (n > 0) {
+    HIPCHK(c, hipMemcpy(s.cell, cell.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(s.tag, tag.data(), n * 4, hipMemcpyHostToDevice));
+    const void* src[5] = {dx, dy, px, py, pz};
+    void* dst[5] = {s.dx, s.dy, s.px, s.py, s.pz};
+    for (int k = 0; k < 5; ++k) HIPCHK(c, hipMemcpy(dst[k], src[k], n * c->esz, hipMemcpyHostToDevice));
+  }
+  c->species.push_back(s);
+  if (id) *id = (int)c->species.size() - 1;
+  return SP_OK;
+}
+
+int sp_species_init(sp_ctx* c, double charge, double mass, double vth, uint64_t seed, int* id) {
+  if (!c) return SP_EINVAL;
+  if (!(mass != 0.0) || !(vth >= 0.0)) return fail(c, SP_EINVAL, "bad mass or vth");
+  int rc = flush(c);
+  if (rc) return rc;
+  const int64_t n = (int64_t)c->p.dens * (c->p.size_x - 2 * c->p.delta) * c->p.size_y;  // src/lib.rs:258-260
+  if (n > INT32_MAX) return fail(c, SP_EINVAL, "species too large for int32 tags");
+  SpeciesBuf s;
+  s.charge = charge;
+  s.mass = mass;
+  rc = alloc_species(c, s, n);
+  if (rc) { free_species(s); return rc; }
+  const int sid = (int)c->species.size();
+  if (c->f64) launch_init_species<double>(sdev<double>(c, s), geo<double>(c), vth, c->p.gamma_inj, seed, sid, c->st);
+  else launch_init_species<float>(sdev<float>(c, s), geo<float>(c), vth, c->p.gamma_inj, seed, sid, c->st);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->species.push_back(s);
+  if (id) *id = sid;
+  return SP_OK;
+}
+
+int sp_species_count(const sp_ctx* c, int id, int64_t* n) {
+  if (!c || !n || id < 0 || id >= (int)c->species.size()) return SP_EINVAL;
+  *n = c->species[id].n;
+  return SP_OK;
+}
+
+int sp_get_species(sp_ctx* c, int id, int64_t n, int32_t* ix, int32_t* iy, void* dx, void* dy, void* px, void* py,
+                   void* pz, void* psa) {
+  int rc = valid_species(c, id);
+  if (rc) return rc;
+  const SpeciesBuf& s = c->species[id];
+  if (n != s.n) return fail(c, SP_EINVAL, "count does not match the species size");
+  rc = flush(c);
+  if (rc) return rc;
+  rc = sync_check(c);
+  if (rc) return rc;
+  if (n == 0) return SP_OK;
+  std::vector<int32_t> cell(n), tag(n);
+  HIPCHK(c, hipMemcpy(cell.data(), s.cell, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(tag.data(), s.tag, n * 4, hipMemcpyDeviceToHost));
+  std::vector<unsigned char> buf(n * c->esz);
+  void* outs[5] = {dx, dy, px, py, pz};
+  void* srcs[5] = {s.dx, s.dy, s.px, s.py, s.pz};
+  const size_t e = c->esz;
+  for (int k = 0; k < 5; ++k) {
+    HIPCHK(c, hipMemcpy(buf.data(), srcs[k], n * e, hipMemcpyDeviceToHost));
+    if (!outs[k]) continue;
+    unsigned char* o = (unsigned char*)outs[k];
+    for (int64_t i = 0; i < n; ++i) std::memcpy(o + (size_t)tag[i] * e, buf.data() + (size_t)i * e, e);
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    if (ix) ix[tag[i]] = cell_ix(cell[i]);
+    if (iy) iy[tag[i]] = cell_iy(cell[i]);
+  }
+  if (psa) {  // src/prtls.rs:440 in Float
+    for (int64_t i = 0; i < n; ++i) {
+      if (c->f64) {
+        const double* X = (const double*)px; const double* Y = (const double*)py; const double* Z = (const double*)pz;
+        const double cc = c->p.c, csq = 1. / (cc * cc);
+        ((double*)psa)[i] = std::sqrt(1.0 + (X[i] * X[i] + Y[i] * Y[i] + Z[i] * Z[i]) * csq);
+      } else {
+        const float* X = (const float*)px; const float* Y = (const float*)py; const float* Z = (const float*)pz;
+        const float cc = (float)c->p.c, csq = 1.f / (cc * cc);
+        ((float*)psa)[i] = std::sqrt(1.0f + (X[i] * X[i] + Y[i] * Y[i] + Z[i] * Z[i]) * csq);
+      }
+    }
+  }
+  return SP_OK;
+}
+
+static int grid_io(sp_ctx* c, int grid, void* host, size_t count, bool to_dev, bool spectrum) {
+  if (!c || !host) return SP_EINVAL;
+  const int nmax = spectrum ? 14 : 11;
+  if (grid < 0 || grid >= nmax) return fail(c, SP_EINVAL, "unknown grid id");
+  const size_t need = spectrum ? 2 * c->nspec_elems() : c->ngrid_elems();
+  if (count != need) return fail(c, SP_EINVAL, "count must be " + std::to_string(need));
+  int rc = flush(c);
+  if (rc) return rc;
+  rc = sync_check(c);
+  if (rc) return rc;
+  void* dev = spectrum ? c->spec[grid] : c->grid[grid];
+  if (to_dev) HIPCHK(c, hipMemcpy(dev, host, need * c->esz, hipMemcpyHostToDevice));
+  else HIPCHK(c, hipMemcpy(host, dev, need * c->esz, hipMemcpyDeviceToHost));
+  return SP_OK;
+}
+
+int sp_get_grid(sp_ctx* c, int g, void* out, size_t n) { return grid_io(c, g, out, n, false, false); }
+int sp_set_grid(sp_ctx* c, int g, const void* in, size_t n) { return grid_io(c, g, (void*)in, n, true, false); }
+int sp_get_spectrum(sp_ctx* c, int g, void* out, size_t n) { return grid_io(c, g, out, n, false, true); }
+int sp_set_spectrum(sp_ctx* c, int g, const void* in, size_t n) { return grid_io(c, g, (void*)in, n, true, true); }
+
+#define ENTRY_PRELUDE(c)        \
+  if (!(c)) return SP_EINVAL;   \
+  {                             \
+    int rc_ = flush(c);         \
+    if (rc_) return rc_;        \
+  }
+
+int sp_zero_currents(sp_ctx* c) {
+  ENTRY_PRELUDE(c);
+  if (c->f64) zero_currents<double>(c);
+  else zero_currents<float>(c);
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_move_and_deposit(sp_ctx* c, int id) {
+  ENTRY_PRELUDE(c);
+  int rc = valid_species(c, id);
+  if (rc) return rc;
+  PhaseScope ps(c, 0);
+  if (c->f64) launch_move_deposit<double>(sdev<double>(c, c->species[id]), geo<double>(c), gset(c), c->flags_d, c->st);
+  else launch_move_deposit<float>(sdev<float>(c, c->species[id]), geo<float>(c), gset(c), c->flags_d, c->st);
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_deposit_ghosts(sp_ctx* c) {
+  ENTRY_PRELUDE(c);
+  if (c->f64) fold<double>(c);
+  else fold<float>(c);
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_fields_update(sp_ctx* c) {
+  ENTRY_PRELUDE(c);
+  PhaseScope ps(c, 1);
+  if (c->f64) fields_update<double>(c);
+  else fields_update<float>(c);
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_boris_push(sp_ctx* c, int id) {
+  ENTRY_PRELUDE(c);
+  int rc = valid_species(c, id);
+  if (rc) return rc;
+  PhaseScope ps(c, 2);
+  if (c->f64) launch_push<double>(sdev<double>(c, c->species[id]), geo<double>(c), gset(c), c->flags_d, c->st);
+  else launch_push<float>(sdev<float>(c, c->species[id]), geo<float>(c), gset(c), c->flags_d, c->st);
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_filter(sp_ctx* c, int grid, int passes) {
+  ENTRY_PRELUDE(c);
+  if (grid < 0 || grid >= 11 || passes < 0) return fail(c, SP_EINVAL, "bad grid id or pass count");
+  if (c->f64) {
+    double* g[1] = {(double*)c->grid[grid]};
+    double* w[1] = {(double*)c->work[0]};
+    launch_binomial<double>(g, w, 1, c->p.size_x, c->p.size_y, passes, c->st);
+  } else {
+    float* g[1] = {(float*)c->grid[grid]};
+    float* w[1] = {(float*)c->work[0]};
+    launch_binomial<float>(g, w, 1, c->p.size_x, c->p.size_y, passes, c->st);
+  }
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_fft2d(sp_ctx* c, int spec, int inverse) {
+  ENTRY_PRELUDE(c);
+  if (spec < 0 || spec >= 14) return fail(c, SP_EINVAL, "unknown spectrum id");
+  FftBatch b{};
+  b.count = 1;
+  b.spec[0] = c->spec[spec];
+  b.grid[0] = nullptr;
+  b.scale[0] = 1.0;
+  const int nx = c->p.size_x, ny = c->p.size_y;
+  if (c->f64) {
+    const double norm = inverse ? 1.0 / (double)((int64_t)nx * ny) : 1.0;
+    launch_fft_rows<double>(b, c->fx, ny, inverse, LOAD_COMPLEX, STORE_COMPLEX, 1.0, c->st);
+    launch_fft_cols<double>(b, c->fy, nx, inverse, norm, c->st);
+  } else {
+    const float norm = inverse ? 1.0f / (float)((int64_t)nx * ny) : 1.0f;
+    launch_fft_rows<float>(b, c->fx, ny, inverse, LOAD_COMPLEX, STORE_COMPLEX, 1.0, c->st);
+    launch_fft_cols<float>(b, c->fy, nx, inverse, (double)norm, c->st);
+  }
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_spectral_update(sp_ctx* c) {
+  ENTRY_PRELUDE(c);
+  if (c->f64) launch_spectral_update<double>(c->spec, geo<double>(c), c->p.dens, c->st);
+  else launch_spectral_update<float>(c->spec, geo<float>(c), c->p.dens, c->st);
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_prestep(sp_ctx* c) {
+  ENTRY_PRELUDE(c);
+  PhaseScope ps(c, 0);
+  for (auto& s : c->species) {
+    if (c->f64) launch_move_deposit<double>(sdev<double>(c, s), geo<double>(c), gset(c), c->flags_d, c->st);
+    else launch_move_deposit<float>(sdev<float>(c, s), geo<float>(c), gset(c), c->flags_d, c->st);
+  }
+  if (c->f64) fold<double>(c);
+  else fold<float>(c);
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_step(sp_ctx* c, int n) {
+  if (!c) return SP_EINVAL;
+  if (n < 0) return fail(c, SP_EINVAL, "negative step count");
+  for (int i = 0; i < n; ++i) {
+    if (c->f64) one_step<double>(c);
+    else one_step<float>(c);
+  }
+  HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_sync(sp_ctx* c) {
+  if (!c) return SP_EINVAL;
+  int rc = flush(c);
+  if (rc) return rc;
+  return sync_check(c);
+}
+
+int sp_set_timing(sp_ctx* c, int on) {
+  if (!c) return SP_EINVAL;
+  c->timing = on != 0;
+  return SP_OK;
+}
+
+int sp_get_timers(sp_ctx* c, double* ms, int reset) {
+  if (!c || !ms) return SP_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  harvest_timers(c);
+  for (int i = 0; i < 4; ++i) ms[i] = c->t_ms[i];
+  if (reset)
+    for (double& t : c->t_ms) t = 0;
+  return SP_OK;
+}
+
+int sp_get_diag(sp_ctx* c, sp_diag* out) {
+  if (!c || !out) return SP_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  out->steps = c->steps;
+  return SP_OK;
+}
+
+}  // extern "C"

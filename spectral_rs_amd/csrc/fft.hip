@@ -1,0 +1,287 @@
+// fft.hip -- hand-written 2D complex FFT for gfx950, standing in for the
+// rustfft row passes of Fft2D (src/flds/fft_2d.rs:91-172).
+//
+// Each 1D transform runs entirely in LDS as a Stockham autosort sequence of
+// radix-8/4/2/3 passes (in place: every thread loads its butterflies into
+// registers, the block synchronises, then writes).  Rows (x, contiguous) are
+// transformed in blocks of whole rows; columns (y, stride Nx) in blocks of C
+// adjacent columns so each global access touches C consecutive elements.
+// The spatial->spectral copy (real interior of the padded grid, times an
+// optional scale: 1/dens for rho, src/flds/mod.rs:79-84) is fused into the
+// forward row load, and the 1/(Nx Ny) normalisation, real-part extraction and
+// periodic ghost refresh of copy_to_spatial (src/flds/field.rs:109-139) into
+// the inverse row store.  Twiddles exp(-2 pi i k/N) are generated in FP64 on
+// the device and rounded to Float once.
+#pragma clang fp contract(off)
+
+#include "sp_internal.h"
+
+namespace sp {
+
+template <class T> struct V2 {
+  T x, y;
+};
+template <class T> __device__ __forceinline__ V2<T> operator+(V2<T> a, V2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <class T> __device__ __forceinline__ V2<T> operator-(V2<T> a, V2<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <class T> __device__ __forceinline__ V2<T> cm(V2<T> a, V2<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by -i (forward) or +i (inverse)
+template <class T, bool INV> __device__ __forceinline__ V2<T> rot90(V2<T> a) {
+  return INV ? V2<T>{-a.y, a.x} : V2<T>{a.y, -a.x};
+}
+
+template <class T, bool INV> __device__ __forceinline__ void dft2(V2<T>* v) {
+  const V2<T> a = v[0], b = v[1];
+  v[0] = a + b;
+  v[1] = a - b;
+}
+template <class T, bool INV> __device__ __forceinline__ void dft3(V2<T>* v) {
+  const T s = T(0.86602540378443864676372317075293618);  // sin(pi/3)
+  const V2<T> a = v[0], b = v[1], c = v[2];
+  const V2<T> t1 = b + c;
+  const V2<T> t2 = {a.x - T(0.5) * t1.x, a.y - T(0.5) * t1.y};
+  const V2<T> d = b - c;
+  const V2<T> t3 = rot90<T, INV>(V2<T>{s * d.x, s * d.y});
+  v[0] = a + t1;
+  v[1] = t2 + t3;
+  v[2] = t2 - t3;
+}
+template <class T, bool INV> __device__ __forceinline__ void dft4(V2<T>* v) {
+  const V2<T> t0 = v[0] + v[2], t1 = v[0] - v[2], t2 = v[1] + v[3];
+  const V2<T> t3 = rot90<T, INV>(v[1] - v[3]);
+  v[0] = t0 + t2;
+  v[2] = t0 - t2;
+  v[1] = t1 + t3;
+  v[3] = t1 - t3;
+}
+template <class T, bool INV> __device__ __forceinline__ void dft8(V2<T>* v) {
+  V2<T> e[4] = {v[0], v[2], v[4], v[6]};
+  V2<T> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<T, INV>(e);
+  dft4<T, INV>(o);
+  const T h = T(0.70710678118654752440084436210484904);
+  // o[k] *= w8^k, w8 = exp(-+ 2 pi i / 8)
+  const V2<T> o1 = INV ? V2<T>{h * (o[1].x - o[1].y), h * (o[1].x + o[1].y)}
+                       : V2<T>{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};
+  const V2<T> o2 = rot90<T, INV>(o[2]);
+  const V2<T> o3 = INV ? V2<T>{-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)}
+                       : V2<T>{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};
+  v[0] = e[0] + o[0];
+  v[4] = e[0] - o[0];
+  v[1] = e[1] + o1;
+  v[5] = e[1] - o1;
+  v[2] = e[2] + o2;
+  v[6] = e[2] - o2;
+  v[3] = e[3] + o3;
+  v[7] = e[3] - o3;
+}
+template <class T, int R, bool INV> __device__ __forceinline__ void dftR(V2<T>* v) {
+  if constexpr (R == 2) dft2<T, INV>(v);
+  else if constexpr (R == 3) dft3<T, INV>(v);
+  else if constexpr (R == 4) dft4<T, INV>(v);
+  else dft8<T, INV>(v);
+}
+
+constexpr int EPT = 16;  // elements per thread per pass
+
+// One Stockham pass of radix R over `seqs` sequences of length N held in LDS.
+// Sequence s, element n lives at lds[s * sstride + n * estride].  SEQ_FAST
+// picks which index varies fastest across threads (columns: the sequence,
+// rows: the butterfly) so that LDS accesses stay unit-stride.
+template <class T, int R, bool INV, bool SEQ_FAST>
+__device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, int seqs, int sstride, int estride,
+                                              const V2<T>* __restrict__ tw, int nthreads) {
+  constexpr int BPT = (EPT + R - 1) / R;
+  const int nb = N / R;
+  const int total = nb * seqs;
+  const int twstep = N / (Ns * R);
+  V2<T> v[BPT][R];
+  int dst[BPT];
+#pragma unroll
+  for (int m = 0; m < BPT; ++m) {
+    const int b = threadIdx.x + m * nthreads;
+    dst[m] = -1;
+    if (b < total) {
+      const int s = SEQ_FAST ? b % seqs : b / nb;
+      const int j = SEQ_FAST ? b / seqs : b % nb;
+      const V2<T>* src = lds + s * sstride;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[m][r] = src[(j + r * nb) * estride];
+      const int k = j % Ns;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          V2<T> w = tw[r * k * twstep];
+          if (INV) w.y = -w.y;
+          v[m][r] = cm(v[m][r], w);
+        }
+      }
+      dftR<T, R, INV>(v[m]);
+      dst[m] = s * sstride + ((j / Ns) * Ns * R + k) * estride;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < BPT; ++m) {
+    if (dst[m] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[dst[m] + r * Ns * estride] = v[m][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T, bool INV, bool SEQ_FAST>
+__device__ void fft_lds(V2<T>* lds, const FftDim& d, int seqs, int sstride, int estride, int nthreads) {
+  const V2<T>* tw = (const V2<T>*)d.tw;
+  int Ns = 1;
+  for (int p = 0; p < d.nrad; ++p) {
+    const int R = (int)((d.radix_code >> (4 * p)) & 15u);
+    if (R == 8) stockham_pass<T, 8, INV, SEQ_FAST>(lds, d.n, Ns, seqs, sstride, estride, tw, nthreads);
+    else if (R == 4) stockham_pass<T, 4, INV, SEQ_FAST>(lds, d.n, Ns, seqs, sstride, estride, tw, nthreads);
+    else if (R == 2) stockham_pass<T, 2, INV, SEQ_FAST>(lds, d.n, Ns, seqs, sstride, estride, tw, nthreads);
+    else stockham_pass<T, 3, INV, SEQ_FAST>(lds, d.n, Ns, seqs, sstride, estride, tw, nthreads);
+    Ns *= R;
+  }
+}
+
+// Row transforms: block (bx, field) handles rows [bx*rpb, bx*rpb + rpb).
+template <class T, bool INV>
+__global__ void __launch_bounds__(512) k_fft_rows(FftBatch b, FftDim d, int ny, int rpb, int load, int store, T scale_store) {
+  extern __shared__ unsigned char smem[];
+  V2<T>* lds = (V2<T>*)smem;
+  const int f = blockIdx.y;
+  const int nx = d.n, P = nx + 2;
+  const int r0 = blockIdx.x * rpb;
+  const int rows = min(rpb, ny - r0);
+  const int E = rows * nx;
+  const int nt = blockDim.x;
+  V2<T>* spec = (V2<T>*)b.spec[f];
+  T* grid = (T*)b.grid[f];
+  if (load == LOAD_REAL_PADDED) {
+    const T sc = (T)b.scale[f];
+    const bool scaled = b.scale[f] != 1.0;
+    for (int i = threadIdx.x; i < E; i += nt) {
+      const int s = i / nx, n = i - s * nx;
+      T v = grid[(r0 + s + 1) * P + n + 1];
+      if (scaled) v = v * sc;
+      lds[i] = V2<T>{v, T(0.0)};
+    }
+  } else {
+    for (int i = threadIdx.x; i < E; i += nt) lds[i] = spec[(int64_t)r0 * nx + i];
+  }
+  __syncthreads();
+  fft_lds<T, INV, false>(lds, d, rows, nx, 1, nt);
+  if (store == STORE_REAL_PADDED) {
+    // copy_to_spatial + update_ghosts: interior, the two ghost columns of this
+    // row, and the ghost row that mirrors it (row Ny -> row 0, row 1 -> Ny+1).
+    for (int i = threadIdx.x; i < rows * P; i += nt) {
+      const int s = i / P, c = i - s * P;
+      const int n = (c == 0) ? nx - 1 : (c == P - 1 ? 0 : c - 1);
+      const T v = lds[s * nx + n].x * scale_store;
+      const int r = r0 + s + 1;
+      grid[r * P + c] = v;
+      if (r == ny) grid[c] = v;
+      if (r == 1) grid[(ny + 1) * P + c] = v;
+    }
+  } else {
+    const bool scaled = scale_store != T(1);
+    for (int i = threadIdx.x; i < E; i += nt) {
+      V2<T> v = lds[i];
+      if (scaled) v = V2<T>{v.x * scale_store, v.y * scale_store};
+      spec[(int64_t)r0 * nx + i] = v;
+    }
+  }
+}
+
+// Column transforms: block (bx, field) handles columns [bx*C, bx*C + C).
+template <class T, bool INV>
+__global__ void __launch_bounds__(512) k_fft_cols(FftBatch b, FftDim d, int nx, int C, T scale_store) {
+  extern __shared__ unsigned char smem[];
+  V2<T>* lds = (V2<T>*)smem;
+  const int f = blockIdx.y;
+  const int ny = d.n;
+  const int c0 = blockIdx.x * C;
+  const int cols = min(C, nx - c0);
+  const int E = cols * ny;
+  const int nt = blockDim.x;
+  V2<T>* spec = (V2<T>*)b.spec[f];
+  for (int i = threadIdx.x; i < E; i += nt) {
+    const int n = i / cols, c = i - n * cols;
+    lds[i] = spec[(int64_t)n * nx + c0 + c];
+  }
+  __syncthreads();
+  fft_lds<T, INV, true>(lds, d, cols, 1, cols, nt);
+  const bool scaled = scale_store != T(1);
+  for (int i = threadIdx.x; i < E; i += nt) {
+    const int n = i / cols, c = i - n * cols;
+    V2<T> v = lds[i];
+    if (scaled) v = V2<T>{v.x * scale_store, v.y * scale_store};
+    spec[(int64_t)n * nx + c0 + c] = v;
+  }
+}
+
+template <class T> __global__ void k_twiddles(V2<T>* tw, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double s, c;
+  sincospi(-2.0 * (double)k / (double)n, &s, &c);  // FP64 generation
+  tw[k] = V2<T>{(T)c, (T)s};
+}
+
+template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st) {
+  hipLaunchKernelGGL(k_twiddles<T>, dim3((d.n + 255) / 256), dim3(256), 0, st, (V2<T>*)d.tw, d.n);
+}
+
+static int round_threads(int e) {
+  int t = (e + EPT - 1) / EPT;
+  t = ((t + 63) / 64) * 64;
+  return t < 64 ? 64 : (t > 512 ? 512 : t);
+}
+template <class T> static int max_elems() { return fft_max_len<T>(); }
+
+template <class T>
+void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, int load, int store,
+                     double store_scale, hipStream_t st) {
+  int rpb = max_elems<T>() / dx.n;
+  if (rpb < 1) rpb = 1;
+  if (rpb > 16) rpb = 16;
+  if (rpb > ny) rpb = ny;
+  const int E = rpb * dx.n;
+  const int nt = round_threads(E);
+  const size_t shm = (size_t)E * sizeof(V2<T>);
+  dim3 grid((ny + rpb - 1) / rpb, b.count);
+  const T scale = (T)store_scale;
+  if (inverse)
+    hipLaunchKernelGGL((k_fft_rows<T, true>), grid, dim3(nt), shm, st, b, dx, ny, rpb, load, store, scale);
+  else
+    hipLaunchKernelGGL((k_fft_rows<T, false>), grid, dim3(nt), shm, st, b, dx, ny, rpb, load, store, scale);
+}
+
+template <class T>
+void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, double store_scale, hipStream_t st) {
+  int C = max_elems<T>() / dy.n;
+  if (C < 1) C = 1;
+  if (C > 16) C = 16;
+  if (C > nx) C = nx;
+  const int E = C * dy.n;
+  const int nt = round_threads(E);
+  const size_t shm = (size_t)E * sizeof(V2<T>);
+  dim3 grid((nx + C - 1) / C, b.count);
+  const T scale = (T)store_scale;
+  if (inverse)
+    hipLaunchKernelGGL((k_fft_cols<T, true>), grid, dim3(nt), shm, st, b, dy, nx, C, scale);
+  else
+    hipLaunchKernelGGL((k_fft_cols<T, false>), grid, dim3(nt), shm, st, b, dy, nx, C, scale);
+}
+
+#define INST(T)                                                                                 \
+  template void fft_make_twiddles<T>(FftDim&, hipStream_t);                                     \
+  template void launch_fft_rows<T>(const FftBatch&, const FftDim&, int, int, int, int, double, hipStream_t); \
+  template void launch_fft_cols<T>(const FftBatch&, const FftDim&, int, int, double, hipStream_t);
+INST(float)
+INST(double)
+#undef INST
+
+}  // namespace sp

@@ -1,0 +1,91 @@
+// sp_internal.h -- device-side types shared by the HIP translation units.
+//
+// Layout in HBM (one context = one simulation on one GPU):
+//   * grids: padded (Ny+2) x (Nx+2) row-major arrays of Float, one per field
+//     (Ex Ey Ez Bx By Bz Jx Jy Jz rho dens) plus a filter work grid;
+//   * spectra: Ny x Nx interleaved complex, one per field plus B_wrk x3;
+//   * particles: per species SoA of `cell` (int32, iy << 16 | ix), dx, dy,
+//     px, py, pz (Float) and tag (int32, index in the order the species was
+//     added, carried through cell sorts).  gamma is never stored: it is
+//     sqrt(1 + |p|^2 csqinv), the exact expression the reference stores in psa.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/spectral_rs.h"
+
+namespace sp {
+
+enum Flags : int {
+  F_CELL = 1,   // ix or iy outside the interior (src/prtls.rs:100-101, src/lib.rs:324-325)
+  F_WALL = 2,   // max ix > wall + 1 (src/prtls.rs:155-162)
+  F_LEFT = 4,   // min ix <= delta (src/prtls.rs:163-170)
+  F_FINITE = 8  // non-finite momentum (no reference assert; reported as invariant)
+};
+
+template <class T> struct Geo {
+  int nx, ny, pitch;       // pitch = nx + 2
+  int delta, wall, lwall;  // wall = nx + 1 - delta (src/prtls.rs:153), lwall = delta + 1
+  int periodic_x, checked, left_edge;
+  T dt, c, csqinv;         // csqinv = 1 / (c * c) in Float (src/prtls.rs:265)
+  T half_dt;               // 0.5 * dt in Float (src/prtls.rs:103)
+};
+
+template <class T> struct SpeciesDev {
+  int64_t n;
+  int32_t* cell;
+  T *dx, *dy, *px, *py, *pz;
+  int32_t* tag;
+  T charge, alpha, beta;
+};
+
+struct GridSet {  // device pointers of one precision, indexed by sp_grid
+  void* g[11];
+};
+
+__host__ __device__ inline int cell_ix(int32_t c) { return c & 0xFFFF; }
+__host__ __device__ inline int cell_iy(int32_t c) { return c >> 16; }
+__host__ __device__ inline int32_t make_cell(int ix, int iy) { return (iy << 16) | (ix & 0xFFFF); }
+
+// ---- launchers (all asynchronous on `st`) ----
+template <class T> void fill(T* p, T v, int64_t n, hipStream_t st);
+template <class T> void launch_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags,
+                                            hipStream_t st);
+template <class T> void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags,
+                                   hipStream_t st);
+template <class T> void launch_push_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs,
+                                                 int* flags, hipStream_t st);
+template <class T> void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, double gamma_inj,
+                                            uint64_t seed, int sid, hipStream_t st);
+template <class T> void launch_fold_ghosts(T* const* grids, int ngrids, int nx, int ny, hipStream_t st);
+template <class T> void launch_update_ghosts(T* const* grids, int ngrids, int nx, int ny, hipStream_t st);
+template <class T> void launch_binomial(T* const* grids, T* const* work, int ngrids, int nx, int ny, int passes,
+                                        hipStream_t st);
+template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, hipStream_t st);
+
+// FFT plan for one dimension: radices and an FP64-generated twiddle table.
+struct FftDim {
+  int n;
+  int nrad;
+  uint64_t radix_code;  // radix of pass p in bits [4p, 4p+4)
+  void* tw;             // n complex Float: exp(-2 pi i k / n)
+};
+template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st);
+// Row (x) and column (y) passes over up to 8 spectra at once.
+enum FftLoad { LOAD_COMPLEX = 0, LOAD_REAL_PADDED = 1 };
+enum FftStore { STORE_COMPLEX = 0, STORE_REAL_PADDED = 1 };
+struct FftBatch {
+  int count;
+  void* spec[8];     // spectra (Ny x Nx complex)
+  void* grid[8];     // padded real grids (for the fused load / store)
+  double scale[8];   // multiplier applied at the fused real load
+};
+// `store_scale` multiplies every stored value (the 1/(Nx Ny) of an inverse).
+template <class T> void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, int load, int store,
+                                        double store_scale, hipStream_t st);
+template <class T> void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, double store_scale,
+                                        hipStream_t st);
+// Largest transform length one block holds in LDS.
+template <class T> constexpr int fft_max_len() { return (int)(65536 / (2 * sizeof(T))); }
+
+}  // namespace sp

@@ -1,0 +1,262 @@
+"""GPU parity: every entry point of the HIP library against the CPU oracle on
+identical inputs (marked gpu; run on an MI355X).
+
+Bars (stated per test):
+  * bit-exact: Boris push momenta and gamma, mover cells/offsets/momenta, ghost
+    fold, binomial filter, spectral Maxwell update, GPU initialiser positions;
+  * tolerance: deposits (float atomics sum in another order than the
+    reference's serial loop), FFTs (own algorithm vs the oracle's), and
+    everything downstream of them.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from spectral_rs_amd import capi
+from tests.helpers import TEST_SIM, ref_vectors, shock_species
+
+pytestmark = pytest.mark.gpu
+
+F32_EPS = np.finfo(np.float32).eps
+
+
+def pair(prec=4, **kw):
+    p = dict(TEST_SIM)
+    p.update(kw)
+    return capi.GpuSim(prec=prec, **p), O.RefSim(prec=prec, **p)
+
+
+def random_fields(ny, nx, ft, seed, amp=(0.05, 0.05, 0.05, 1.0, 1.0, 1.0)):
+    rng = np.random.default_rng(seed)
+    return {k: (rng.standard_normal((ny + 2, nx + 2)) * a).astype(ft)
+            for k, a in zip(["ex", "ey", "ez", "bx", "by", "bz"], amp)}
+
+
+def random_particles(n, nx, ny, delta, ft, seed, pscale=3.0):
+    rng = np.random.default_rng(seed)
+    ix = rng.integers(delta + 2, nx - delta - 1, n)
+    iy = rng.integers(1, ny + 1, n)
+    d = lambda: (rng.random(n) - 0.5).astype(ft)
+    p = lambda: (rng.standard_normal(n) * pscale).astype(ft)
+    return dict(ix=ix, iy=iy, dx=d(), dy=d(), px=p(), py=p(), pz=p())
+
+
+def both(g, r, fn, *a):
+    getattr(g, fn)(*a)
+    getattr(r, fn)(*a)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_boris_push_bit_exact(prec):
+    g, r = pair(prec, size_x=96, size_y=48, delta=4)
+    ft = g.ft
+    for k, v in random_fields(48, 96, ft, 1).items():
+        g.set_grid(k, v)
+        r.set_grid(k, v)
+    for q, m, sd in [(1.0, 16.0, 2), (-1.0, 1.0, 3)]:
+        sp = random_particles(20000, 96, 48, 4, ft, sd)
+        assert g.add_species(q, m, **sp) == r.add_species(q, m, **sp)
+    for sid in range(2):
+        both(g, r, "boris_push", sid)
+        a, b = g.species(sid), r.species(sid)
+        for k in ["px", "py", "pz", "psa"]:
+            assert np.array_equal(a[k], b[k]), (sid, k, np.max(np.abs(a[k] - b[k])))
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_move_and_deposit_cells_bit_exact_and_deposit_within_tolerance(prec):
+    nx, ny, delta = 64, 32, 4
+    g, r = pair(prec, size_x=nx, size_y=ny, delta=delta)
+    ft = g.ft
+    sp = random_particles(30000, nx, ny, delta, ft, 7, pscale=6.0)
+    g.add_species(-1.0, 1.0, **sp)
+    r.add_species(-1.0, 1.0, **sp)
+    both(g, r, "move_and_deposit", 0)
+    a, b = g.species(0), r.species(0)
+    for k in ["ix", "iy", "dx", "dy", "px", "py", "pz"]:
+        assert np.array_equal(a[k], b[k]), k
+    # per-cell bound: |J_gpu - J_ref| <= n_c * eps * sum|w q v|, with n_c the
+    # number of contributions; sum|w q v| is the deposit of |q|, |p| at the
+    # same mid-step positions (TSC weights are non-negative).
+    absr = O.RefSim(prec=prec, **{**TEST_SIM, "size_x": nx, "size_y": ny, "delta": delta})
+    mid = O.RefSim(prec=prec, **{**TEST_SIM, "size_x": nx, "size_y": ny, "delta": delta})
+    mid.add_species(-1.0, 1.0, **sp)
+    mid.update_position(0)
+    mid.apply_bc(0)
+    m = mid.species(0)
+    absr.add_species(1.0, 1.0, m["ix"], m["iy"], m["dx"], m["dy"], np.abs(m["px"]), np.abs(m["py"]),
+                     np.abs(m["pz"]), m["psa"])
+    absr.deposit_current(0)
+    eps = np.finfo(ft).eps
+    cnt = 9 * 30000 / (nx * ny) * 8  # generous bound on contributions per cell
+    for name in ["jx", "jy", "jz"]:
+        A = absr.grid("jx" if name == "jx" else name)
+        d = np.abs(g.grid(name) - r.grid(name))
+        assert np.all(d <= cnt * eps * A + 1e-30), (name, float(np.max(d / (A + 1e-30))))
+    for name in ["rho", "dens"]:
+        d = np.abs(g.grid(name) - r.grid(name))
+        assert np.max(d) <= cnt * eps * np.max(r.grid("dens")), name
+    assert abs(g.grid("dens").sum() - 30000) < 1e-3 * (1 if prec == 4 else 1e-6)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_deposit_ghosts_bit_exact(prec):
+    V = ref_vectors()
+    g, r = pair(prec)
+    for name in ["jx", "jy", "jz", "rho", "dens"]:
+        g.set_grid(name, V[("ghost_deposit", "input")])
+        r.set_grid(name, V[("ghost_deposit", "input")])
+    g.fold_all()
+    r.fold_all()
+    for name in ["jx", "rho", "dens"]:
+        assert np.array_equal(g.grid(name), r.grid(name))
+    assert np.max(np.abs(g.grid("jx").ravel() - V[("ghost_deposit", "expected_output")])) < 1e-4
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("shape", [(12, 24), (64, 256), (48, 96)])
+def test_binomial_filter_bit_exact_with_stale_ring(prec, shape):
+    ny, nx = shape
+    g, r = pair(prec, size_x=nx, size_y=ny, delta=2)
+    a = np.random.default_rng(11).standard_normal((ny + 2, nx + 2)).astype(g.ft)  # non-periodic ring
+    g.set_grid("jy", a)
+    r.set_grid("jy", a)
+    g.filter("jy", 4)
+    for _ in range(4):
+        r.binomial_pass("jy")
+    assert np.array_equal(g.grid("jy"), r.grid("jy"))
+
+
+def test_binomial_filter_reference_vector():
+    V = ref_vectors()
+    g, _ = pair(4)
+    g.set_grid("jx", V[("binomial_filter", "input")])
+    g.filter("jx", 4)
+    assert np.max(np.abs(g.grid("jx").ravel() - V[("binomial_filter", "expected_out.spatial")])) < 1e-4
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("shape", [(12, 24), (1024, 4096), (64, 256), (6, 48), (512, 2048)])
+def test_fft2d_against_fp64(prec, shape):
+    ny, nx = shape
+    g, _ = pair(prec, size_x=nx, size_y=ny, delta=2)
+    rng = np.random.default_rng(ny + nx)
+    x = (rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))).astype(g.ct)
+    g.set_spectrum("jx", x)
+    g.fft2d("jx")
+    ref = np.fft.fft2(x.astype(np.complex128))
+    err = np.linalg.norm(g.spectrum("jx") - ref) / np.linalg.norm(ref)
+    tol = (5e-7 if prec == 4 else 1e-15) * np.log2(nx * ny)
+    assert err < tol, err
+    g.fft2d("jx", inverse=True)
+    back = g.spectrum("jx")
+    err = np.linalg.norm(back - x) / np.linalg.norm(x)
+    assert err < tol, err
+
+
+def test_fft2d_reference_vector():
+    V = ref_vectors()
+    g, _ = pair(4)
+    x = V[("get_2d_input", "input")]
+    g.set_spectrum("jx", (x[:, 0] + 1j * x[:, 1]).reshape(12, 24))
+    g.fft2d("jx")
+    out = V[("forward_fft2d", "out")]
+    out = (out[:, 0] + 1j * out[:, 1]).reshape(12, 24)
+    y = g.spectrum("jx")
+    assert np.max(np.abs(y - out)) < 1e-4
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_spectral_update_bit_exact(prec):
+    nx, ny = 64, 32
+    g, r = pair(prec, size_x=nx, size_y=ny, delta=2, dens=3)
+    rng = np.random.default_rng(5)
+    for name in ["ex", "ey", "ez", "jx", "jy", "jz", "rho", "bxw", "byw", "bzw"]:
+        s = (rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))).astype(g.ct)
+        g.set_spectrum(name, s)
+        r.set_spectrum(name, s)
+    both(g, r, "spectral_update")
+    for name in ["ex", "ey", "ez", "bx", "by", "bz", "bxw", "byw", "bzw", "jx", "jy"]:
+        assert np.array_equal(g.spectrum(name), r.spectrum(name)), name
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_fields_update_matches_oracle(prec):
+    nx, ny = 128, 64
+    g, r = pair(prec, size_x=nx, size_y=ny, delta=4, dens=4, n_pass=4)
+    rng = np.random.default_rng(9)
+    for name in ["ex", "ey", "ez", "jx", "jy", "jz", "rho"]:
+        a = rng.standard_normal((ny + 2, nx + 2)).astype(g.ft)
+        g.set_grid(name, a)
+        r.set_grid(name, a)
+    for _ in range(3):
+        both(g, r, "fields_update")
+    tol = 2e-5 if prec == 4 else 1e-12
+    for name in ["ex", "ey", "ez", "bx", "by", "bz", "jx", "rho"]:
+        assert rel(g.grid(name), r.grid(name)) < tol, (name, rel(g.grid(name), r.grid(name)))
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_full_step_on_reference_fixture(prec):
+    # build_test_sim (src/lib.rs:219-244), two species as in run() (lib.rs:124-126)
+    g, r = pair(prec)
+    for q, sd in [(1.0, 1), (-1.0, 2)]:
+        sp = shock_species(24, 12, 5, 2, 3.0, 5.0, 1e-3, seed=sd, ft=g.ft)
+        g.add_species(q, 1.0, **sp)
+        r.add_species(q, 1.0, **sp)
+    both(g, r, "prestep")
+    for sid in range(2):  # the pre-loop move depends only on the initial state
+        a, b = g.species(sid), r.species(sid)
+        for k in ["ix", "iy", "dx", "dy"]:
+            assert np.array_equal(a[k], b[k]), k
+    both(g, r, "step", 5)
+    tol = 1e-4 if prec == 4 else 1e-11
+    for name in ["ex", "ey", "bz", "jx", "rho"]:
+        assert rel(g.grid(name), r.grid(name)) < tol, (name, rel(g.grid(name), r.grid(name)))
+    for sid in range(2):
+        a, b = g.species(sid), r.species(sid)
+        assert np.mean(a["ix"] == b["ix"]) > 0.999
+        assert rel(a["px"], b["px"]) < tol
+
+
+def test_shock_many_steps_fused_path_tracks_oracle():
+    # C1-shaped slab at reduced size: 256x32 cells, 8 ppc per species, mass ratio 16,
+    # c = 10, dt = 0.045 (0.45 cells/step), 60 steps with the fused push+deposit pass.
+    kw = dict(size_x=256, size_y=32, delta=8, dens=8, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0)
+    g = capi.GpuSim(prec=4, **kw)
+    r = O.RefSim(prec=4, **kw)
+    for q, m, vth, sd in [(1.0, 16.0, 0.0025, 3), (-1.0, 1.0, 0.01, 4)]:
+        sp = shock_species(256, 32, 8, 8, 10.0, 10.0, vth, seed=sd)
+        g.add_species(q, m, **sp)
+        r.add_species(q, m, **sp)
+    both(g, r, "prestep")
+    for _ in range(3):
+        both(g, r, "step", 20)
+        for name in ["ex", "bz", "rho"]:
+            scale = np.max(np.abs(r.grid(name))) + 1e-30
+            assert np.max(np.abs(g.grid(name) - r.grid(name))) / scale < 5e-3, name
+    for sid in range(2):
+        a, b = g.species(sid), r.species(sid)
+        assert np.mean((a["ix"] == b["ix"]) & (a["iy"] == b["iy"])) > 0.99
+    assert abs(g.grid("dens")[1:-1, 1:-1].sum() - 2 * 8 * 240 * 32) < 1.0
+
+
+def test_gpu_initialiser_positions_and_momentum_moments():
+    kw = dict(size_x=128, size_y=64, delta=8, dens=16, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0)
+    g = capi.GpuSim(prec=4, **kw)
+    sid = g.init_species(-1.0, 1.0, 0.01, 0)
+    s = g.species(sid)
+    ix, iy, dx, dy = O.init_positions(128, 64, 8, 16, prec=4)
+    assert np.array_equal(s["ix"], ix) and np.array_equal(s["iy"], iy)
+    assert np.array_equal(s["dx"], dx) and np.array_equal(s["dy"], dy)
+    # Zenitani-boosted drifting Gaussian: <px> ~ gamma0 beta0 c, py, pz ~ N(0, (vth c)^2)
+    c, g0 = 10.0, 10.0
+    b0 = np.sqrt(1 - 1 / g0**2)
+    assert np.mean(s["px"]) == pytest.approx(g0 * b0 * c, rel=2e-3)
+    assert np.std(s["py"]) == pytest.approx(0.01 * c, rel=2e-2)
+    assert abs(np.mean(s["pz"])) < 5e-3 * 0.01 * c * 10
+    assert np.all(s["px"] > 0)
