@@ -12,6 +12,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libspectral_rs_hip.so")
+BINDIR = os.path.join(HERE, "bin")
+HOST = os.path.join(BINDIR, "spectral_rs")
 SOURCES = ["api.hip", "particles.hip", "fields.hip", "fft.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -36,7 +38,21 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd, cwd=CSRC)
     shutil.move(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose=False):
+    """The C++ host program (mirror of the reference binary) over the C ABI."""
+    os.makedirs(BINDIR, exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HERE, "host", "spectral_rs.cpp"), "-o", HOST + ".tmp",
+           "-L", LIBDIR, "-lspectral_rs_hip", "-Wl,-rpath,$ORIGIN/../lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    shutil.move(HOST + ".tmp", HOST)
+    return HOST
 
 
 if __name__ == "__main__":

@@ -1,0 +1,75 @@
+"""The C++ host program (mirror of the reference binary): config.toml parsing
+and Config::new validation (src/lib.rs:69-117) run on the CPU; the end-to-end
+run with the output writer (src/save/mod.rs:25-81) is a GPU test."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from spectral_rs_amd import build as B
+
+REF_CONFIG = """[setup]
+t_final = {t_final}
+
+[params]
+size_x = {size_x}
+size_y = {size_y}
+delta = {delta}
+dt = 0.1 # In units 1/omega_pe
+c = 3.0 # This is really c/omega_pe. Don't touch this.
+dens = 2   # of prtls per species per cell
+gamma_inj = {gamma_inj} # Speed of upstream flow
+n_pass = 4 # Number of filter passes.
+
+[output]
+track_prtls = false # need to think of a better way of doing this.
+track_interval = 100 # In timesteps
+write_output = {write_output} # save output tsteps
+output_interval = 1 # In timesteps
+stride = 1 # Prtl downsampling
+istep = {istep} # Flds downsampling
+"""
+
+
+def run_host(tmp_path, **kw):
+    p = dict(t_final=500, size_x=2048, size_y=256, delta=64, gamma_inj=10.0, write_output="false", istep=1)
+    p.update(kw)
+    (tmp_path / "config.toml").write_text(REF_CONFIG.format(**p))
+    if not os.path.exists(B.HOST):
+        B.build_host()
+    r = subprocess.run([B.HOST], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    return r.returncode, r.stdout
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(size_x=2047), "Number of cells in x direction must be even"),
+    (dict(size_y=255), "Number of cells in y direction must be even"),
+    (dict(delta=1024), "Delta must be less than 1/2 the size of size_x"),
+    (dict(gamma_inj=1.0), "Lorentz factor must be greater than 1"),
+    (dict(istep=3), "Number of cells in x & y direction must be divisible by 2"),
+])
+def test_config_errors(tmp_path, kw, msg):
+    rc, out = run_host(tmp_path, **kw)
+    assert rc == 1
+    assert out.strip() == "Problem parsing arguments: " + msg
+
+
+def test_missing_config(tmp_path):
+    r = subprocess.run([B.HOST], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Could not open the config.toml file" in r.stdout
+
+
+@pytest.mark.gpu
+def test_host_run_writes_reference_output_layout(tmp_path):
+    rc, out = run_host(tmp_path, t_final=3, size_x=48, size_y=24, delta=4, write_output="true")
+    assert rc == 0, out
+    for t in range(4):
+        d = tmp_path / "output" / f"dat_{t:05d}"
+        x = np.load(d / "x.npy")
+        assert x.dtype == np.float32 and x.size == 2 * 40 * 24
+        assert np.all((x >= 4) & (x < 46))
+        gam = np.load(d / "gam.npy")
+        assert np.all(gam >= 1.0)
+        for f in ["Jx", "Jy", "Jz", "Ex", "Ey", "Ez", "Bx", "By", "Bz", "Dens"]:
+            assert np.load(d / "flds" / f"{f}.npy").shape == (48 * 24,)
