@@ -25,12 +25,16 @@ namespace {
 
 thread_local std::string g_create_error;
 
-struct SpeciesBuf {
-  int64_t n = 0;
-  double charge = 0, mass = 0;
+struct SoA {
   int32_t* cell = nullptr;
   void *dx = nullptr, *dy = nullptr, *px = nullptr, *py = nullptr, *pz = nullptr;
   int32_t* tag = nullptr;
+};
+
+struct SpeciesBuf {
+  int64_t n = 0;
+  double charge = 0, mass = 0;
+  SoA cur, alt;  // alt is the target of the tile sort; the two swap after it
 };
 
 struct TimerPair {
@@ -52,6 +56,8 @@ struct sp_ctx {
   FftDim fx{}, fy{};
   int* flags_d = nullptr;
   int* flags_h = nullptr;
+  int* tile_count = nullptr;   // per-tile particle counts of the tile sort
+  int* tile_cursor = nullptr;  // per-tile write positions
   std::vector<SpeciesBuf> species;
   std::string err;
   bool push_pending = false;
@@ -98,16 +104,16 @@ template <class T> Geo<T> geo(const sp_ctx* c) {
   return g;
 }
 
-template <class T> SpeciesDev<T> sdev(const sp_ctx* c, const SpeciesBuf& s) {
+template <class T> SpeciesDev<T> sdev(const sp_ctx* c, const SpeciesBuf& s, const SoA& a) {
   SpeciesDev<T> d;
   d.n = s.n;
-  d.cell = s.cell;
-  d.dx = (T*)s.dx;
-  d.dy = (T*)s.dy;
-  d.px = (T*)s.px;
-  d.py = (T*)s.py;
-  d.pz = (T*)s.pz;
-  d.tag = s.tag;
+  d.cell = a.cell;
+  d.dx = (T*)a.dx;
+  d.dy = (T*)a.dy;
+  d.px = (T*)a.px;
+  d.py = (T*)a.py;
+  d.pz = (T*)a.pz;
+  d.tag = a.tag;
   const T q = (T)s.charge, m = (T)s.mass;
   const T dt = (T)c->p.dt, cc = (T)c->p.c;
   d.charge = q;
@@ -115,6 +121,7 @@ template <class T> SpeciesDev<T> sdev(const sp_ctx* c, const SpeciesBuf& s) {
   d.alpha = (q / m) * T(0.5) * dt / cc;    // src/prtls.rs:62
   return d;
 }
+template <class T> SpeciesDev<T> sdev(const sp_ctx* c, const SpeciesBuf& s) { return sdev<T>(c, s, s.cur); }
 
 GridSet gset(const sp_ctx* c) {
   GridSet g;
@@ -237,6 +244,17 @@ template <class T> void fields_update(sp_ctx* c) {
   }
 }
 
+// Tile-sorts every species (particle order only; no value changes).
+template <class T> void sort_species(sp_ctx* c) {
+  for (auto& s : c->species) {
+    launch_tile_sort<T>(sdev<T>(c, s, s.cur), sdev<T>(c, s, s.alt), c->p.size_x, c->p.size_y, c->tile_count,
+                        c->tile_cursor, c->st);
+    std::swap(s.cur, s.alt);
+  }
+}
+
+bool sort_due(const sp_ctx* c) { return c->p.sort_interval > 0 && c->steps % c->p.sort_interval == 0; }
+
 // One iteration of the loop body (src/lib.rs:155-199), with the push left
 // pending (see the file comment).
 template <class T> void one_step(sp_ctx* c) {
@@ -244,6 +262,7 @@ template <class T> void one_step(sp_ctx* c) {
   const GridSet gs = gset(c);
   {
     PhaseScope ps(c, 0);
+    if (sort_due(c)) sort_species<T>(c);
     zero_currents<T>(c);
     {
       PhaseScope pp(c, 3);
@@ -300,18 +319,22 @@ int valid_species(sp_ctx* c, int id) {
 int alloc_species(sp_ctx* c, SpeciesBuf& s, int64_t n) {
   s.n = n;
   const size_t nb = (size_t)(n > 0 ? n : 1);
-  HIPCHK(c, hipMalloc(&s.cell, nb * 4));
-  HIPCHK(c, hipMalloc(&s.tag, nb * 4));
-  void** f[5] = {&s.dx, &s.dy, &s.px, &s.py, &s.pz};
-  for (auto* p : f) HIPCHK(c, hipMalloc(p, nb * c->esz));
+  for (SoA* a : {&s.cur, &s.alt}) {
+    HIPCHK(c, hipMalloc(&a->cell, nb * 4));
+    HIPCHK(c, hipMalloc(&a->tag, nb * 4));
+    void** f[5] = {&a->dx, &a->dy, &a->px, &a->py, &a->pz};
+    for (auto* p : f) HIPCHK(c, hipMalloc(p, nb * c->esz));
+  }
   return SP_OK;
 }
 
 void free_species(SpeciesBuf& s) {
-  hipFree(s.cell);
-  hipFree(s.tag);
-  void* f[5] = {s.dx, s.dy, s.px, s.py, s.pz};
-  for (auto* p : f) hipFree(p);
+  for (SoA* a : {&s.cur, &s.alt}) {
+    hipFree(a->cell);
+    hipFree(a->tag);
+    void* f[5] = {a->dx, a->dy, a->px, a->py, a->pz};
+    for (auto* p : f) hipFree(p);
+  }
 }
 
 }  // namespace
@@ -381,6 +404,11 @@ int sp_create(const sp_params* params, sp_ctx** out) {
     fft_make_twiddles<float>(c->fx, c->st);
     fft_make_twiddles<float>(c->fy, c->st);
   }
+  {
+    const size_t ntiles = (size_t)((p.size_x + TILE_W - 1) / TILE_W) * ((p.size_y + TILE_H - 1) / TILE_H);
+    CK(hipMalloc(&c->tile_count, ntiles * sizeof(int)));
+    CK(hipMalloc(&c->tile_cursor, ntiles * sizeof(int)));
+  }
   CK(hipMalloc(&c->flags_d, sizeof(int)));
   CK(hipHostMalloc((void**)&c->flags_h, sizeof(int), hipHostMallocDefault));
   CK(hipMemsetAsync(c->flags_d, 0, sizeof(int), c->st));
@@ -401,6 +429,8 @@ int sp_destroy(sp_ctx* c) {
   hipFree(c->fx.tw);
   hipFree(c->fy.tw);
   hipFree(c->flags_d);
+  hipFree(c->tile_count);
+  hipFree(c->tile_cursor);
   if (c->flags_h) hipHostFree(c->flags_h);
   for (auto& tp : c->open_pairs) { hipEventDestroy(tp.a); hipEventDestroy(tp.b); }
   for (auto e : c->free_events) hipEventDestroy(e);
@@ -433,10 +463,10 @@ int sp_species_add(sp_ctx* c, double charge, double mass, int64_t n, const int32
   rc = alloc_species(c, s, n);
   if (rc) { free_species(s); return rc; }
   if (n > 0) {
-    HIPCHK(c, hipMemcpy(s.cell, cell.data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(s.tag, tag.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(s.cur.cell, cell.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(s.cur.tag, tag.data(), n * 4, hipMemcpyHostToDevice));
     const void* src[5] = {dx, dy, px, py, pz};
-    void* dst[5] = {s.dx, s.dy, s.px, s.py, s.pz};
+    void* dst[5] = {s.cur.dx, s.cur.dy, s.cur.px, s.cur.py, s.cur.pz};
     for (int k = 0; k < 5; ++k) HIPCHK(c, hipMemcpy(dst[k], src[k], n * c->esz, hipMemcpyHostToDevice));
   }
   c->species.push_back(s);
@@ -484,11 +514,11 @@ int sp_get_species(sp_ctx* c, int id, int64_t n, int32_t* ix, int32_t* iy, void*
   if (rc) return rc;
   if (n == 0) return SP_OK;
   std::vector<int32_t> cell(n), tag(n);
-  HIPCHK(c, hipMemcpy(cell.data(), s.cell, n * 4, hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(tag.data(), s.tag, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(cell.data(), s.cur.cell, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(tag.data(), s.cur.tag, n * 4, hipMemcpyDeviceToHost));
   std::vector<unsigned char> buf(n * c->esz);
   void* outs[5] = {dx, dy, px, py, pz};
-  void* srcs[5] = {s.dx, s.dy, s.px, s.py, s.pz};
+  void* srcs[5] = {s.cur.dx, s.cur.dy, s.cur.px, s.cur.py, s.cur.pz};
   const size_t e = c->esz;
   for (int k = 0; k < 5; ++k) {
     HIPCHK(c, hipMemcpy(buf.data(), srcs[k], n * e, hipMemcpyDeviceToHost));
@@ -640,6 +670,10 @@ int sp_spectral_update(sp_ctx* c) {
 int sp_prestep(sp_ctx* c) {
   ENTRY_PRELUDE(c);
   PhaseScope ps(c, 0);
+  if (c->p.sort_interval > 0) {
+    if (c->f64) sort_species<double>(c);
+    else sort_species<float>(c);
+  }
   for (auto& s : c->species) {
     if (c->f64) launch_move_deposit<double>(sdev<double>(c, s), geo<double>(c), gset(c), c->flags_d, c->st);
     else launch_move_deposit<float>(sdev<float>(c, s), geo<float>(c), gset(c), c->flags_d, c->st);

@@ -282,5 +282,6 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
 INST(float)
 INST(double)
 #undef INST
+template void fill<int>(int*, int, int64_t, hipStream_t);
 
 }  // namespace sp

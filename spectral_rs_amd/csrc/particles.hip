@@ -92,46 +92,77 @@ template <class T> __device__ __forceinline__ bool inside(int ix, int iy, const 
   return ix >= 1 && ix <= g.nx && iy >= 1 && iy <= g.ny;
 }
 
-template <class T> __device__ __forceinline__ void atomic_add(T* p, T v) { unsafeAtomicAdd(p, v); }
+// Deposit target of one workgroup: a window [x0, x0+w) x [y0, y0+h) of the
+// padded grids held in LDS (planes Jx Jy Jz rho dens, w*h cells each), plus
+// the global grids for stencils that leave the window.  use_lds == false sends
+// everything to the global grids.
+// Pointers with their address space spelled out, so every atomic lowers to
+// the native LDS (ds_add_f32/f64) or global (global_atomic_add_f32/f64)
+// instruction rather than a flat atomic.
+template <class T> using lds_ptr = __attribute__((address_space(3))) T*;
+template <class T> using glb_ptr = __attribute__((address_space(1))) T*;
+template <class T> __device__ __forceinline__ void lds_add(lds_ptr<T> p, T v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T> __device__ __forceinline__ void glb_add(glb_ptr<T> p, T v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T> struct Sink {
+  lds_ptr<T> lds;
+  bool use_lds;
+  int x0, y0, w, h, wh;
+  int pitch;
+  glb_ptr<T> g[5];
+  __device__ __forceinline__ bool holds(int ix, int iy) const {
+    return use_lds && ix - 1 >= x0 && ix + 1 < x0 + w && iy - 1 >= y0 && iy + 1 < y0 + h;
+  }
+};
+
+// Adds w[k] * v to the 3x3 stencil around (ix, iy) of one plane.
+template <class T>
+__device__ __forceinline__ void add9(const Sink<T>& k, int plane, int ix, int iy, bool in_lds, const T (&w)[9], T v) {
+  if (in_lds) {
+    lds_ptr<T> a = k.lds + plane * k.wh + (iy - 1 - k.y0) * k.w + (ix - 1 - k.x0);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) lds_add<T>(a + r * k.w + c, w[3 * r + c] * v);
+  } else {
+    glb_ptr<T> a = k.g[plane] + (iy - 1) * k.pitch + (ix - 1);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) glb_add<T>(a + r * k.pitch + c, w[3 * r + c] * v);
+  }
+}
 
 // Sim::deposit_current (src/lib.rs:266-393): v = q p / gamma, J += w v on the
 // 3x3 stencil around (ix, iy), ghosts included.
 template <class T>
 __device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px, T py, T pz, T psa, T q,
-                                                const Geo<T>& g, T* jx, T* jy, T* jz, int& bad) {
+                                                const Geo<T>& g, const Sink<T>& k, int& bad) {
   if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
   const T inv = T(1) / psa;
   const T vx = q * px * inv, vy = q * py * inv, vz = q * pz * inv;
   T w[9];
   tsc(dx, dy, w);
-  const int base = (iy - 1) * g.pitch + ix - 1;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const int o = base + a * g.pitch + b;
-      atomic_add(jx + o, w[3 * a + b] * vx);
-      atomic_add(jy + o, w[3 * a + b] * vy);
-      atomic_add(jz + o, w[3 * a + b] * vz);
-    }
+  const bool in = k.holds(ix, iy);
+  add9(k, 0, ix, iy, in, w, vx);
+  add9(k, 1, ix, iy, in, w, vy);
+  add9(k, 2, ix, iy, in, w, vz);
 }
 
 // Sim::calc_density (src/lib.rs:395-476): rho += w q, dens += w.
 template <class T>
-__device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, T q, const Geo<T>& g, T* rho, T* dens,
+__device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, T q, const Geo<T>& g, const Sink<T>& k,
                                                 int& bad) {
   if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
   T w[9];
   tsc(dx, dy, w);
-  const int base = (iy - 1) * g.pitch + ix - 1;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const int o = base + a * g.pitch + b;
-      atomic_add(rho + o, w[3 * a + b] * q);
-      atomic_add(dens + o, w[3 * a + b]);
-    }
+  const bool in = k.holds(ix, iy);
+  add9(k, 3, ix, iy, in, w, q);
+  add9(k, 4, ix, iy, in, w, T(1));
 }
 
 // Prtl::boris_push for one particle (src/prtls.rs:306-440): ordered 9-term
@@ -177,31 +208,149 @@ template <class T> __device__ __forceinline__ bool finite3(T a, T b, T c) {
 // Sim::move_and_deposit (src/lib.rs:478-489) for one particle.
 template <class T>
 __device__ __forceinline__ void move_and_deposit(int& ix, int& iy, T& dx, T& dy, T& px, T py, T pz, T q,
-                                                 const Geo<T>& g, const GridSet& gs, int& bad) {
+                                                 const Geo<T>& g, const Sink<T>& k, int& bad) {
   const T psa = gamma_of(px, py, pz, g.csqinv);
   const T c1 = g.half_dt * (T(1) / psa);
   move_half(ix, iy, dx, dy, px, py, c1, g, bad);
-  deposit_current(ix, iy, dx, dy, px, py, pz, psa, q, g, (T*)gs.g[SP_JX], (T*)gs.g[SP_JY], (T*)gs.g[SP_JZ], bad);
+  deposit_current(ix, iy, dx, dy, px, py, pz, psa, q, g, k, bad);
   move_half(ix, iy, dx, dy, px, py, c1, g, bad);
-  deposit_density(ix, iy, dx, dy, q, g, (T*)gs.g[SP_RHO], (T*)gs.g[SP_DENS], bad);
+  deposit_density(ix, iy, dx, dy, q, g, k, bad);
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_move_deposit(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  int bad = 0;
-  if (i < s.n) {
-    const int32_t c = s.cell[i];
-    int ix = cell_ix(c), iy = cell_iy(c);
-    T dx = s.dx[i], dy = s.dy[i], px = s.px[i];
-    const T py = s.py[i], pz = s.pz[i];
-    move_and_deposit(ix, iy, dx, dy, px, py, pz, s.charge, g, gs, bad);
-    s.cell[i] = make_cell(ix, iy);
-    s.dx[i] = dx;
-    s.dy[i] = dy;
-    s.px[i] = px;
+__device__ __forceinline__ int wave_or(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off);
+  return v;
+}
+
+// The particle pass.  One workgroup owns a chunk of K consecutive particles:
+//  1. the chunk's state is staged in LDS with coalesced loads, and the
+//     bounding box of its cells is reduced;
+//  2. particles move at most one cell per step (c dt < 1) and deposit on a
+//     3x3 stencil, so every deposit of the chunk lands within 2 cells of that
+//     box: the box + 2 becomes an LDS window of the five deposit grids;
+//  3. each thread processes PPT particles, visiting slot (s * 33) mod K for
+//     s = tid + NT j: consecutive lanes get particles ~33 slots apart, i.e. in
+//     different cells of a cell-sorted chunk, so LDS atomics rarely collide;
+//     optionally the Boris push of the previous step runs first (PUSH);
+//  4. state goes back through LDS with coalesced stores and the window is
+//     added to HBM with one atomic per non-zero cell and grid.
+// Particles whose stencil leaves the window (periodic wrap, unsorted input)
+// and chunks whose window would not fit use global atomics directly, so the
+// result never depends on the particle order; only the speed does.
+template <class T> struct ChunkShape {
+  static constexpr int NT = 512;
+  static constexpr int K = sizeof(T) == 4 ? 2048 : 1024;
+  static constexpr int WMAX = sizeof(T) == 4 ? 1536 : 640;
+};
+
+template <class T, bool PUSH>
+__global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
+  constexpr int NT = ChunkShape<T>::NT, K = ChunkShape<T>::K, WMAX = ChunkShape<T>::WMAX, PPT = K / NT;
+  __shared__ int32_t s_cell[K];
+  __shared__ T s_dx[K], s_dy[K], s_px[K], s_py[K], s_pz[K];
+  __shared__ T s_acc[5 * WMAX];
+  __shared__ int s_box[4];
+  const int tid = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * K;
+  const int cnt = (int)min((int64_t)K, s.n - base);
+  if (tid == 0) {
+    s_box[0] = INT_MAX;
+    s_box[1] = INT_MIN;
+    s_box[2] = INT_MAX;
+    s_box[3] = INT_MIN;
   }
-  if (bad) atomicOr(flags, bad);
+  __syncthreads();
+  int xmn = INT_MAX, xmx = INT_MIN, ymn = INT_MAX, ymx = INT_MIN;
+  for (int i = tid; i < cnt; i += NT) {
+    const int32_t c = s.cell[base + i];
+    s_cell[i] = c;
+    s_dx[i] = s.dx[base + i];
+    s_dy[i] = s.dy[base + i];
+    s_px[i] = s.px[base + i];
+    s_py[i] = s.py[base + i];
+    s_pz[i] = s.pz[base + i];
+    const int ix = cell_ix(c), iy = cell_iy(c);
+    xmn = min(xmn, ix);
+    xmx = max(xmx, ix);
+    ymn = min(ymn, iy);
+    ymx = max(ymx, iy);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    xmn = min(xmn, __shfl_xor(xmn, off));
+    xmx = max(xmx, __shfl_xor(xmx, off));
+    ymn = min(ymn, __shfl_xor(ymn, off));
+    ymx = max(ymx, __shfl_xor(ymx, off));
+  }
+  if ((tid & 63) == 0) {
+    atomicMin(&s_box[0], xmn);
+    atomicMax(&s_box[1], xmx);
+    atomicMin(&s_box[2], ymn);
+    atomicMax(&s_box[3], ymx);
+  }
+  __syncthreads();
+  Sink<T> k;
+  k.pitch = g.pitch;
+  k.g[0] = (glb_ptr<T>)gs.g[SP_JX];
+  k.g[1] = (glb_ptr<T>)gs.g[SP_JY];
+  k.g[2] = (glb_ptr<T>)gs.g[SP_JZ];
+  k.g[3] = (glb_ptr<T>)gs.g[SP_RHO];
+  k.g[4] = (glb_ptr<T>)gs.g[SP_DENS];
+  k.x0 = max(s_box[0] - 2, 0);
+  k.y0 = max(s_box[2] - 2, 0);
+  k.w = min(s_box[1] + 2, g.nx + 1) - k.x0 + 1;
+  k.h = min(s_box[3] + 2, g.ny + 1) - k.y0 + 1;
+  const bool use_lds = cnt > 0 && k.w > 0 && k.h > 0 && k.w * k.h <= WMAX;
+  k.wh = use_lds ? k.w * k.h : 0;
+  k.lds = (lds_ptr<T>)s_acc;
+  k.use_lds = use_lds;
+  for (int i = tid; i < 5 * k.wh; i += NT) s_acc[i] = T(0);
+  __syncthreads();
+
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int p = ((tid + NT * j) * 33) & (K - 1);
+    if (p < cnt) {
+      const int32_t c = s_cell[p];
+      int ix = cell_ix(c), iy = cell_iy(c);
+      T dx = s_dx[p], dy = s_dy[p], px = s_px[p], py = s_py[p], pz = s_pz[p];
+      if (PUSH) {
+        boris(ix, iy, dx, dy, px, py, pz, gs, g, s.alpha, s.beta, bad);
+        if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
+        s_py[p] = py;
+        s_pz[p] = pz;
+      }
+      move_and_deposit(ix, iy, dx, dy, px, py, pz, s.charge, g, k, bad);
+      s_cell[p] = make_cell(ix, iy);
+      s_dx[p] = dx;
+      s_dy[p] = dy;
+      s_px[p] = px;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < cnt; i += NT) {
+    s.cell[base + i] = s_cell[i];
+    s.dx[base + i] = s_dx[i];
+    s.dy[base + i] = s_dy[i];
+    s.px[base + i] = s_px[i];
+    if (PUSH) {
+      s.py[base + i] = s_py[i];
+      s.pz[base + i] = s_pz[i];
+    }
+  }
+  for (int i = tid; i < k.wh; i += NT) {
+    const int r = i / k.w, cc = i - r * k.w;
+    const int64_t o = (int64_t)(k.y0 + r) * g.pitch + k.x0 + cc;
+#pragma unroll
+    for (int pl = 0; pl < 5; ++pl) {
+      const T v = s_acc[pl * k.wh + i];
+      if (v != T(0)) glb_add<T>(k.g[pl] + o, v);
+    }
+  }
+  bad = wave_or(bad);
+  if (bad && (tid & 63) == 0) atomicOr(flags, bad);
 }
 
 template <class T>
@@ -213,32 +362,6 @@ __global__ void __launch_bounds__(256) k_push(SpeciesDev<T> s, Geo<T> g, GridSet
     T px = s.px[i], py = s.py[i], pz = s.pz[i];
     boris(cell_ix(c), cell_iy(c), s.dx[i], s.dy[i], px, py, pz, gs, g, s.alpha, s.beta, bad);
     if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
-    s.px[i] = px;
-    s.py[i] = py;
-    s.pz[i] = pz;
-  }
-  if (bad) atomicOr(flags, bad);
-}
-
-// boris_push of step t followed by move_and_deposit of step t+1: adjacent in
-// the reference loop (src/lib.rs:171-173, :193-196) with nothing between them
-// that reads or writes particles, so one pass reads and writes each particle
-// once per step.
-template <class T>
-__global__ void __launch_bounds__(256) k_push_move_deposit(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  int bad = 0;
-  if (i < s.n) {
-    const int32_t c = s.cell[i];
-    int ix = cell_ix(c), iy = cell_iy(c);
-    T dx = s.dx[i], dy = s.dy[i];
-    T px = s.px[i], py = s.py[i], pz = s.pz[i];
-    boris(ix, iy, dx, dy, px, py, pz, gs, g, s.alpha, s.beta, bad);
-    if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
-    move_and_deposit(ix, iy, dx, dy, px, py, pz, s.charge, g, gs, bad);
-    s.cell[i] = make_cell(ix, iy);
-    s.dx[i] = dx;
-    s.dy[i] = dy;
     s.px[i] = px;
     s.py[i] = py;
     s.pz[i] = pz;
@@ -299,10 +422,102 @@ __global__ void __launch_bounds__(256) k_init(SpeciesDev<T> s, Geo<T> g, int den
 
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// ------------------------------------------------------------- tile sort
+// Counting sort of a species by tile (TILE_W x TILE_H cells, tiles in
+// row-major order); the order inside a tile is whatever the scatter's atomics
+// give.  The key of a particle is a pure function of its cell, so the keys
+// are bit-exact; only the order within a tile is not reproducible.
+__device__ __forceinline__ int tile_of(int32_t c, int ntx, int nty) {
+  const int tx = min(max((cell_ix(c) - 1) >> TILE_W_LOG, 0), ntx - 1);
+  const int ty = min(max((cell_iy(c) - 1) >> TILE_H_LOG, 0), nty - 1);
+  return ty * ntx + tx;
+}
+
+// Lanes of a wave that share a tile are served by one atomic: the lowest
+// such lane adds the group's size and broadcasts the old value.
+__device__ __forceinline__ int grouped_atomic_add(int* ctr, int key, bool active) {
+  const int lane = __lane_id();
+  uint64_t todo = __ballot(active);
+  int mine = 0;
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const int lkey = __shfl(key, leader);
+    const uint64_t grp = __ballot(active && key == lkey) & todo;
+    int basev = 0;
+    if (lane == leader) basev = atomicAdd(ctr + lkey, __popcll(grp));
+    basev = __shfl(basev, leader);
+    if ((grp >> lane) & 1ull) mine = basev + __popcll(grp & ((1ull << lane) - 1ull));
+    todo &= ~grp;
+  }
+  return mine;
+}
+
+__global__ void __launch_bounds__(256) k_tile_hist(const int32_t* __restrict__ cell, int64_t n, int ntx, int nty,
+                                                   int* count) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool act = i < n;
+  const int t = act ? tile_of(cell[i], ntx, nty) : 0;
+  grouped_atomic_add(count, t, act);
+}
+
+// Exclusive prefix sum of `count` into `cursor`, one workgroup.
+__global__ void __launch_bounds__(1024) k_tile_scan(const int* __restrict__ count, int* cursor, int ntiles) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int per = (ntiles + 1023) / 1024;
+  const int b = tid * per, e = min(b + per, ntiles);
+  int sum = 0;
+  for (int i = b; i < e; ++i) sum += count[i];
+  part[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - sum;
+  for (int i = b; i < e; ++i) {
+    cursor[i] = run;
+    run += count[i];
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_tile_scatter(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty,
+                                                      int* cursor) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool act = i < a.n;
+  const int32_t c = act ? a.cell[i] : 0;
+  const int t = act ? tile_of(c, ntx, nty) : 0;
+  const int pos = grouped_atomic_add(cursor, t, act);
+  if (!act) return;
+  b.cell[pos] = c;
+  b.dx[pos] = a.dx[i];
+  b.dy[pos] = a.dy[i];
+  b.px[pos] = a.px[i];
+  b.py[pos] = a.py[i];
+  b.pz[pos] = a.pz[i];
+  b.tag[pos] = a.tag[i];
+}
+
+template <class T>
+void launch_tile_sort(const SpeciesDev<T>& a, const SpeciesDev<T>& b, int nx, int ny, int* count, int* cursor,
+                      hipStream_t st) {
+  if (a.n == 0) return;
+  const int ntx = (nx + TILE_W - 1) / TILE_W, nty = (ny + TILE_H - 1) / TILE_H;
+  fill<int>(count, 0, (int64_t)ntx * nty, st);
+  hipLaunchKernelGGL(k_tile_hist, dim3(blocks_for(a.n, 256)), dim3(256), 0, st, a.cell, a.n, ntx, nty, count);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, count, cursor, ntx * nty);
+  hipLaunchKernelGGL(k_tile_scatter<T>, dim3(blocks_for(a.n, 256)), dim3(256), 0, st, a, b, ntx, nty, cursor);
+}
+
 template <class T>
 void launch_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
   if (s.n == 0) return;
-  hipLaunchKernelGGL(k_move_deposit<T>, dim3(blocks_for(s.n, 256)), dim3(256), 0, st, s, g, gs, flags);
+  const int K = ChunkShape<T>::K;
+  hipLaunchKernelGGL((k_particles<T, false>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s, g, gs,
+                     flags);
 }
 template <class T>
 void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
@@ -313,7 +528,9 @@ template <class T>
 void launch_push_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags,
                               hipStream_t st) {
   if (s.n == 0) return;
-  hipLaunchKernelGGL(k_push_move_deposit<T>, dim3(blocks_for(s.n, 256)), dim3(256), 0, st, s, g, gs, flags);
+  const int K = ChunkShape<T>::K;
+  hipLaunchKernelGGL((k_particles<T, true>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s, g, gs,
+                     flags);
 }
 template <class T>
 void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, double gamma_inj, uint64_t seed,
@@ -333,7 +550,8 @@ void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, do
   template void launch_push_move_deposit<T>(const SpeciesDev<T>&, const Geo<T>&, const GridSet&, int*,          \
                                             hipStream_t);                                                     \
   template void launch_init_species<T>(const SpeciesDev<T>&, const Geo<T>&, double, double, uint64_t, int,      \
-                                       hipStream_t);
+                                       hipStream_t);                                                          \
+  template void launch_tile_sort<T>(const SpeciesDev<T>&, const SpeciesDev<T>&, int, int, int*, int*, hipStream_t);
 INST(float)
 INST(double)
 #undef INST
