@@ -97,6 +97,7 @@ template <class T> Geo<T> geo(const sp_ctx* c) {
   g.periodic_x = c->p.periodic_x;
   g.checked = c->p.checked;
   g.left_edge = c->p.left_edge;
+  g.lds_ok = c->p.dt * c->p.c < 1.0;
   g.dt = (T)c->p.dt;
   g.c = (T)c->p.c;
   g.csqinv = T(1.) / (g.c * g.c);
@@ -227,6 +228,8 @@ template <class T> void fields_update(sp_ctx* c) {
     b.scale[6] = (double)(T(1.0) / (T)c->p.dens);  // mod.rs:80-84
     launch_fft_rows<T>(b, c->fx, ny, 0, LOAD_REAL_PADDED, STORE_COMPLEX, 1.0, c->st);
     launch_fft_cols<T>(b, c->fy, nx, 0, 1.0, c->st);
+    // transforms of real fields: keep them exactly Hermitian (fields.hip)
+    launch_hermitian<T>(b.spec, 7, nx, ny, c->st);
   }
   launch_spectral_update<T>(c->spec, geo<T>(c), c->p.dens, c->st);  // mod.rs:105-278
   {  // mod.rs:280-307: 6 inverse transforms, spectral->spatial + ghosts fused

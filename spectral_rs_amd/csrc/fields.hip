@@ -260,6 +260,38 @@ __global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T
   st((T*)sp.p[SP_BZ], i, scl(T(0.5), add(bz, wz)));
 }
 
+// Hermitian projection X[k] <- (X[k] + conj(X[-k])) / 2, X[-k] <- conj(X[k])
+// of spectra that are transforms of real fields.  Exact in exact arithmetic;
+// in floating point it removes the anti-Hermitian rounding component.  That
+// component matters because B_wrk persists in spectral space: its
+// anti-Hermitian part evolves as B_a <- (1 - (c dt |k|)^2) B_a each step (the
+// matching E part is dropped by the real-part extraction), which grows for
+// c dt |k| > sqrt(2).  With Hermitian inputs the pointwise update of
+// k_spectral maps mirrored modes to exactly conjugate outputs, so B_wrk stays
+// exactly Hermitian.  One thread per mode; the lower index of each mirrored
+// pair writes both.
+template <class T> __global__ void __launch_bounds__(256) k_hermitian(SpecPtrs sp, int count, int nx, int ny) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nx * ny) return;
+  const int row = (int)(i / nx), col = (int)(i - (int64_t)row * nx);
+  const int64_t m = (int64_t)((ny - row) % ny) * nx + (nx - col) % nx;
+  if (m < i) return;
+  for (int f = 0; f < count; ++f) {
+    T* p = (T*)sp.p[f];
+    const Cx<T> a = ld(p, i), b = ld(p, m);
+    const Cx<T> s{T(0.5) * (a.re + b.re), T(0.5) * (a.im - b.im)};
+    st(p, i, s);
+    if (m != i) st(p, m, Cx<T>{s.re, -s.im});
+  }
+}
+
+template <class T> void launch_hermitian(void* const* spec, int count, int nx, int ny, hipStream_t st) {
+  SpecPtrs sp{};
+  for (int i = 0; i < count && i < 14; ++i) sp.p[i] = spec[i];
+  const int64_t n = (int64_t)nx * ny;
+  hipLaunchKernelGGL(k_hermitian<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sp, count, nx, ny);
+}
+
 template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, hipStream_t st) {
   SpecPtrs sp{};
   for (int i = 0; i < 14; ++i) sp.p[i] = spec[i];
@@ -278,7 +310,8 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   template void launch_fold_ghosts<T>(T* const*, int, int, int, hipStream_t);                   \
   template void launch_update_ghosts<T>(T* const*, int, int, int, hipStream_t);                 \
   template void launch_binomial<T>(T* const*, T* const*, int, int, int, int, hipStream_t);      \
-  template void launch_spectral_update<T>(void* const*, const Geo<T>&, int, hipStream_t);
+  template void launch_spectral_update<T>(void* const*, const Geo<T>&, int, hipStream_t);            \
+  template void launch_hermitian<T>(void* const*, int, int, int, hipStream_t);
 INST(float)
 INST(double)
 #undef INST

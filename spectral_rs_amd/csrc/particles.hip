@@ -108,28 +108,56 @@ template <class T> __device__ __forceinline__ void glb_add(glb_ptr<T> p, T v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// LDS accumulation.  Float atomics in LDS run ~40x slower than integer ones
+// on gfx950, so FP32 windows accumulate in int32 fixed point: a contribution
+// x becomes rint(x * 2^e), with e chosen per chunk and plane so that no
+// window cell can overflow (see k_particles).  Integer sums are exact and
+// order-independent; the only rounding is the per-contribution quantisation
+// (|error| <= 2^-(e+1)) and the conversion of the window sum at the flush.
+// FP64 windows (the dprec precision) accumulate with native f64 atomics.
+template <class T> struct Acc;
+template <> struct Acc<float> {
+  using type = int;
+  static constexpr bool fixed = true;
+};
+template <> struct Acc<double> {
+  using type = double;
+  static constexpr bool fixed = false;
+};
+
 template <class T> struct Sink {
-  lds_ptr<T> lds;
+  lds_ptr<typename Acc<T>::type> lds;
   bool use_lds;
   int x0, y0, w, h, wh;
   int pitch;
+  T scale[4];  // fixed point: 2^e per LDS plane (Jx Jy Jz dens)
   glb_ptr<T> g[5];
   __device__ __forceinline__ bool holds(int ix, int iy) const {
     return use_lds && ix - 1 >= x0 && ix + 1 < x0 + w && iy - 1 >= y0 && iy + 1 < y0 + h;
   }
 };
 
-// Adds w[k] * v to the 3x3 stencil around (ix, iy) of one plane.
+// Adds w[k] * v to the 3x3 stencil around (ix, iy) of one plane
+// (0..2 = Jx Jy Jz, 3 = dens in LDS; 0..4 = Jx Jy Jz rho dens globally).
 template <class T>
-__device__ __forceinline__ void add9(const Sink<T>& k, int plane, int ix, int iy, bool in_lds, const T (&w)[9], T v) {
+__device__ __forceinline__ void add9(const Sink<T>& k, int lplane, int gplane, int ix, int iy, bool in_lds,
+                                     const T (&w)[9], T v) {
   if (in_lds) {
-    lds_ptr<T> a = k.lds + plane * k.wh + (iy - 1 - k.y0) * k.w + (ix - 1 - k.x0);
+    auto a = k.lds + lplane * k.wh + (iy - 1 - k.y0) * k.w + (ix - 1 - k.x0);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) lds_add<T>(a + r * k.w + c, w[3 * r + c] * v);
+      for (int c = 0; c < 3; ++c) {
+        const T x = w[3 * r + c] * v;
+        if constexpr (Acc<T>::fixed) {
+          __hip_atomic_fetch_add(a + r * k.w + c, __float2int_rn(x * k.scale[lplane]), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+          lds_add<T>(a + r * k.w + c, x);
+        }
+      }
   } else {
-    glb_ptr<T> a = k.g[plane] + (iy - 1) * k.pitch + (ix - 1);
+    glb_ptr<T> a = k.g[gplane] + (iy - 1) * k.pitch + (ix - 1);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -148,9 +176,9 @@ __device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px
   T w[9];
   tsc(dx, dy, w);
   const bool in = k.holds(ix, iy);
-  add9(k, 0, ix, iy, in, w, vx);
-  add9(k, 1, ix, iy, in, w, vy);
-  add9(k, 2, ix, iy, in, w, vz);
+  add9(k, 0, 0, ix, iy, in, w, vx);
+  add9(k, 1, 1, ix, iy, in, w, vy);
+  add9(k, 2, 2, ix, iy, in, w, vz);
 }
 
 // Sim::calc_density (src/lib.rs:395-476): rho += w q, dens += w.
@@ -160,9 +188,14 @@ __device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, T q,
   if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
   T w[9];
   tsc(dx, dy, w);
-  const bool in = k.holds(ix, iy);
-  add9(k, 3, ix, iy, in, w, q);
-  add9(k, 4, ix, iy, in, w, T(1));
+  // Within a window only dens = sum w is accumulated; the flush adds q * sum w
+  // to rho (one species per pass; w * q is exact for |q| = 1).
+  if (k.holds(ix, iy)) {
+    add9(k, 3, 4, ix, iy, true, w, T(1));
+  } else {
+    add9(k, 3, 3, ix, iy, false, w, q);
+    add9(k, 3, 4, ix, iy, false, w, T(1));
+  }
 }
 
 // Prtl::boris_push for one particle (src/prtls.rs:306-440): ordered 9-term
@@ -228,7 +261,12 @@ __device__ __forceinline__ int wave_or(int v) {
 //     bounding box of its cells is reduced;
 //  2. particles move at most one cell per step (c dt < 1) and deposit on a
 //     3x3 stencil, so every deposit of the chunk lands within 2 cells of that
-//     box: the box + 2 becomes an LDS window of the five deposit grids;
+//     box: the box + 2 becomes an LDS window of Jx, Jy, Jz and dens (rho of
+//     one species is q * dens).  For the fixed-point FP32 windows, a window
+//     cell only receives contributions from particles that start within 2
+//     cells of it, each bounded by |q| c for J (|v| < c) and 1 for dens, so
+//     with m the largest particle count of any cell of the chunk a cell's
+//     |sum| <= 25 m |q| c (resp. 25 m); the scale 2^e keeps that below 2^30;
 //  3. each thread processes PPT particles, visiting slot (s * 33) mod K for
 //     s = tid + NT j: consecutive lanes get particles ~33 slots apart, i.e. in
 //     different cells of a cell-sorted chunk, so LDS atomics rarely collide;
@@ -241,16 +279,17 @@ __device__ __forceinline__ int wave_or(int v) {
 template <class T> struct ChunkShape {
   static constexpr int NT = 512;
   static constexpr int K = sizeof(T) == 4 ? 2048 : 1024;
-  static constexpr int WMAX = sizeof(T) == 4 ? 1536 : 640;
+  static constexpr int WMAX = sizeof(T) == 4 ? 1536 : 768;
 };
 
 template <class T, bool PUSH>
 __global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   constexpr int NT = ChunkShape<T>::NT, K = ChunkShape<T>::K, WMAX = ChunkShape<T>::WMAX, PPT = K / NT;
+  using A = typename Acc<T>::type;
   __shared__ int32_t s_cell[K];
   __shared__ T s_dx[K], s_dy[K], s_px[K], s_py[K], s_pz[K];
-  __shared__ T s_acc[5 * WMAX];
-  __shared__ int s_box[4];
+  __shared__ A s_acc[4 * WMAX];
+  __shared__ int s_box[5];
   const int tid = threadIdx.x;
   const int64_t base = (int64_t)blockIdx.x * K;
   const int cnt = (int)min((int64_t)K, s.n - base);
@@ -259,6 +298,7 @@ __global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, Gr
     s_box[1] = INT_MIN;
     s_box[2] = INT_MAX;
     s_box[3] = INT_MIN;
+    s_box[4] = 0;
   }
   __syncthreads();
   int xmn = INT_MAX, xmx = INT_MIN, ymn = INT_MAX, ymx = INT_MIN;
@@ -301,11 +341,36 @@ __global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, Gr
   k.y0 = max(s_box[2] - 2, 0);
   k.w = min(s_box[1] + 2, g.nx + 1) - k.x0 + 1;
   k.h = min(s_box[3] + 2, g.ny + 1) - k.y0 + 1;
-  const bool use_lds = cnt > 0 && k.w > 0 && k.h > 0 && k.w * k.h <= WMAX;
+  // g.lds_ok: c dt < 1, i.e. no particle moves more than one cell per step
+  const bool use_lds = g.lds_ok && cnt > 0 && k.w > 0 && k.h > 0 && k.w * k.h <= WMAX;
   k.wh = use_lds ? k.w * k.h : 0;
-  k.lds = (lds_ptr<T>)s_acc;
+  k.lds = (lds_ptr<A>)s_acc;
   k.use_lds = use_lds;
-  for (int i = tid; i < 5 * k.wh; i += NT) s_acc[i] = T(0);
+  if constexpr (Acc<T>::fixed) {
+    if (use_lds) {  // largest particle count of a cell of this chunk
+      lds_ptr<int> cntg = (lds_ptr<int>)s_acc;
+      for (int i = tid; i < k.wh; i += NT) cntg[i] = 0;
+      __syncthreads();
+      for (int i = tid; i < cnt; i += NT) {
+        const int32_t c = s_cell[i];
+        __hip_atomic_fetch_add(cntg + (cell_iy(c) - k.y0) * k.w + (cell_ix(c) - k.x0), 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      __syncthreads();
+      int m = 0;
+      for (int i = tid; i < k.wh; i += NT) m = max(m, (int)cntg[i]);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+      if ((tid & 63) == 0) atomicMax(&s_box[4], m);
+      __syncthreads();
+    }
+    const float m25 = 25.0f * (float)max(s_box[4], 1);
+    const float bj = m25 * fabsf((float)s.charge) * (float)g.c;
+    // 2^e with e = 30 - ceil(log2(bound)): sums stay below 2^30
+    k.scale[0] = k.scale[1] = k.scale[2] = ldexpf(1.0f, 30 - ilogbf(bj) - 1);
+    k.scale[3] = ldexpf(1.0f, 30 - ilogbf(m25) - 1);
+  }
+  for (int i = tid; i < 4 * k.wh; i += NT) s_acc[i] = A(0);
   __syncthreads();
 
   int bad = 0;
@@ -344,9 +409,18 @@ __global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, Gr
     const int r = i / k.w, cc = i - r * k.w;
     const int64_t o = (int64_t)(k.y0 + r) * g.pitch + k.x0 + cc;
 #pragma unroll
-    for (int pl = 0; pl < 5; ++pl) {
-      const T v = s_acc[pl * k.wh + i];
-      if (v != T(0)) glb_add<T>(k.g[pl] + o, v);
+    for (int pl = 0; pl < 4; ++pl) {
+      const A a = s_acc[pl * k.wh + i];
+      if (a == A(0)) continue;
+      T v;
+      if constexpr (Acc<T>::fixed) v = (T)a * (T(1) / k.scale[pl]);
+      else v = a;
+      if (pl < 3) {
+        glb_add<T>(k.g[pl] + o, v);
+      } else {
+        glb_add<T>(k.g[3] + o, s.charge * v);  // rho
+        glb_add<T>(k.g[4] + o, v);             // dens
+      }
     }
   }
   bad = wave_or(bad);

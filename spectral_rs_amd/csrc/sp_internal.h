@@ -27,6 +27,7 @@ template <class T> struct Geo {
   int nx, ny, pitch;       // pitch = nx + 2
   int delta, wall, lwall;  // wall = nx + 1 - delta (src/prtls.rs:153), lwall = delta + 1
   int periodic_x, checked, left_edge;
+  int lds_ok;              // dt * c < 1: a particle moves less than one cell per step
   T dt, c, csqinv;         // csqinv = 1 / (c * c) in Float (src/prtls.rs:265)
   T half_dt;               // 0.5 * dt in Float (src/prtls.rs:103)
 };
@@ -68,6 +69,8 @@ template <class T> void launch_update_ghosts(T* const* grids, int ngrids, int nx
 template <class T> void launch_binomial(T* const* grids, T* const* work, int ngrids, int nx, int ny, int passes,
                                         hipStream_t st);
 template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, hipStream_t st);
+// Hermitian projection of `count` spectra of real fields (see fields.hip).
+template <class T> void launch_hermitian(void* const* spec, int count, int nx, int ny, hipStream_t st);
 
 // FFT plan for one dimension: radices and an FP64-generated twiddle table.
 struct FftDim {

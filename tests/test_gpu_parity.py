@@ -89,13 +89,24 @@ def test_move_and_deposit_cells_bit_exact_and_deposit_within_tolerance(prec):
     absr.deposit_current(0)
     eps = np.finfo(ft).eps
     cnt = 9 * 30000 / (nx * ny) * 8  # generous bound on contributions per cell
+    # FP32 windows accumulate in fixed point with step 2^-e per chunk, where
+    # 2^e <= 2^29 / (25 m |q| c) and m is the largest per-cell count of the
+    # chunk (<= the global largest count used here); each of the <= 25 m
+    # contributions to a cell is off by at most half a step.
+    m = np.max(np.bincount((sp["iy"] * (nx + 2) + sp["ix"]).astype(np.int64)))
+    if prec == 4:
+        lsb_j = 2.0 ** -(29 - np.floor(np.log2(25 * m * 1.0 * TEST_SIM["c"])))
+        lsb_d = 2.0 ** -(29 - np.floor(np.log2(25 * m)))
+    else:
+        lsb_j = lsb_d = 0.0
     for name in ["jx", "jy", "jz"]:
         A = absr.grid(name)
         d = np.abs(g.grid(name) - r.grid(name))
-        assert np.all(d <= cnt * eps * A + 1e-30), (name, float(np.max(d / (A + 1e-30))))
+        bound = cnt * eps * A + 25 * m * 0.5 * lsb_j + eps * np.abs(r.grid(name))
+        assert np.all(d <= bound + 1e-30), (name, float(np.max(d / (bound + 1e-30))))
     for name in ["rho", "dens"]:
         d = np.abs(g.grid(name) - r.grid(name))
-        assert np.max(d) <= cnt * eps * np.max(r.grid("dens")), name
+        assert np.max(d) <= cnt * eps * np.max(r.grid("dens")) + 25 * m * 0.5 * lsb_d, name
     assert abs(g.grid("dens").astype(np.float64).sum() - 30000) < (0.05 if prec == 4 else 1e-8)
 
 
