@@ -69,12 +69,15 @@ def cpu_baseline(seconds_hint=15.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--size", default=None, help="override grid, e.g. 1024x256 (not the metric workload)")
     ap.add_argument("--sort-interval", type=int, default=20, help="steps between tile sorts of the particles")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="after the timed region, run one odd filter pass (its k_copy moves a known byte "
+                         "count) as the reference kernel for the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
@@ -106,6 +109,9 @@ def main():
     sim.sync()
     el = time.perf_counter() - t0
     tm = sim.timers(reset=True)
+    if args.calibrate:
+        sim.filter("jx", 1)
+        sim.sync()
     value = n * args.steps / el
     print(f"bench: {el:.3f} s, {value:.4g} particle-steps/s, phases ms {tm}", file=sys.stderr, flush=True)
 
@@ -115,6 +121,15 @@ def main():
     part_ms = tm["particles"] / args.steps
     traffic_alg = 48.0 * n + 44.0 * cells
     achieved = traffic_alg / (part_ms * 1e-3) / 1e9
+    # HBM bytes per pass measured by rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+    # workload (calibrated against a copy kernel), when that measurement exists.
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "c2_particles_traffic.json")
+    if not args.size and os.path.exists(tf):
+        with open(tf) as f:
+            t = json.load(f)
+        traffic = 1024.0 * t["species_launches_per_pass"] * (t["fetch_kib_per_species_launch"] * t["fetch_factor"]
+                                                             + t["write_kib_per_species_launch"] * t["write_factor"])
     out = {
         "metric": "particle-steps/s on 1 MI355X",
         "value": value,
@@ -133,7 +148,7 @@ def main():
                    "particles": n, "grid": [cfg["size_x"], cfg["size_y"]]},
         "phases_ms_per_step": {k: v / args.steps for k, v in tm.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_particles<float, PUSH> (fused push + move + deposit, both species per step)",
                      "bytes_per_launch_algorithmic": traffic_alg, "ms_per_launch": part_ms},
     }
