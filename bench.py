@@ -75,6 +75,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--size", default=None, help="override grid, e.g. 1024x256 (not the metric workload)")
     ap.add_argument("--sort-interval", type=int, default=20, help="steps between tile sorts of the particles")
+    ap.add_argument("--trace", type=int, default=0, help="print phase times every N steps (diagnostic)")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed region, run one odd filter pass (its k_copy moves a known byte "
                          "count) as the reference kernel for the FETCH_SIZE/WRITE_SIZE counters")
@@ -105,10 +106,25 @@ def main():
     sim.set_timing(True)
     sim.timers(reset=True)
     t0 = time.perf_counter()
-    sim.step(args.steps)
-    sim.sync()
+    if args.trace:  # phase times every `trace` steps (adds a sync per block of steps)
+        tm = {"dep": 0.0, "solve": 0.0, "push": 0.0, "particles": 0.0}
+        done = 0
+        while done < args.steps:
+            k = min(args.trace, args.steps - done)
+            sim.step(k)
+            sim.sync()
+            t = sim.timers(reset=True)
+            done += k
+            print(f"trace: steps {done - k}-{done}: " + " ".join(f"{n} {v / k:.3f} ms" for n, v in t.items()),
+                  file=sys.stderr, flush=True)
+            for n_ in tm:
+                tm[n_] += t[n_]
+    else:
+        sim.step(args.steps)
+        sim.sync()
     el = time.perf_counter() - t0
-    tm = sim.timers(reset=True)
+    if not args.trace:
+        tm = sim.timers(reset=True)
     if args.calibrate:
         sim.filter("jx", 1)
         sim.sync()

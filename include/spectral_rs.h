@@ -74,14 +74,21 @@ typedef struct sp_params {
   int32_t reserved[3];
 } sp_params;
 
+/* FP64 diagnostics; none has a reference counterpart.  Energies share one
+ * unit: with J and rho divided by dens in the solve (src/flds/mod.rs:80-84,
+ * :127), energy_e + energy_b + sum(energy_kin) is the conserved total. */
 typedef struct sp_diag {
-  double energy_e;      /* sum |E|^2 / 2 over the interior (FP64) */
-  double energy_b;      /* sum |B|^2 / 2 */
-  double energy_kin[8]; /* per species sum (gamma - 1) m c^2 */
-  double momentum[3];   /* sum m p over all species */
-  double gauss_pre;     /* pre-correction Gauss residual of the last solve */
-  double gauss_post;    /* post-correction residual (round-off level) */
-  int64_t steps;        /* loop iterations run so far */
+  double energy_e;       /* sum |E|^2 / 2 over the interior cells */
+  double energy_b;       /* sum |B|^2 / 2 */
+  double energy_kin[8];  /* species 0..7: m c^2 sum (gamma - 1) / dens */
+  double momentum[3];    /* all species: sum m p / dens (p = gamma v) */
+  /* Gauss law of the last field solve, relative RMS over the modes off k=0
+   * and off the Nyquist lines: |k.E + i rho| / |rho| before the correction
+   * (charge-continuity error of the deposit) and after it (round-off). */
+  double gauss_pre;
+  double gauss_post;
+  double gauss_post_max; /* largest gauss_post over every solve so far */
+  int64_t steps;         /* loop iterations run so far */
 } sp_diag;
 
 typedef struct sp_ctx sp_ctx;

@@ -38,7 +38,7 @@ class SpParams(ct.Structure):
 class SpDiag(ct.Structure):
     _fields_ = [("energy_e", ct.c_double), ("energy_b", ct.c_double), ("energy_kin", ct.c_double * 8),
                 ("momentum", ct.c_double * 3), ("gauss_pre", ct.c_double), ("gauss_post", ct.c_double),
-                ("steps", ct.c_int64)]
+                ("gauss_post_max", ct.c_double), ("steps", ct.c_int64)]
 
 
 class SpError(RuntimeError):
@@ -196,6 +196,12 @@ class GpuSim:
         return dict(dep=t[0], solve=t[1], push=t[2], particles=t[3])
 
     def diag(self):
+        """FP64 diagnostics as a dict (energies, momentum, Gauss residuals)."""
         d = SpDiag()
         self._chk(load().sp_get_diag(self.h, ct.byref(d)))
-        return d
+        ns = min(self.nspecies, 8)
+        return {"energy_e": d.energy_e, "energy_b": d.energy_b,
+                "energy_kin": [d.energy_kin[k] for k in range(ns)],
+                "momentum": [d.momentum[k] for k in range(3)],
+                "gauss_pre": d.gauss_pre, "gauss_post": d.gauss_post, "gauss_post_max": d.gauss_post_max,
+                "steps": d.steps}

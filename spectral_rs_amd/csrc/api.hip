@@ -58,6 +58,8 @@ struct sp_ctx {
   int* flags_h = nullptr;
   int* tile_count = nullptr;   // per-tile particle counts of the tile sort
   int* tile_cursor = nullptr;  // per-tile write positions
+  double* gauss_d = nullptr;   // Gauss-law sums of the last solve + block partials
+  double* diag_d = nullptr;    // energy / momentum sums of sp_get_diag
   std::vector<SpeciesBuf> species;
   std::string err;
   bool push_pending = false;
@@ -231,7 +233,7 @@ template <class T> void fields_update(sp_ctx* c) {
     // transforms of real fields: keep them exactly Hermitian (fields.hip)
     launch_hermitian<T>(b.spec, 7, nx, ny, c->st);
   }
-  launch_spectral_update<T>(c->spec, geo<T>(c), c->p.dens, c->st);  // mod.rs:105-278
+  launch_spectral_update<T>(c->spec, geo<T>(c), c->p.dens, c->gauss_d, c->st);  // mod.rs:105-278
   {  // mod.rs:280-307: 6 inverse transforms, spectral->spatial + ghosts fused
     FftBatch b{};
     const int ids[6] = {SP_BX, SP_BY, SP_BZ, SP_EX, SP_EY, SP_EZ};
@@ -245,6 +247,21 @@ template <class T> void fields_update(sp_ctx* c) {
     launch_fft_cols<T>(b, c->fy, nx, 1, 1.0, c->st);
     launch_fft_rows<T>(b, c->fx, ny, 1, LOAD_COMPLEX, STORE_REAL_PADDED, (double)norm, c->st);
   }
+}
+
+// FP64 sums for sp_get_diag into diag_d: [0] sum E^2, [1] sum B^2 over the
+// interior, then per species [sum (gamma - 1), sum px, sum py, sum pz].
+template <class T> void diag_sums(sp_ctx* c) {
+  const int nd = DIAG_FIXED + 4 * (int)c->species.size();
+  fill<double>(c->diag_d, 0.0, nd, c->st);
+  const int nx = c->p.size_x, ny = c->p.size_y;
+  launch_sumsq3<T>((const T*)c->grid[SP_EX], (const T*)c->grid[SP_EY], (const T*)c->grid[SP_EZ], nx, ny,
+                   c->diag_d + 0, c->st);
+  launch_sumsq3<T>((const T*)c->grid[SP_BX], (const T*)c->grid[SP_BY], (const T*)c->grid[SP_BZ], nx, ny,
+                   c->diag_d + 1, c->st);
+  const double csqinv = 1.0 / ((double)c->p.c * c->p.c);
+  for (size_t s = 0; s < c->species.size(); ++s)
+    launch_kinetic<T>(sdev<T>(c, c->species[s]), csqinv, c->diag_d + DIAG_FIXED + 4 * s, c->st);
 }
 
 // Tile-sorts every species (particle order only; no value changes).
@@ -413,6 +430,12 @@ int sp_create(const sp_params* params, sp_ctx** out) {
     CK(hipMalloc(&c->tile_cursor, ntiles * sizeof(int)));
   }
   CK(hipMalloc(&c->flags_d, sizeof(int)));
+  {
+    const size_t gn = gauss_doubles(p.size_x, p.size_y);
+    CK(hipMalloc(&c->gauss_d, gn * sizeof(double)));
+    CK(hipMemsetAsync(c->gauss_d, 0, gn * sizeof(double), c->st));
+    CK(hipMalloc(&c->diag_d, DIAG_DOUBLES * sizeof(double)));
+  }
   CK(hipHostMalloc((void**)&c->flags_h, sizeof(int), hipHostMallocDefault));
   CK(hipMemsetAsync(c->flags_d, 0, sizeof(int), c->st));
   CK(hipStreamSynchronize(c->st));
@@ -434,6 +457,8 @@ int sp_destroy(sp_ctx* c) {
   hipFree(c->flags_d);
   hipFree(c->tile_count);
   hipFree(c->tile_cursor);
+  hipFree(c->gauss_d);
+  hipFree(c->diag_d);
   if (c->flags_h) hipHostFree(c->flags_h);
   for (auto& tp : c->open_pairs) { hipEventDestroy(tp.a); hipEventDestroy(tp.b); }
   for (auto e : c->free_events) hipEventDestroy(e);
@@ -451,6 +476,7 @@ int sp_species_add(sp_ctx* c, double charge, double mass, int64_t n, const int32
     return fail(c, SP_EINVAL, "bad species arrays");
   if (!(mass != 0.0)) return fail(c, SP_EINVAL, "mass must be non-zero");
   if (n > INT32_MAX) return fail(c, SP_EINVAL, "species too large for int32 tags");
+  if ((int)c->species.size() >= MAX_SPECIES) return fail(c, SP_EINVAL, "too many species");
   int rc = flush(c);
   if (rc) return rc;
   std::vector<int32_t> cell(n), tag(n);
@@ -484,6 +510,7 @@ int sp_species_init(sp_ctx* c, double charge, double mass, double vth, uint64_t 
   if (rc) return rc;
   const int64_t n = (int64_t)c->p.dens * (c->p.size_x - 2 * c->p.delta) * c->p.size_y;  // src/lib.rs:258-260
   if (n > INT32_MAX) return fail(c, SP_EINVAL, "species too large for int32 tags");
+  if ((int)c->species.size() >= MAX_SPECIES) return fail(c, SP_EINVAL, "too many species");
   SpeciesBuf s;
   s.charge = charge;
   s.mass = mass;
@@ -664,8 +691,8 @@ int sp_fft2d(sp_ctx* c, int spec, int inverse) {
 
 int sp_spectral_update(sp_ctx* c) {
   ENTRY_PRELUDE(c);
-  if (c->f64) launch_spectral_update<double>(c->spec, geo<double>(c), c->p.dens, c->st);
-  else launch_spectral_update<float>(c->spec, geo<float>(c), c->p.dens, c->st);
+  if (c->f64) launch_spectral_update<double>(c->spec, geo<double>(c), c->p.dens, c->gauss_d, c->st);
+  else launch_spectral_update<float>(c->spec, geo<float>(c), c->p.dens, c->gauss_d, c->st);
   HIPCHK(c, hipGetLastError());
   return SP_OK;
 }
@@ -723,7 +750,33 @@ int sp_get_timers(sp_ctx* c, double* ms, int reset) {
 
 int sp_get_diag(sp_ctx* c, sp_diag* out) {
   if (!c || !out) return SP_EINVAL;
+  int rc = flush(c);
+  if (rc) return rc;
   std::memset(out, 0, sizeof(*out));
+  const int ns = (int)c->species.size();
+  const int nd = DIAG_FIXED + 4 * ns;
+  std::vector<double> h((size_t)nd + 4);
+  if (c->f64) diag_sums<double>(c);
+  else diag_sums<float>(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->diag_d, (size_t)nd * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(h.data() + nd, c->gauss_d, 4 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  rc = sync_check(c);
+  if (rc) return rc;
+  const double dens = (double)c->p.dens, cc = c->p.c;
+  out->energy_e = 0.5 * h[0];
+  out->energy_b = 0.5 * h[1];
+  for (int s = 0; s < ns; ++s) {
+    const double m = c->species[s].mass;
+    const double* k = h.data() + DIAG_FIXED + 4 * s;
+    if (s < 8) out->energy_kin[s] = m * cc * cc * k[0] / dens;
+    for (int d = 0; d < 3; ++d) out->momentum[d] += m * k[1 + d] / dens;
+  }
+  const double* g = h.data() + nd;
+  const double rho2 = g[2];
+  out->gauss_pre = rho2 > 0 ? std::sqrt(g[0] / rho2) : std::sqrt(g[0]);
+  out->gauss_post = rho2 > 0 ? std::sqrt(g[1] / rho2) : std::sqrt(g[1]);
+  out->gauss_post_max = g[3];
   out->steps = c->steps;
   return SP_OK;
 }

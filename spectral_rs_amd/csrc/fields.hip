@@ -191,11 +191,23 @@ struct SpecPtrs {
 // (src/flds/wave_num.rs:22-37), so they equal its tables bit for bit.  Every
 // reference loop in this range is pointwise in the mode, except the saves of
 // Ex[0], Ey[0] around the Gauss correction, which thread 0 handles locally.
+// It also accumulates, in FP64, the Gauss-law residual |k.E + i rho|^2 before
+// and after the correction and |rho|^2 over the modes that are neither k=0 nor
+// on a Nyquist line (gauss[0..2]); the reference has no such check.
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
 template <class T>
-__global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T cdt, T ckc, T twopi_x, T twopi_y) {
+__global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T cdt, T ckc, T twopi_x, T twopi_y,
+                                                  double* gauss) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= (int64_t)nx * ny) return;
+  double g_pre = 0, g_post = 0, g_rho = 0;
+  if (i < (int64_t)nx * ny) {
   const int row = (int)(i / nx), col = (int)(i - (int64_t)row * nx);
+  const bool nyq = col == nx / 2 || row == ny / 2;
   T kx = T(col);
   if (col >= nx / 2 + 1) kx -= T(nx);
   kx *= twopi_x;
@@ -237,13 +249,20 @@ __global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T
     const Cx<T> tmp{d.re * kn, d.im * kn};
     ex = sub(ex, Cx<T>{tmp.re * kx, tmp.im * kx});
     ey = sub(ey, Cx<T>{tmp.re * ky, tmp.im * ky});
+    if (!nyq) {
+      const double ar = (double)kx * ex.re + (double)ky * ey.re - (double)rho.im;
+      const double ai = (double)kx * ex.im + (double)ky * ey.im + (double)rho.re;
+      g_pre = (double)d.re * d.re + (double)d.im * d.im;
+      g_post = ar * ar + ai * ai;
+      g_rho = (double)rho.re * rho.re + (double)rho.im * rho.im;
+    }
   }
   // mod.rs:211-225: B_wrk to the next half step.
   Cx<T> wx = sub(bx, mul(Cx<T>{icdt.re * ky, icdt.im * ky}, ez));
   Cx<T> wy = add(by, mul(Cx<T>{icdt.re * kx, icdt.im * kx}, ez));
   Cx<T> wz = add(bz, mul(icdt, sub(scl(ky, ex), scl(kx, ey))));
   // mod.rs:230-266: Nyquist column Nx/2 and row Ny/2 of B_wrk and E.
-  if (col == nx / 2 || row == ny / 2) {
+  if (nyq) {
     const Cx<T> z{T(0), T(0)};
     wx = wy = wz = z;
     ex = ey = ez = z;
@@ -258,6 +277,48 @@ __global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T
   st((T*)sp.p[SP_BX], i, scl(T(0.5), add(bx, wx)));
   st((T*)sp.p[SP_BY], i, scl(T(0.5), add(by, wy)));
   st((T*)sp.p[SP_BZ], i, scl(T(0.5), add(bz, wz)));
+  }
+  if (gauss) {  // block partials in a fixed order (reproducible sums)
+    __shared__ double red[3][4];
+    g_pre = wave_sum_d(g_pre);
+    g_post = wave_sum_d(g_post);
+    g_rho = wave_sum_d(g_rho);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      red[0][w] = g_pre;
+      red[1][w] = g_post;
+      red[2][w] = g_rho;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3)
+      gauss[(int64_t)blockIdx.x * 3 + threadIdx.x] =
+          ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+  }
+}
+
+// Sums the block partials of k_spectral (one block, fixed order) into
+// out[0..2] = (pre, post, |rho|^2) of this solve, and keeps in out[3] the
+// largest relative post-correction residual sqrt(post / |rho|^2) seen so far.
+__global__ void __launch_bounds__(256) k_gauss_reduce(const double* part, int nblk, double* out) {
+  __shared__ double red[3][4];
+  double s[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < nblk; b += 256)
+    for (int k = 0; k < 3; ++k) s[k] += part[(int64_t)b * 3 + k];
+  const int w = threadIdx.x >> 6;
+  for (int k = 0; k < 3; ++k) {
+    s[k] = wave_sum_d(s[k]);
+    if ((threadIdx.x & 63) == 0) red[k][w] = s[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[3];
+    for (int k = 0; k < 3; ++k) t[k] = ((red[k][0] + red[k][1]) + red[k][2]) + red[k][3];
+    out[0] = t[0];
+    out[1] = t[1];
+    out[2] = t[2];
+    const double rel = t[2] > 0 ? sqrt(t[1] / t[2]) : sqrt(t[1]);
+    out[3] = fmax(out[3], rel);
+  }
 }
 
 // Hermitian projection X[k] <- (X[k] + conj(X[-k])) / 2, X[-k] <- conj(X[k])
@@ -292,7 +353,8 @@ template <class T> void launch_hermitian(void* const* spec, int count, int nx, i
   hipLaunchKernelGGL(k_hermitian<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sp, count, nx, ny);
 }
 
-template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, hipStream_t st) {
+template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
+                                               hipStream_t st) {
   SpecPtrs sp{};
   for (int i = 0; i < 14; ++i) sp.p[i] = spec[i];
   const T PI = (T)3.14159265358979323846264338327950288;
@@ -301,8 +363,10 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   const T ckc = g.dt / T(dens);  // mod.rs:127
   const T cdt = g.dt * g.c;      // mod.rs:128
   const int64_t n = (int64_t)g.nx * g.ny;
-  hipLaunchKernelGGL(k_spectral<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sp, g.nx, g.ny, cdt, ckc,
-                     twopi_x, twopi_y);
+  const int nblk = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(k_spectral<T>, dim3((unsigned)nblk), dim3(256), 0, st, sp, g.nx, g.ny, cdt, ckc, twopi_x,
+                     twopi_y, gauss ? gauss + GAUSS_PARTIALS : nullptr);
+  if (gauss) hipLaunchKernelGGL(k_gauss_reduce, dim3(1), dim3(256), 0, st, gauss + GAUSS_PARTIALS, nblk, gauss);
 }
 
 #define INST(T)                                                                                 \
@@ -310,7 +374,7 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   template void launch_fold_ghosts<T>(T* const*, int, int, int, hipStream_t);                   \
   template void launch_update_ghosts<T>(T* const*, int, int, int, hipStream_t);                 \
   template void launch_binomial<T>(T* const*, T* const*, int, int, int, int, hipStream_t);      \
-  template void launch_spectral_update<T>(void* const*, const Geo<T>&, int, hipStream_t);            \
+  template void launch_spectral_update<T>(void* const*, const Geo<T>&, int, double*, hipStream_t);            \
   template void launch_hermitian<T>(void* const*, int, int, int, hipStream_t);
 INST(float)
 INST(double)

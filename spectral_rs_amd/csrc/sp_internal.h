@@ -68,7 +68,25 @@ template <class T> void launch_fold_ghosts(T* const* grids, int ngrids, int nx, 
 template <class T> void launch_update_ghosts(T* const* grids, int ngrids, int nx, int ny, hipStream_t st);
 template <class T> void launch_binomial(T* const* grids, T* const* work, int ngrids, int nx, int ny, int passes,
                                         hipStream_t st);
-template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, hipStream_t st);
+// `gauss` (may be null) receives the FP64 Gauss-law sums of this solve:
+// gauss[0..2] = sum |k.E + i rho|^2 before / after the correction and
+// sum |rho|^2; gauss[3] keeps the largest relative post-correction residual
+// (never cleared here); gauss[GAUSS_PARTIALS..] holds 3 partials per block of
+// nx*ny/256 blocks.
+constexpr int GAUSS_PARTIALS = 8;
+// Species per context, and the layout of the diagnostic sums buffer:
+// DIAG_FIXED field sums, then 4 per species.
+constexpr int MAX_SPECIES = 64;
+constexpr int DIAG_FIXED = 4;
+constexpr int DIAG_DOUBLES = DIAG_FIXED + 4 * MAX_SPECIES;
+inline size_t gauss_doubles(int nx, int ny) { return GAUSS_PARTIALS + 3 * (((size_t)nx * ny + 255) / 256); }
+template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
+                                               hipStream_t st);
+// FP64 diagnostics: out[0] += sum over the interior of a^2 + b^2 + c^2;
+// kinetic: out[0] += sum (gamma - 1), out[1..3] += sum p.
+template <class T> void launch_sumsq3(const T* a, const T* b, const T* c, int nx, int ny, double* out,
+                                      hipStream_t st);
+template <class T> void launch_kinetic(const SpeciesDev<T>& s, double csqinv, double* out, hipStream_t st);
 // Hermitian projection of `count` spectra of real fields (see fields.hip).
 template <class T> void launch_hermitian(void* const* spec, int count, int nx, int ny, hipStream_t st);
 

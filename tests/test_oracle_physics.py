@@ -249,3 +249,33 @@ def test_f32_and_f64_oracles_agree_on_a_shock_step():
     e4, e8 = out[4][0], out[8][0]
     assert np.max(np.abs(e4 - e8)) <= 1e-4 * max(np.max(np.abs(e8)), 1e-3)
     assert np.array_equal(out[4][1]["ix"], out[8][1]["ix"])
+
+
+def test_langmuir_energy_exchange_and_conservation():
+    # Cold plasma oscillation at omega_pe = 1 (J/dens normalisation, m = |q| = 1):
+    # field and kinetic energy trade places every quarter period, and the total
+    # with the kinetic energy centred in time (p lives on half steps) stays
+    # within 1% over two plasma periods.
+    from tests.helpers import LANGMUIR, energies, langmuir_species
+    s = O.RefSim(prec=8, **LANGMUIR)
+    sp = langmuir_species()
+    for q, m, a in sp:
+        s.add_species(q, m, **a)
+    s.prestep()
+    masses = [m for _, m, _ in sp]
+    prev = energies(s, masses, LANGMUIR["dens"], LANGMUIR["c"])
+    k0 = sum(prev[2])
+    tot, we = [], []
+    for n in range(128):
+        s.step(1)
+        e = energies(s, masses, LANGMUIR["dens"], LANGMUIR["c"])
+        tot.append(e[0] + e[1] + 0.5 * (sum(e[2]) + sum(prev[2])))
+        we.append(e[0])
+        prev = e
+    tot, we = np.array(tot), np.array(we)
+    assert we.max() > 0.9 * k0 and we.min() < 0.05 * k0
+    assert (tot.max() - tot.min()) < 0.01 * tot.mean()
+    assert abs(tot.mean() - k0) < 0.03 * k0
+    # energy period = half the plasma period 2 pi / (omega_pe dt) = 62.8 steps
+    up = np.flatnonzero((we[1:] > 0.5 * k0) & (we[:-1] <= 0.5 * k0))
+    assert np.diff(up).mean() == pytest.approx(np.pi / LANGMUIR["dt"], rel=0.05)
