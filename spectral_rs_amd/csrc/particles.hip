@@ -92,6 +92,66 @@ template <class T> __device__ __forceinline__ bool inside(int ix, int iy, const 
   return ix >= 1 && ix <= g.nx && iy >= 1 && iy <= g.ny;
 }
 
+// Raw buffer resources of the six field grids (Ex Ey Ez Bx By Bz).  The
+// gather then needs one address register per particle: the column offset
+// is an instruction immediate and the row offset a scalar register, where
+// global loads would need a 64-bit address add per stencil row and field.
+// Word 3 0x00020000 is the gfx9 raw-buffer format; reads past `bytes`
+// return 0 (the callers only read inside the padded grid).
+struct FieldBufs {
+  __amdgpu_buffer_rsrc_t r[6];
+  int row_bytes;
+};
+__device__ __forceinline__ FieldBufs field_bufs(const GridSet& gs, const Geo<float>& g) {
+  FieldBufs b;
+  const int bytes = (g.ny + 2) * g.pitch * (int)sizeof(float);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) b.r[k] = __builtin_amdgcn_make_buffer_rsrc(gs.g[k], (short)0, bytes, 0x00020000);
+  b.row_bytes = g.pitch * (int)sizeof(float);
+  return b;
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// src/prtls.rs:351-411: each field is the ordered sum w0 a0 + w1 a1 + ... + w8 a8
+// over the 3x3 stencil in row-major order around (ix, iy).
+template <class T>
+__device__ __forceinline__ void gather6(const GridSet& gs, const Geo<T>& g, int ix, int iy, const T (&w)[9],
+                                        T (&f)[6]) {
+  const int P = g.pitch;
+  const int base = (iy - 1) * P + ix - 1;
+  const int off[9] = {0, 1, 2, P, P + 1, P + 2, 2 * P, 2 * P + 1, 2 * P + 2};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const T* __restrict__ a = (const T*)gs.g[k] + base;
+    T s = w[0] * a[off[0]];
+#pragma unroll
+    for (int m = 1; m < 9; ++m) s += w[m] * a[off[m]];
+    f[k] = s;
+  }
+}
+__device__ __forceinline__ void gather6(const FieldBufs& fb, const Geo<float>& g, int ix, int iy,
+                                        const float (&w)[9], float (&f)[6]) {
+  // sp_create caps FP32 grids at 8192 x 8192 (the FFT length limit), so a
+  // grid has < 2^29 bytes; the mask states that and tells the compiler that vo + 8
+  // cannot wrap, so the column offsets fold into the load's immediate and each
+  // stencil row becomes one 3-dword load
+  const int vo = (((iy - 1) * g.pitch + ix - 1) * (int)sizeof(float)) & 0x1FFFFFFC;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    float a[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a[3 * r + c] = bload(fb.r[k], vo + 4 * c, r * fb.row_bytes);
+    float s = w[0] * a[0];
+#pragma unroll
+    for (int m = 1; m < 9; ++m) s += w[m] * a[m];
+    f[k] = s;
+  }
+}
+
 // Deposit target of one workgroup: a window [x0, x0+w) x [y0, y0+h) of the
 // padded grids held in LDS (planes Jx Jy Jz rho dens, w*h cells each), plus
 // the global grids for stencils that leave the window.  use_lds == false sends
@@ -110,11 +170,12 @@ template <class T> __device__ __forceinline__ void glb_add(glb_ptr<T> p, T v) {
 
 // LDS accumulation.  Float atomics in LDS run ~40x slower than integer ones
 // on gfx950, so FP32 windows accumulate in int32 fixed point: a contribution
-// x becomes rint(x * 2^e), with e chosen per chunk and plane so that no
+// x becomes round(x * 2^e), with e chosen per chunk and plane so that no
 // window cell can overflow (see k_particles).  Integer sums are exact and
 // order-independent; the only rounding is the per-contribution quantisation
-// (|error| <= 2^-(e+1)) and the conversion of the window sum at the flush.
-// FP64 windows (the dprec precision) accumulate with native f64 atomics.
+// (|error| <= 2^-e, see rint_half_up) and the conversion of the window sum at
+// the flush.  FP64 windows (the dprec precision) accumulate with native f64
+// atomics.
 template <class T> struct Acc;
 template <> struct Acc<float> {
   using type = int;
@@ -125,37 +186,61 @@ template <> struct Acc<double> {
   static constexpr bool fixed = false;
 };
 
+// One workgroup's chunk: NT threads, K particles, deposit windows of at most
+// WMAX cells per plane (planes at fixed strides of WMAX, so a plane offset is
+// an instruction immediate).
+#ifndef SP_FP32_CHUNK
+#define SP_FP32_CHUNK 2048
+#endif
+#ifndef SP_WINDOW_COPIES
+#define SP_WINDOW_COPIES 16
+#endif
+#ifndef SP_PARTICLE_WAVES_PER_EU
+#define SP_PARTICLE_WAVES_PER_EU 1
+#endif
+template <class T> struct ChunkShape {
+  static constexpr int NT = 512;
+  static constexpr int K = sizeof(T) == 4 ? SP_FP32_CHUNK : 1024;
+  static constexpr int WMAX = sizeof(T) == 4 ? 2000 : 1000;  // 2 workgroups fit one CU's LDS
+  static constexpr int NCOPY = SP_WINDOW_COPIES;  // most private window copies
+};
+
+// floor(x + 0.5) in one instruction (v_cvt_rpi_i32_f32).  The hardware adds
+// 0.5 in FP32, so for |x| >= 2^22 the result can be off by one from the
+// exact round-half-up; either way |result - x| <= 1.
+__device__ __forceinline__ int rint_half_up(float x) {
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 template <class T> struct Sink {
   lds_ptr<typename Acc<T>::type> lds;
   bool use_lds;
   int x0, y0, w, h, wh;
   int pitch;
   T scale[4];  // fixed point: 2^e per LDS plane (Jx Jy Jz dens)
+  T qsj;       // charge * scale[0] (fixed point)
   glb_ptr<T> g[5];
   __device__ __forceinline__ bool holds(int ix, int iy) const {
     return use_lds && ix - 1 >= x0 && ix + 1 < x0 + w && iy - 1 >= y0 && iy + 1 < y0 + h;
   }
+  // window cell of the stencil's corner (ix - 1, iy - 1)
+  __device__ __forceinline__ int corner(int ix, int iy) const { return (iy - 1 - y0) * w + (ix - 1 - x0); }
 };
 
 // Adds w[k] * v to the 3x3 stencil around (ix, iy) of one plane
 // (0..2 = Jx Jy Jz, 3 = dens in LDS; 0..4 = Jx Jy Jz rho dens globally).
+// The FP32 LDS windows use the separable forms below instead.
 template <class T>
 __device__ __forceinline__ void add9(const Sink<T>& k, int lplane, int gplane, int ix, int iy, bool in_lds,
                                      const T (&w)[9], T v) {
   if (in_lds) {
-    auto a = k.lds + lplane * k.wh + (iy - 1 - k.y0) * k.w + (ix - 1 - k.x0);
+    auto a = k.lds + lplane * ChunkShape<T>::WMAX + k.corner(ix, iy);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const T x = w[3 * r + c] * v;
-        if constexpr (Acc<T>::fixed) {
-          __hip_atomic_fetch_add(a + r * k.w + c, __float2int_rn(x * k.scale[lplane]), __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else {
-          lds_add<T>(a + r * k.w + c, x);
-        }
-      }
+      for (int c = 0; c < 3; ++c) lds_add<typename Acc<T>::type>(a + r * k.w + c, w[3 * r + c] * v);
   } else {
     glb_ptr<T> a = k.g[gplane] + (iy - 1) * k.pitch + (ix - 1);
 #pragma unroll
@@ -165,17 +250,56 @@ __device__ __forceinline__ void add9(const Sink<T>& k, int lplane, int gplane, i
   }
 }
 
+// The TSC weights as separate x and y factors: w[3r + c] = wy[r] * wx[c] up
+// to rounding (the reference forms each w as one left-to-right product
+// chain, tsc above).  Deposits are compared with a tolerance, so the FP32
+// windows use this cheaper form: 9 products per component instead of 18.
+__device__ __forceinline__ void tsc_sep(float dx, float dy, float (&wx)[3], float (&wy)[3]) {
+  const float h = 0.5f, q = 0.75f;
+  const float a = h - dx, b = h + dx, c = h - dy, d = h + dy;
+  wx[0] = (h * a) * a;
+  wx[1] = q - dx * dx;
+  wx[2] = (h * b) * b;
+  wy[0] = (h * c) * c;
+  wy[1] = q - dy * dy;
+  wy[2] = (h * d) * d;
+}
+
+__device__ __forceinline__ void lds_addi(lds_ptr<int> p, int v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Sim::deposit_current (src/lib.rs:266-393): v = q p / gamma, J += w v on the
-// 3x3 stencil around (ix, iy), ghosts included.
+// 3x3 stencil around (ix, iy), ghosts included.  `inv` = 1 / psa.
 template <class T>
-__device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px, T py, T pz, T psa, T q,
+__device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px, T py, T pz, T inv, T q,
                                                 const Geo<T>& g, const Sink<T>& k, int& bad) {
   if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
-  const T inv = T(1) / psa;
+  const bool in = k.holds(ix, iy);
+  if constexpr (Acc<T>::fixed) {
+    if (in) {  // fixed point: contributions (wy v 2^e) wx, rounded
+      constexpr int W = ChunkShape<T>::WMAX;
+      const T f = k.qsj * inv;
+      const T vs[3] = {px * f, py * f, pz * f};
+      T wx[3], wy[3];
+      tsc_sep(dx, dy, wx, wy);
+      lds_ptr<int> a = k.lds + k.corner(ix, iy);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        lds_ptr<int> row = a + r * k.w;
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) {
+          const T t = wy[r] * vs[pl];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) lds_addi(row + pl * W + c, rint_half_up(t * wx[c]));
+        }
+      }
+      return;
+    }
+  }
   const T vx = q * px * inv, vy = q * py * inv, vz = q * pz * inv;
   T w[9];
   tsc(dx, dy, w);
-  const bool in = k.holds(ix, iy);
   add9(k, 0, 0, ix, iy, in, w, vx);
   add9(k, 1, 1, ix, iy, in, w, vy);
   add9(k, 2, 2, ix, iy, in, w, vz);
@@ -186,11 +310,27 @@ template <class T>
 __device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, T q, const Geo<T>& g, const Sink<T>& k,
                                                 int& bad) {
   if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
-  T w[9];
-  tsc(dx, dy, w);
   // Within a window only dens = sum w is accumulated; the flush adds q * sum w
   // to rho (one species per pass; w * q is exact for |q| = 1).
-  if (k.holds(ix, iy)) {
+  const bool in = k.holds(ix, iy);
+  if constexpr (Acc<T>::fixed) {
+    if (in) {
+      constexpr int W = ChunkShape<T>::WMAX;
+      T wx[3], wy[3];
+      tsc_sep(dx, dy, wx, wy);
+      lds_ptr<int> a = k.lds + 3 * W + k.corner(ix, iy);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const T t = wy[r] * k.scale[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) lds_addi(a + r * k.w + c, rint_half_up(t * wx[c]));
+      }
+      return;
+    }
+  }
+  T w[9];
+  tsc(dx, dy, w);
+  if (in) {
     add9(k, 3, 4, ix, iy, true, w, T(1));
   } else {
     add9(k, 3, 3, ix, iy, false, w, q);
@@ -200,24 +340,15 @@ __device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, T q,
 
 // Prtl::boris_push for one particle (src/prtls.rs:306-440): ordered 9-term
 // TSC gather of E and B, then the relativistic Boris rotation.
-template <class T>
-__device__ __forceinline__ void boris(int ix, int iy, T dx, T dy, T& px, T& py, T& pz, const GridSet& gs,
+// `Fields` is a GridSet (global loads) or, for FP32, FieldBufs.
+template <class T, class Fields>
+__device__ __forceinline__ void boris(int ix, int iy, T dx, T dy, T& px, T& py, T& pz, const Fields& fs,
                                       const Geo<T>& g, T alpha, T beta, int& bad) {
   if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
   T w[9];
   tsc(dx, dy, w);
-  const int P = g.pitch;
-  const int base = (iy - 1) * P + ix - 1;
-  const int off[9] = {0, 1, 2, P, P + 1, P + 2, 2 * P, 2 * P + 1, 2 * P + 2};
   T f[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const T* __restrict__ a = (const T*)gs.g[k] + base;  // Ex Ey Ez Bx By Bz
-    T s = w[0] * a[off[0]];
-#pragma unroll
-    for (int m = 1; m < 9; ++m) s += w[m] * a[off[m]];
-    f[k] = s;
-  }
+  gather6(fs, g, ix, iy, w, f);  // Ex Ey Ez Bx By Bz
   const T ext = f[0] * beta, eyt = f[1] * beta, ezt = f[2] * beta;
   T bxt = f[3] * alpha, byt = f[4] * alpha, bzt = f[5] * alpha;
   const T ux = px + ext, uy = py + eyt, uz = pz + ezt;
@@ -243,9 +374,10 @@ template <class T>
 __device__ __forceinline__ void move_and_deposit(int& ix, int& iy, T& dx, T& dy, T& px, T py, T pz, T q,
                                                  const Geo<T>& g, const Sink<T>& k, int& bad) {
   const T psa = gamma_of(px, py, pz, g.csqinv);
-  const T c1 = g.half_dt * (T(1) / psa);
+  const T inv = T(1) / psa;
+  const T c1 = g.half_dt * inv;
   move_half(ix, iy, dx, dy, px, py, c1, g, bad);
-  deposit_current(ix, iy, dx, dy, px, py, pz, psa, q, g, k, bad);
+  deposit_current(ix, iy, dx, dy, px, py, pz, inv, q, g, k, bad);
   move_half(ix, iy, dx, dy, px, py, c1, g, bad);
   deposit_density(ix, iy, dx, dy, q, g, k, bad);
 }
@@ -273,18 +405,15 @@ __device__ __forceinline__ int wave_or(int v) {
 //     optionally the Boris push of the previous step runs first (PUSH);
 //  4. state goes back through LDS with coalesced stores and the window is
 //     added to HBM with one atomic per non-zero cell and grid.
-// Particles whose stencil leaves the window (periodic wrap, unsorted input)
-// and chunks whose window would not fit use global atomics directly, so the
-// result never depends on the particle order; only the speed does.
-template <class T> struct ChunkShape {
-  static constexpr int NT = 512;
-  static constexpr int K = sizeof(T) == 4 ? 2048 : 1024;
-  static constexpr int WMAX = sizeof(T) == 4 ? 1536 : 768;
-};
-
+// When the box of the whole chunk would not fit, the window covers the
+// particles near the chunk's middle particle.  Particles whose stencil leaves
+// the window (periodic wrap, unsorted input) use global atomics directly, so
+// the result never depends on the particle order; only the speed does.
 template <class T, bool PUSH>
-__global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SP_PARTICLE_WAVES_PER_EU)))
+k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   constexpr int NT = ChunkShape<T>::NT, K = ChunkShape<T>::K, WMAX = ChunkShape<T>::WMAX, PPT = K / NT;
+  constexpr int NCOPY = ChunkShape<T>::NCOPY;
   using A = typename Acc<T>::type;
   __shared__ int32_t s_cell[K];
   __shared__ T s_dx[K], s_dy[K], s_px[K], s_py[K], s_pz[K];
@@ -341,6 +470,52 @@ __global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, Gr
   k.y0 = max(s_box[2] - 2, 0);
   k.w = min(s_box[1] + 2, g.nx + 1) - k.x0 + 1;
   k.h = min(s_box[3] + 2, g.ny + 1) - k.y0 + 1;
+  if (g.lds_ok && cnt > 0 && k.w * k.h > WMAX) {
+    // A few particles far from the rest (crossed the periodic y boundary since
+    // the last sort, or a chunk that spans the end of the tile order): box
+    // only the particles within RBOX cells of the chunk's middle particle;
+    // the others miss the window and deposit with global atomics.
+    constexpr int RBOX = 16;
+    const int32_t mc = s_cell[cnt / 2];
+    const int mx = cell_ix(mc), my = cell_iy(mc);
+    __syncthreads();  // everyone has read the full box
+    if (tid == 0) {
+      s_box[0] = INT_MAX;
+      s_box[1] = INT_MIN;
+      s_box[2] = INT_MAX;
+      s_box[3] = INT_MIN;
+    }
+    __syncthreads();
+    int a0 = INT_MAX, a1 = INT_MIN, b0 = INT_MAX, b1 = INT_MIN;
+    for (int i = tid; i < cnt; i += NT) {
+      const int32_t c = s_cell[i];
+      const int ix = cell_ix(c), iy = cell_iy(c);
+      if (abs(ix - mx) <= RBOX && abs(iy - my) <= RBOX) {
+        a0 = min(a0, ix);
+        a1 = max(a1, ix);
+        b0 = min(b0, iy);
+        b1 = max(b1, iy);
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a0 = min(a0, __shfl_xor(a0, off));
+      a1 = max(a1, __shfl_xor(a1, off));
+      b0 = min(b0, __shfl_xor(b0, off));
+      b1 = max(b1, __shfl_xor(b1, off));
+    }
+    if ((tid & 63) == 0) {
+      atomicMin(&s_box[0], a0);
+      atomicMax(&s_box[1], a1);
+      atomicMin(&s_box[2], b0);
+      atomicMax(&s_box[3], b1);
+    }
+    __syncthreads();
+    k.x0 = max(s_box[0] - 2, 0);
+    k.y0 = max(s_box[2] - 2, 0);
+    k.w = min(s_box[1] + 2, g.nx + 1) - k.x0 + 1;
+    k.h = min(s_box[3] + 2, g.ny + 1) - k.y0 + 1;
+  }
   // g.lds_ok: c dt < 1, i.e. no particle moves more than one cell per step
   const bool use_lds = g.lds_ok && cnt > 0 && k.w > 0 && k.h > 0 && k.w * k.h <= WMAX;
   k.wh = use_lds ? k.w * k.h : 0;
@@ -353,8 +528,9 @@ __global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, Gr
       __syncthreads();
       for (int i = tid; i < cnt; i += NT) {
         const int32_t c = s_cell[i];
-        __hip_atomic_fetch_add(cntg + (cell_iy(c) - k.y0) * k.w + (cell_ix(c) - k.x0), 1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        const int cx = cell_ix(c) - k.x0, cy = cell_iy(c) - k.y0;
+        if (cx >= 0 && cx < k.w && cy >= 0 && cy < k.h)  // particles outside deposit globally
+          __hip_atomic_fetch_add(cntg + cy * k.w + cx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       __syncthreads();
       int m = 0;
@@ -369,11 +545,28 @@ __global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, Gr
     // 2^e with e = 30 - ceil(log2(bound)): sums stay below 2^30
     k.scale[0] = k.scale[1] = k.scale[2] = ldexpf(1.0f, 30 - ilogbf(bj) - 1);
     k.scale[3] = ldexpf(1.0f, 30 - ilogbf(m25) - 1);
+    k.qsj = (float)s.charge * k.scale[0];
   }
-  for (int i = tid; i < 4 * k.wh; i += NT) s_acc[i] = A(0);
+  // Private copies of the window: lane l of each wave deposits into copy
+  // l mod nc, so lanes whose particles share a cell mostly hit different
+  // addresses (same-address LDS atomics serialise).  Small windows (dense
+  // chunks) get more copies; the copy stride is odd so that one cell of
+  // consecutive copies falls into different banks.
+  int cs = k.wh | 1, nc = 1;
+  if (cs <= WMAX) nc = min(NCOPY, WMAX / cs);
+  else cs = k.wh;
+  k.lds = (lds_ptr<A>)s_acc + ((tid & 63) % nc) * cs;
+  for (int i = tid; i < nc * cs; i += NT) {
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl) s_acc[pl * WMAX + i] = A(0);
+  }
   __syncthreads();
 
   int bad = 0;
+  const auto fields = [&] {
+    if constexpr (sizeof(T) == 4) return field_bufs(gs, g);
+    else return gs;
+  }();
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     const int p = ((tid + NT * j) * 33) & (K - 1);
@@ -382,7 +575,7 @@ __global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, Gr
       int ix = cell_ix(c), iy = cell_iy(c);
       T dx = s_dx[p], dy = s_dy[p], px = s_px[p], py = s_py[p], pz = s_pz[p];
       if (PUSH) {
-        boris(ix, iy, dx, dy, px, py, pz, gs, g, s.alpha, s.beta, bad);
+        boris(ix, iy, dx, dy, px, py, pz, fields, g, s.alpha, s.beta, bad);
         if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
         s_py[p] = py;
         s_pz[p] = pz;
@@ -410,7 +603,8 @@ __global__ void __launch_bounds__(512) k_particles(SpeciesDev<T> s, Geo<T> g, Gr
     const int64_t o = (int64_t)(k.y0 + r) * g.pitch + k.x0 + cc;
 #pragma unroll
     for (int pl = 0; pl < 4; ++pl) {
-      const A a = s_acc[pl * k.wh + i];
+      A a = s_acc[pl * WMAX + i];
+      for (int c = 1; c < nc; ++c) a += s_acc[pl * WMAX + c * cs + i];
       if (a == A(0)) continue;
       T v;
       if constexpr (Acc<T>::fixed) v = (T)a * (T(1) / k.scale[pl]);
@@ -497,14 +691,16 @@ __global__ void __launch_bounds__(256) k_init(SpeciesDev<T> s, Geo<T> g, int den
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // ------------------------------------------------------------- tile sort
-// Counting sort of a species by tile (TILE_W x TILE_H cells, tiles in
-// row-major order); the order inside a tile is whatever the scatter's atomics
-// give.  The key of a particle is a pure function of its cell, so the keys
-// are bit-exact; only the order within a tile is not reproducible.
+// Counting sort of a species by tile (TILE_W x TILE_H cells, tile rows in
+// serpentine order: even rows left to right, odd rows right to left, so
+// consecutive tiles are always neighbours); the order inside a tile is
+// whatever the scatter's atomics give.  The key of a particle is a pure
+// function of its cell, so the keys are bit-exact; only the order within a
+// tile is not reproducible.
 __device__ __forceinline__ int tile_of(int32_t c, int ntx, int nty) {
   const int tx = min(max((cell_ix(c) - 1) >> TILE_W_LOG, 0), ntx - 1);
   const int ty = min(max((cell_iy(c) - 1) >> TILE_H_LOG, 0), nty - 1);
-  return ty * ntx + tx;
+  return ty * ntx + ((ty & 1) ? ntx - 1 - tx : tx);
 }
 
 // Lanes of a wave that share a tile are served by one atomic: the lowest

@@ -57,8 +57,8 @@ template <class T> void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, con
 template <class T> void launch_push_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs,
                                                  int* flags, hipStream_t st);
 // Cell-tile sort keeping each particle chunk spatially compact (see
-// k_particles): tiles of TILE_W x TILE_H cells in row-major tile order.
-constexpr int TILE_W_LOG = 4, TILE_H_LOG = 3;
+// k_particles): tiles of TILE_W x TILE_H cells, tile rows in serpentine order.
+constexpr int TILE_W_LOG = 3, TILE_H_LOG = 3;
 constexpr int TILE_W = 1 << TILE_W_LOG, TILE_H = 1 << TILE_H_LOG;
 template <class T> void launch_tile_sort(const SpeciesDev<T>& src, const SpeciesDev<T>& dst, int nx, int ny,
                                          int* count, int* cursor, hipStream_t st);
