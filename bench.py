@@ -76,9 +76,6 @@ def main():
     ap.add_argument("--size", default=None, help="override grid, e.g. 1024x256 (not the metric workload)")
     ap.add_argument("--sort-interval", type=int, default=20, help="steps between tile sorts of the particles")
     ap.add_argument("--trace", type=int, default=0, help="print phase times every N steps (diagnostic)")
-    ap.add_argument("--calibrate", action="store_true",
-                    help="after the timed region, run one odd filter pass (its k_copy moves a known byte "
-                         "count) as the reference kernel for the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
@@ -125,9 +122,6 @@ def main():
     el = time.perf_counter() - t0
     if not args.trace:
         tm = sim.timers(reset=True)
-    if args.calibrate:
-        sim.filter("jx", 1)
-        sim.sync()
     value = n * args.steps / el
     print(f"bench: {el:.3f} s, {value:.4g} particle-steps/s, phases ms {tm}", file=sys.stderr, flush=True)
 

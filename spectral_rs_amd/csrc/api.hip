@@ -213,11 +213,9 @@ template <class T> void flush_push(sp_ctx* c) {
 
 template <class T> void fields_update(sp_ctx* c) {
   const int nx = c->p.size_x, ny = c->p.size_y;
-  {  // mod.rs:90-94
-    T* g[4] = {(T*)c->grid[SP_JX], (T*)c->grid[SP_JY], (T*)c->grid[SP_JZ], (T*)c->grid[SP_RHO]};
-    T* w[4] = {(T*)c->work[0], (T*)c->work[1], (T*)c->work[2], (T*)c->work[3]};
-    launch_binomial<T>(g, w, 4, nx, ny, c->p.n_pass, c->st);
-  }
+  // mod.rs:90-94 (Jx Jy Jz rho are consecutive grid ids)
+  static_assert(SP_JY == SP_JX + 1 && SP_JZ == SP_JX + 2 && SP_RHO == SP_JX + 3, "grid ids");
+  launch_binomial<T>(&c->grid[SP_JX], c->work, 4, nx, ny, c->p.n_pass, c->st);
   {  // mod.rs:97-103, 122-124: 7 forward transforms, spatial->spectral fused
     FftBatch b{};
     const int ids[7] = {SP_JX, SP_JY, SP_JZ, SP_EX, SP_EY, SP_EZ, SP_RHO};
@@ -654,15 +652,8 @@ int sp_boris_push(sp_ctx* c, int id) {
 int sp_filter(sp_ctx* c, int grid, int passes) {
   ENTRY_PRELUDE(c);
   if (grid < 0 || grid >= 11 || passes < 0) return fail(c, SP_EINVAL, "bad grid id or pass count");
-  if (c->f64) {
-    double* g[1] = {(double*)c->grid[grid]};
-    double* w[1] = {(double*)c->work[0]};
-    launch_binomial<double>(g, w, 1, c->p.size_x, c->p.size_y, passes, c->st);
-  } else {
-    float* g[1] = {(float*)c->grid[grid]};
-    float* w[1] = {(float*)c->work[0]};
-    launch_binomial<float>(g, w, 1, c->p.size_x, c->p.size_y, passes, c->st);
-  }
+  if (c->f64) launch_binomial<double>(&c->grid[grid], c->work, 1, c->p.size_x, c->p.size_y, passes, c->st);
+  else launch_binomial<float>(&c->grid[grid], c->work, 1, c->p.size_x, c->p.size_y, passes, c->st);
   HIPCHK(c, hipGetLastError());
   return SP_OK;
 }

@@ -107,59 +107,120 @@ template <class T> void launch_update_ghosts(T* const* grids, int ngrids, int nx
   hipLaunchKernelGGL(k_update_ghosts<T>, dim3((cells + 255) / 256, ngrids), dim3(256), 0, st, gp, nx, ny);
 }
 
-// One pass of Field::binomial_filter_2_d (src/flds/field.rs:385-420) over the
-// interior: 9 taps summed in the reference's order, reading the input's ring
-// as it stands.  Tiles of 64 x 4 outputs stage their 66 x 6 input in LDS.
-constexpr int BTX = 64, BTY = 4;
-template <class T> __global__ void __launch_bounds__(BTX * BTY) k_binomial(Ptrs8<T> in, Ptrs8<T> out, int nx, int ny) {
-  __shared__ T tile[BTY + 2][BTX + 2];
+// Field::binomial_filter_2_d (src/flds/field.rs:385-420) applied NP times,
+// each pass followed by the ghost refresh of field.rs:422 -- in one launch.
+// A workgroup owns FTY x FTX output cells and keeps an extended region of
+// (FTY + 2H) x (FTX + 2H) cells, H = NP - 1, in LDS:
+//  * pass 1 of every extended cell reads its 3x3 padded neighbourhood from
+//    global memory, so cells next to the domain edge see the input's ghost
+//    ring as it stands (stale after the deposit fold, as in the reference);
+//    extended cells beyond the edge are the periodic images of interior
+//    cells, which is what the refreshed ghosts of passes 2.. hold;
+//  * passes 2..NP run in LDS on a region that shrinks by one cell a side;
+//  * the last pass writes the interior and, for edge cells, their periodic
+//    copies in the output's ghost ring.
+// Each pass sums its 9 taps in the reference's order, so the result is
+// bit-identical to NP separate passes.
+constexpr int FTX = 64, FTY = 16, FMAX = 8;
+template <class T, int NP>
+__global__ void __launch_bounds__(256) k_binomial_fused(Ptrs8<T> in, Ptrs8<T> out, int nx, int ny) {
+  constexpr int H = NP - 1, EH = FTY + 2 * H, EW = FTX + 2 * H, LW = EW + 1;
+  __shared__ T buf[2][EH * LW];
   const T* __restrict__ src = in.p[blockIdx.z];
   T* __restrict__ dst = out.p[blockIdx.z];
   const int P = nx + 2;
-  const int c0 = blockIdx.x * BTX, r0 = blockIdx.y * BTY;  // interior offsets (0-based)
-  for (int i = threadIdx.y * BTX + threadIdx.x; i < (BTY + 2) * (BTX + 2); i += BTX * BTY) {
-    const int lr = i / (BTX + 2), lc = i % (BTX + 2);
-    const int gr = r0 + lr, gc = c0 + lc;  // padded coordinates of the tile corner (r0, c0)
-    tile[lr][lc] = (gr <= ny + 1 && gc <= nx + 1) ? src[gr * P + gc] : T(0);
+  const int r0 = blockIdx.y * FTY - H, c0 = blockIdx.x * FTX - H;  // 0-based interior coords of cell (0, 0)
+  const T q = T(0.0625), e = T(0.125), m = T(0.25);
+  for (int i = threadIdx.x; i < EH * EW; i += 256) {
+    const int lr = i / EW, lc = i - lr * EW;
+    int ir = (r0 + lr) % ny, ic = (c0 + lc) % nx;
+    if (ir < 0) ir += ny;
+    if (ic < 0) ic += nx;
+    const T* a = src + (int64_t)ir * P + ic;  // padded (ir+1-1, ic+1-1)
+    T v = q * a[0];
+    v += e * a[1];
+    v += q * a[2];
+    v += e * a[P];
+    v += m * a[P + 1];
+    v += e * a[P + 2];
+    v += q * a[2 * P];
+    v += e * a[2 * P + 1];
+    v += q * a[2 * P + 2];
+    buf[0][lr * LW + lc] = v;
+  }
+#pragma unroll
+  for (int k = 1; k < NP; ++k) {
+    __syncthreads();
+    const T* f = buf[(k - 1) & 1];
+    T* o = buf[k & 1];
+    const int w = EW - 2 * k, h = EH - 2 * k;
+    for (int i = threadIdx.x; i < w * h; i += 256) {
+      const int lr = k + i / w, lc = k + i % w;
+      const T* a = f + (lr - 1) * LW + (lc - 1);
+      T v = q * a[0];
+      v += e * a[1];
+      v += q * a[2];
+      v += e * a[LW];
+      v += m * a[LW + 1];
+      v += e * a[LW + 2];
+      v += q * a[2 * LW];
+      v += e * a[2 * LW + 1];
+      v += q * a[2 * LW + 2];
+      o[lr * LW + lc] = v;
+    }
   }
   __syncthreads();
-  const int r = r0 + threadIdx.y + 1, c = c0 + threadIdx.x + 1;
-  if (r > ny || c > nx) return;
-  const int y = threadIdx.y, x = threadIdx.x;
-  T v = T(0.0625) * tile[y][x];
-  v += T(0.125) * tile[y][x + 1];
-  v += T(0.0625) * tile[y][x + 2];
-  v += T(0.125) * tile[y + 1][x];
-  v += T(0.25) * tile[y + 1][x + 1];
-  v += T(0.125) * tile[y + 1][x + 2];
-  v += T(0.0625) * tile[y + 2][x];
-  v += T(0.125) * tile[y + 2][x + 1];
-  v += T(0.0625) * tile[y + 2][x + 2];
-  dst[r * P + c] = v;
-}
-
-// n passes on up to 8 grids.  Each pass filters grid -> work, refreshes the
-// work ring (field.rs:422) and swaps roles; an odd count ends with a copy
-// back so the result always lands in the caller's grid.
-template <class T> __global__ void k_copy(Ptrs8<T> from, Ptrs8<T> to, int64_t n) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) to.p[blockIdx.y][i] = from.p[blockIdx.y][i];
-}
-
-template <class T>
-void launch_binomial(T* const* grids, T* const* work, int ngrids, int nx, int ny, int passes, hipStream_t st) {
-  if (passes <= 0 || ngrids <= 0) return;
-  Ptrs8<T> a{}, b{};
-  for (int i = 0; i < ngrids; ++i) { a.p[i] = grids[i]; b.p[i] = work[i]; }
-  dim3 grid((nx + BTX - 1) / BTX, (ny + BTY - 1) / BTY, ngrids);
-  for (int k = 0; k < passes; ++k) {
-    hipLaunchKernelGGL(k_binomial<T>, grid, dim3(BTX, BTY), 0, st, a, b, nx, ny);
-    launch_update_ghosts<T>(b.p, ngrids, nx, ny, st);
-    std::swap(a, b);
+  const T* f = buf[(NP - 1) & 1];
+  for (int i = threadIdx.x; i < FTY * FTX; i += 256) {
+    const int tr = i / FTX, tc = i - tr * FTX;
+    const int ir = r0 + H + tr, ic = c0 + H + tc;
+    if (ir >= ny || ic >= nx) continue;
+    const T v = f[(H + tr) * LW + H + tc];
+    const int r = ir + 1, c = ic + 1;
+    dst[(int64_t)r * P + c] = v;
+    // ghost ring = periodic copies (field.rs:141-257)
+    const bool top = r == 1, bot = r == ny, lft = c == 1, rgt = c == nx;
+    if (top) dst[(int64_t)(ny + 1) * P + c] = v;
+    if (bot) dst[c] = v;
+    if (lft) dst[(int64_t)r * P + nx + 1] = v;
+    if (rgt) dst[(int64_t)r * P] = v;
+    if (top && lft) dst[(int64_t)(ny + 1) * P + nx + 1] = v;
+    if (top && rgt) dst[(int64_t)(ny + 1) * P] = v;
+    if (bot && lft) dst[nx + 1] = v;
+    if (bot && rgt) dst[0] = v;
   }
-  if (passes & 1) {
-    const int64_t n = (int64_t)(nx + 2) * (ny + 2);
-    hipLaunchKernelGGL(k_copy<T>, dim3((unsigned)((n + 255) / 256), ngrids), dim3(256), 0, st, a, b, n);
+}
+
+template <class T, int NP>
+static void launch_fused(const Ptrs8<T>& a, const Ptrs8<T>& b, int ngrids, int nx, int ny, hipStream_t st) {
+  dim3 grid((nx + FTX - 1) / FTX, (ny + FTY - 1) / FTY, ngrids);
+  hipLaunchKernelGGL((k_binomial_fused<T, NP>), grid, dim3(256), 0, st, a, b, nx, ny);
+}
+
+// `passes` filter passes on up to 8 grids, FMAX per launch; each launch
+// writes the work grid and the two pointers are swapped, so grids[i] holds the
+// result afterwards (and work[i] the previous contents).
+template <class T>
+void launch_binomial(void** grids, void** work, int ngrids, int nx, int ny, int passes, hipStream_t st) {
+  for (int done = 0; done < passes && ngrids > 0;) {
+    const int n = std::min(FMAX, passes - done);
+    Ptrs8<T> a{}, b{};
+    for (int i = 0; i < ngrids; ++i) {
+      a.p[i] = (T*)grids[i];
+      b.p[i] = (T*)work[i];
+    }
+    switch (n) {
+      case 1: launch_fused<T, 1>(a, b, ngrids, nx, ny, st); break;
+      case 2: launch_fused<T, 2>(a, b, ngrids, nx, ny, st); break;
+      case 3: launch_fused<T, 3>(a, b, ngrids, nx, ny, st); break;
+      case 4: launch_fused<T, 4>(a, b, ngrids, nx, ny, st); break;
+      case 5: launch_fused<T, 5>(a, b, ngrids, nx, ny, st); break;
+      case 6: launch_fused<T, 6>(a, b, ngrids, nx, ny, st); break;
+      case 7: launch_fused<T, 7>(a, b, ngrids, nx, ny, st); break;
+      default: launch_fused<T, 8>(a, b, ngrids, nx, ny, st); break;
+    }
+    for (int i = 0; i < ngrids; ++i) std::swap(grids[i], work[i]);
+    done += n;
   }
 }
 
@@ -373,7 +434,7 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   template void fill<T>(T*, T, int64_t, hipStream_t);                                           \
   template void launch_fold_ghosts<T>(T* const*, int, int, int, hipStream_t);                   \
   template void launch_update_ghosts<T>(T* const*, int, int, int, hipStream_t);                 \
-  template void launch_binomial<T>(T* const*, T* const*, int, int, int, int, hipStream_t);      \
+  template void launch_binomial<T>(void**, void**, int, int, int, int, hipStream_t);              \
   template void launch_spectral_update<T>(void* const*, const Geo<T>&, int, double*, hipStream_t);            \
   template void launch_hermitian<T>(void* const*, int, int, int, hipStream_t);
 INST(float)

@@ -66,7 +66,9 @@ template <class T> void launch_init_species(const SpeciesDev<T>& s, const Geo<T>
                                             uint64_t seed, int sid, hipStream_t st);
 template <class T> void launch_fold_ghosts(T* const* grids, int ngrids, int nx, int ny, hipStream_t st);
 template <class T> void launch_update_ghosts(T* const* grids, int ngrids, int nx, int ny, hipStream_t st);
-template <class T> void launch_binomial(T* const* grids, T* const* work, int ngrids, int nx, int ny, int passes,
+// `passes` binomial passes with ghost refresh; swaps grids[i] <-> work[i] so
+// that grids[i] holds the result.
+template <class T> void launch_binomial(void** grids, void** work, int ngrids, int nx, int ny, int passes,
                                         hipStream_t st);
 // `gauss` (may be null) receives the FP64 Gauss-law sums of this solve:
 // gauss[0..2] = sum |k.E + i rho|^2 before / after the correction and
