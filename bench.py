@@ -125,21 +125,25 @@ def main():
     value = n * args.steps / el
     print(f"bench: {el:.3f} s, {value:.4g} particle-steps/s, phases ms {tm}", file=sys.stderr, flush=True)
 
-    # Fused particle pass (push + move + deposit, both species): algorithmic
-    # bytes = 48 B per particle (24 B state read + 24 B written) + 44 B per
-    # cell (6 field grids gathered, 5 deposit grids written).
-    part_ms = tm["particles"] / args.steps
-    traffic_alg = 48.0 * n + 44.0 * cells
+    # Fused particle pass (push + move + deposit), one launch per species per
+    # step.  Algorithmic bytes of one launch = 48 B per particle of the species
+    # (24 B state read + 24 B written) + 44 B per padded cell (6 field grids
+    # gathered, 5 deposit grids written); its duration is the phase time of
+    # the pass (HIP events on the library's stream) over the launches.
+    launches = len(SPECIES)
+    part_ms = tm["particles"] / args.steps / launches
+    traffic_alg = 48.0 * n / launches + 44.0 * cells
     achieved = traffic_alg / (part_ms * 1e-3) / 1e9
-    # HBM bytes per pass measured by rocprofv3 FETCH_SIZE / WRITE_SIZE on this
-    # workload (calibrated against a copy kernel), when that measurement exists.
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+    # workload (calibrated against kernels of known traffic), when that
+    # measurement exists.
     traffic = None
     tf = os.path.join(ROOT, "profiles", "c2_particles_traffic.json")
     if not args.size and os.path.exists(tf):
         with open(tf) as f:
             t = json.load(f)
-        traffic = 1024.0 * t["species_launches_per_pass"] * (t["fetch_kib_per_species_launch"] * t["fetch_factor"]
-                                                             + t["write_kib_per_species_launch"] * t["write_factor"])
+        traffic = 1024.0 * (t["fetch_kib_per_species_launch"] * t["fetch_factor"]
+                            + t["write_kib_per_species_launch"] * t["write_factor"])
     out = {
         "metric": "particle-steps/s on 1 MI355X",
         "value": value,
@@ -159,8 +163,9 @@ def main():
         "phases_ms_per_step": {k: v / args.steps for k, v in tm.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_particles<float, PUSH> (fused push + move + deposit, both species per step)",
-                     "bytes_per_launch_algorithmic": traffic_alg, "ms_per_launch": part_ms},
+                     "kernel": "k_particles<float, true> (fused push + move + deposit, one species per launch)",
+                     "launches_per_step": launches, "bytes_per_launch_algorithmic": traffic_alg,
+                     "ms_per_launch": part_ms},
     }
     sim.close()
     if not args.no_cpu:

@@ -733,31 +733,110 @@ __device__ __forceinline__ int tile_of(int32_t c, int ntx, int nty) {
   return ty * ntx + ((ty & 1) ? ntx - 1 - tx : tx);
 }
 
-// Lanes of a wave that share a tile are served by one atomic: the lowest
-// such lane adds the group's size and broadcasts the old value.
-__device__ __forceinline__ int grouped_atomic_add(int* ctr, int key, bool active) {
+// Each workgroup takes SORT_PPT * 256 consecutive particles and counts them
+// per tile in an LDS hash table (tile -> count), one LDS atomic per group of
+// lanes of a wave that share a tile; then one global atomic per distinct tile
+// of the workgroup.  The input is sorted by the previous pass except for the
+// particles that changed tile since, so a workgroup holds few distinct tiles
+// and the global counters see ~1-3 atomics per workgroup instead of one per
+// wave and tile.
+constexpr int SORT_NT = 256, SORT_PPT = 8, SORT_HT = 4096;  // table: > 2x the keys of a workgroup
+
+__device__ __forceinline__ int hash_slot(lds_ptr<int> keys, int key) {
+  int h = (int)(((unsigned)key * 2654435761u) >> 20);  // 12 bits
+  for (;;) {
+    int prev = -1;  // holds the slot's old key afterwards
+    __hip_atomic_compare_exchange_strong(keys + h, &prev, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (prev == -1 || prev == key) return h;
+    h = (h + 1) & (SORT_HT - 1);
+  }
+}
+
+// Adds the lanes' keys to the table; returns each active lane's slot and its
+// rank among the workgroup's particles of that tile.
+__device__ __forceinline__ void table_add(lds_ptr<int> keys, lds_ptr<int> cnt, int key, bool active, int& slot,
+                                          int& rank) {
   const int lane = __lane_id();
   uint64_t todo = __ballot(active);
-  int mine = 0;
   while (todo) {
     const int leader = __ffsll((unsigned long long)todo) - 1;
     const int lkey = __shfl(key, leader);
     const uint64_t grp = __ballot(active && key == lkey) & todo;
-    int basev = 0;
-    if (lane == leader) basev = atomicAdd(ctr + lkey, __popcll(grp));
-    basev = __shfl(basev, leader);
-    if ((grp >> lane) & 1ull) mine = basev + __popcll(grp & ((1ull << lane) - 1ull));
+    int s = 0, b = 0;
+    if (lane == leader) {
+      s = hash_slot(keys, lkey);
+      b = __hip_atomic_fetch_add(cnt + s, __popcll(grp), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    s = __shfl(s, leader);
+    b = __shfl(b, leader);
+    if ((grp >> lane) & 1ull) {
+      slot = s;
+      rank = b + __popcll(grp & ((1ull << lane) - 1ull));
+    }
     todo &= ~grp;
   }
-  return mine;
 }
 
-__global__ void __launch_bounds__(256) k_tile_hist(const int32_t* __restrict__ cell, int64_t n, int ntx, int nty,
-                                                   int* count) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const bool act = i < n;
-  const int t = act ? tile_of(cell[i], ntx, nty) : 0;
-  grouped_atomic_add(count, t, act);
+// Counts (HIST) or places (!HIST) one workgroup's particles.
+template <class T, bool HIST>
+__global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty,
+                                                       int* ctr) {
+  __shared__ int s_keys[SORT_HT], s_cnt[SORT_HT];
+  lds_ptr<int> keys = (lds_ptr<int>)s_keys, cnt = (lds_ptr<int>)s_cnt;
+  for (int i = threadIdx.x; i < SORT_HT; i += SORT_NT) {
+    keys[i] = -1;
+    cnt[i] = 0;
+  }
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * (SORT_NT * SORT_PPT);
+  // all loads first, so that they are in flight together
+  int32_t c[SORT_PPT], tg[SORT_PPT];
+  T v[5][SORT_PPT];
+#pragma unroll
+  for (int j = 0; j < SORT_PPT; ++j) {
+    const int64_t i = base + threadIdx.x + SORT_NT * j;
+    c[j] = i < a.n ? a.cell[i] : 0;
+    if (!HIST && i < a.n) {
+      v[0][j] = a.dx[i];
+      v[1][j] = a.dy[i];
+      v[2][j] = a.px[i];
+      v[3][j] = a.py[i];
+      v[4][j] = a.pz[i];
+      tg[j] = a.tag[i];
+    }
+  }
+  int slot[SORT_PPT], rank[SORT_PPT];
+#pragma unroll
+  for (int j = 0; j < SORT_PPT; ++j) {
+    const int64_t i = base + threadIdx.x + SORT_NT * j;
+    const bool act = i < a.n;
+    slot[j] = 0;
+    rank[j] = 0;
+    table_add(keys, cnt, act ? tile_of(c[j], ntx, nty) : 0, act, slot[j], rank[j]);
+  }
+  __syncthreads();
+  for (int sl = threadIdx.x; sl < SORT_HT; sl += SORT_NT) {
+    const int key = keys[sl];
+    if (key < 0) continue;
+    if (HIST) atomicAdd(ctr + key, (int)cnt[sl]);
+    else cnt[sl] = atomicAdd(ctr + key, (int)cnt[sl]);  // the tile's first output position
+  }
+  if (HIST) return;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < SORT_PPT; ++j) {
+    const int64_t i = base + threadIdx.x + SORT_NT * j;
+    if (i >= a.n) continue;
+    const int pos = cnt[slot[j]] + rank[j];
+    b.cell[pos] = c[j];
+    b.dx[pos] = v[0][j];
+    b.dy[pos] = v[1][j];
+    b.px[pos] = v[2][j];
+    b.py[pos] = v[3][j];
+    b.pz[pos] = v[4][j];
+    b.tag[pos] = tg[j];
+  }
 }
 
 // Exclusive prefix sum of `count` into `cursor`, one workgroup.
@@ -784,32 +863,15 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const int* __restrict__ coun
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_tile_scatter(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty,
-                                                      int* cursor) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const bool act = i < a.n;
-  const int32_t c = act ? a.cell[i] : 0;
-  const int t = act ? tile_of(c, ntx, nty) : 0;
-  const int pos = grouped_atomic_add(cursor, t, act);
-  if (!act) return;
-  b.cell[pos] = c;
-  b.dx[pos] = a.dx[i];
-  b.dy[pos] = a.dy[i];
-  b.px[pos] = a.px[i];
-  b.py[pos] = a.py[i];
-  b.pz[pos] = a.pz[i];
-  b.tag[pos] = a.tag[i];
-}
-
-template <class T>
 void launch_tile_sort(const SpeciesDev<T>& a, const SpeciesDev<T>& b, int nx, int ny, int* count, int* cursor,
                       hipStream_t st) {
   if (a.n == 0) return;
   const int ntx = (nx + TILE_W - 1) / TILE_W, nty = (ny + TILE_H - 1) / TILE_H;
   fill<int>(count, 0, (int64_t)ntx * nty, st);
-  hipLaunchKernelGGL(k_tile_hist, dim3(blocks_for(a.n, 256)), dim3(256), 0, st, a.cell, a.n, ntx, nty, count);
+  const unsigned nb = blocks_for(a.n, SORT_NT * SORT_PPT);
+  hipLaunchKernelGGL((k_tile_pass<T, true>), dim3(nb), dim3(SORT_NT), 0, st, a, b, ntx, nty, count);
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, count, cursor, ntx * nty);
-  hipLaunchKernelGGL(k_tile_scatter<T>, dim3(blocks_for(a.n, 256)), dim3(256), 0, st, a, b, ntx, nty, cursor);
+  hipLaunchKernelGGL((k_tile_pass<T, false>), dim3(nb), dim3(SORT_NT), 0, st, a, b, ntx, nty, cursor);
 }
 
 template <class T>
