@@ -85,6 +85,20 @@ template <class T, int R, bool INV> __device__ __forceinline__ void dftR(V2<T>* 
 
 constexpr int EPT = 16;  // elements per thread per pass
 
+// Division by a runtime constant d: q = hi32(x * m) with m = ceil(2^32 / d)
+// is exact whenever x * d < 2^32 (the error x (m - 2^32/d) / 2^32 < x / 2^32
+// stays below the gap 1/d between x/d and the next integer); every use below
+// has x * d < 2^27.  Replaces integer divisions in the index arithmetic, which
+// cost tens of instructions each.
+// (d = 1 would need m = 2^32, which does not fit: it is handled apart.)
+struct FastDiv {
+  uint32_t d, m;
+  __device__ __forceinline__ explicit FastDiv(uint32_t dd)
+      : d(dd), m(dd > 1 ? (uint32_t)((0xFFFFFFFFull + dd) / dd) : 0u) {}
+  __device__ __forceinline__ uint32_t div(uint32_t x) const { return d > 1 ? __umulhi(x, m) : x; }
+  __device__ __forceinline__ uint32_t mod(uint32_t x) const { return x - div(x) * d; }
+};
+
 // One Stockham pass of radix R over `seqs` sequences of length N held in LDS.
 // Sequence s, element n lives at lds[s * sstride + n * estride].  SEQ_FAST
 // picks which index varies fastest across threads (columns: the sequence,
@@ -96,6 +110,7 @@ __device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, int seq
   const int nb = N / R;
   const int total = nb * seqs;
   const int twstep = N / (Ns * R);
+  const FastDiv fseq(SEQ_FAST ? seqs : nb), fns(Ns);
   V2<T> v[BPT][R];
   int dst[BPT];
 #pragma unroll
@@ -103,12 +118,14 @@ __device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, int seq
     const int b = threadIdx.x + m * nthreads;
     dst[m] = -1;
     if (b < total) {
-      const int s = SEQ_FAST ? b % seqs : b / nb;
-      const int j = SEQ_FAST ? b / seqs : b % nb;
+      const int q = (int)fseq.div((uint32_t)b), rm = b - q * (int)fseq.d;
+      const int s = SEQ_FAST ? rm : q;
+      const int j = SEQ_FAST ? q : rm;
       const V2<T>* src = lds + s * sstride;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[m][r] = src[(j + r * nb) * estride];
-      const int k = j % Ns;
+      const int jq = (int)fns.div((uint32_t)j);
+      const int k = j - jq * Ns;
       if (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -118,7 +135,7 @@ __device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, int seq
         }
       }
       dftR<T, R, INV>(v[m]);
-      dst[m] = s * sstride + ((j / Ns) * Ns * R + k) * estride;
+      dst[m] = s * sstride + (jq * Ns * R + k) * estride;
     }
   }
   __syncthreads();
@@ -159,11 +176,12 @@ __global__ void __launch_bounds__(512) k_fft_rows(FftBatch b, FftDim d, int ny, 
   const int nt = blockDim.x;
   V2<T>* spec = (V2<T>*)b.spec[f];
   T* grid = (T*)b.grid[f];
+  const FastDiv fnx(nx), fP(P);
   if (load == LOAD_REAL_PADDED) {
     const T sc = (T)b.scale[f];
     const bool scaled = b.scale[f] != 1.0;
     for (int i = threadIdx.x; i < E; i += nt) {
-      const int s = i / nx, n = i - s * nx;
+      const int s = (int)fnx.div(i), n = i - s * nx;
       T v = grid[(r0 + s + 1) * P + n + 1];
       if (scaled) v = v * sc;
       lds[i] = V2<T>{v, T(0.0)};
@@ -177,7 +195,7 @@ __global__ void __launch_bounds__(512) k_fft_rows(FftBatch b, FftDim d, int ny, 
     // copy_to_spatial + update_ghosts: interior, the two ghost columns of this
     // row, and the ghost row that mirrors it (row Ny -> row 0, row 1 -> Ny+1).
     for (int i = threadIdx.x; i < rows * P; i += nt) {
-      const int s = i / P, c = i - s * P;
+      const int s = (int)fP.div(i), c = i - s * P;
       const int n = (c == 0) ? nx - 1 : (c == P - 1 ? 0 : c - 1);
       const T v = lds[s * nx + n].x * scale_store;
       const int r = r0 + s + 1;
@@ -207,15 +225,16 @@ __global__ void __launch_bounds__(512) k_fft_cols(FftBatch b, FftDim d, int nx, 
   const int E = cols * ny;
   const int nt = blockDim.x;
   V2<T>* spec = (V2<T>*)b.spec[f];
+  const FastDiv fc(cols);
   for (int i = threadIdx.x; i < E; i += nt) {
-    const int n = i / cols, c = i - n * cols;
+    const int n = (int)fc.div(i), c = i - n * cols;
     lds[i] = spec[(int64_t)n * nx + c0 + c];
   }
   __syncthreads();
   fft_lds<T, INV, true>(lds, d, cols, 1, cols, nt);
   const bool scaled = scale_store != T(1);
   for (int i = threadIdx.x; i < E; i += nt) {
-    const int n = i / cols, c = i - n * cols;
+    const int n = (int)fc.div(i), c = i - n * cols;
     V2<T> v = lds[i];
     if (scaled) v = V2<T>{v.x * scale_store, v.y * scale_store};
     spec[(int64_t)n * nx + c0 + c] = v;
@@ -234,6 +253,12 @@ template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st) {
   hipLaunchKernelGGL(k_twiddles<T>, dim3((d.n + 255) / 256), dim3(256), 0, st, (V2<T>*)d.tw, d.n);
 }
 
+#ifndef SP_FFT_ROWS_MAX
+#define SP_FFT_ROWS_MAX 1
+#endif
+#ifndef SP_FFT_COLS_MAX
+#define SP_FFT_COLS_MAX 4
+#endif
 static int round_threads(int e) {
   int t = (e + EPT - 1) / EPT;
   t = ((t + 63) / 64) * 64;
@@ -246,7 +271,7 @@ void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, i
                      double store_scale, hipStream_t st) {
   int rpb = max_elems<T>() / dx.n;
   if (rpb < 1) rpb = 1;
-  if (rpb > 16) rpb = 16;
+  if (rpb > SP_FFT_ROWS_MAX) rpb = SP_FFT_ROWS_MAX;
   if (rpb > ny) rpb = ny;
   const int E = rpb * dx.n;
   const int nt = round_threads(E);
@@ -263,7 +288,7 @@ template <class T>
 void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, double store_scale, hipStream_t st) {
   int C = max_elems<T>() / dy.n;
   if (C < 1) C = 1;
-  if (C > 16) C = 16;
+  if (C > SP_FFT_COLS_MAX) C = SP_FFT_COLS_MAX;
   if (C > nx) C = nx;
   const int E = C * dy.n;
   const int nt = round_threads(E);

@@ -360,10 +360,10 @@ __global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T
 // Sums the block partials of k_spectral (one block, fixed order) into
 // out[0..2] = (pre, post, |rho|^2) of this solve, and keeps in out[3] the
 // largest relative post-correction residual sqrt(post / |rho|^2) seen so far.
-__global__ void __launch_bounds__(256) k_gauss_reduce(const double* part, int nblk, double* out) {
-  __shared__ double red[3][4];
+__global__ void __launch_bounds__(1024) k_gauss_reduce(const double* part, int nblk, double* out) {
+  __shared__ double red[3][16];
   double s[3] = {0, 0, 0};
-  for (int b = threadIdx.x; b < nblk; b += 256)
+  for (int b = threadIdx.x; b < nblk; b += 1024)
     for (int k = 0; k < 3; ++k) s[k] += part[(int64_t)b * 3 + k];
   const int w = threadIdx.x >> 6;
   for (int k = 0; k < 3; ++k) {
@@ -373,7 +373,10 @@ __global__ void __launch_bounds__(256) k_gauss_reduce(const double* part, int nb
   __syncthreads();
   if (threadIdx.x == 0) {
     double t[3];
-    for (int k = 0; k < 3; ++k) t[k] = ((red[k][0] + red[k][1]) + red[k][2]) + red[k][3];
+    for (int k = 0; k < 3; ++k) {
+      t[k] = 0;
+      for (int i = 0; i < 16; ++i) t[k] += red[k][i];
+    }
     out[0] = t[0];
     out[1] = t[1];
     out[2] = t[2];
@@ -427,7 +430,7 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   const int nblk = (int)((n + 255) / 256);
   hipLaunchKernelGGL(k_spectral<T>, dim3((unsigned)nblk), dim3(256), 0, st, sp, g.nx, g.ny, cdt, ckc, twopi_x,
                      twopi_y, gauss ? gauss + GAUSS_PARTIALS : nullptr);
-  if (gauss) hipLaunchKernelGGL(k_gauss_reduce, dim3(1), dim3(256), 0, st, gauss + GAUSS_PARTIALS, nblk, gauss);
+  if (gauss) hipLaunchKernelGGL(k_gauss_reduce, dim3(1), dim3(1024), 0, st, gauss + GAUSS_PARTIALS, nblk, gauss);
 }
 
 #define INST(T)                                                                                 \
