@@ -100,6 +100,7 @@ template <class T> Geo<T> geo(const sp_ctx* c) {
   g.checked = c->p.checked;
   g.left_edge = c->p.left_edge;
   g.lds_ok = c->p.dt * c->p.c < 1.0;
+  g.direct = c->p.sort_interval > 0;
   g.dt = (T)c->p.dt;
   g.c = (T)c->p.c;
   g.csqinv = T(1.) / (g.c * g.c);

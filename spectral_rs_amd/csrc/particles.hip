@@ -223,13 +223,16 @@ template <> struct Acc<double> {
 #define SP_FP32_CHUNK 2048
 #endif
 #ifndef SP_WINDOW_COPIES
-#define SP_WINDOW_COPIES 16
+#define SP_WINDOW_COPIES 4
 #endif
 #ifndef SP_PARTICLE_WAVES_PER_EU
 #define SP_PARTICLE_WAVES_PER_EU 1
 #endif
+#ifndef SP_PARTICLE_NT
+#define SP_PARTICLE_NT 256
+#endif
 template <class T> struct ChunkShape {
-  static constexpr int NT = 512;
+  static constexpr int NT = SP_PARTICLE_NT;
   static constexpr int K = sizeof(T) == 4 ? SP_FP32_CHUNK : 1024;
   static constexpr int WMAX = sizeof(T) == 4 ? 2000 : 1000;  // 2 workgroups fit one CU's LDS
   static constexpr int NCOPY = SP_WINDOW_COPIES;  // most private window copies
@@ -439,14 +442,15 @@ __device__ __forceinline__ int wave_or(int v) {
 // particles near the chunk's middle particle.  Particles whose stencil leaves
 // the window (periodic wrap, unsorted input) use global atomics directly, so
 // the result never depends on the particle order; only the speed does.
-template <class T, bool PUSH>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SP_PARTICLE_WAVES_PER_EU)))
+template <class T, bool PUSH, bool DIRECT>
+__global__ void __launch_bounds__(SP_PARTICLE_NT) __attribute__((amdgpu_waves_per_eu(SP_PARTICLE_WAVES_PER_EU)))
 k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   constexpr int NT = ChunkShape<T>::NT, K = ChunkShape<T>::K, WMAX = ChunkShape<T>::WMAX, PPT = K / NT;
   constexpr int NCOPY = ChunkShape<T>::NCOPY;
+  constexpr int KS = DIRECT ? 1 : K;  // staging arrays (unused when DIRECT)
   using A = typename Acc<T>::type;
-  __shared__ int32_t s_cell[K];
-  __shared__ T s_dx[K], s_dy[K], s_px[K], s_py[K], s_pz[K];
+  __shared__ int32_t s_cell[KS];
+  __shared__ T s_dx[KS], s_dy[KS], s_px[KS], s_py[KS], s_pz[KS];
   __shared__ A s_acc[4 * WMAX];
   __shared__ int s_box[5];
   const int tid = threadIdx.x;
@@ -460,21 +464,34 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
     s_box[4] = 0;
   }
   __syncthreads();
+  // Particle i = tid + NT j of the chunk; DIRECT keeps its cell in a register.
+  int32_t creg[DIRECT ? PPT : 1];
   int xmn = INT_MAX, xmx = INT_MIN, ymn = INT_MAX, ymx = INT_MIN;
-  for (int i = tid; i < cnt; i += NT) {
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int i = tid + NT * j;
+    if (i >= cnt) break;
     const int32_t c = s.cell[base + i];
-    s_cell[i] = c;
-    s_dx[i] = s.dx[base + i];
-    s_dy[i] = s.dy[base + i];
-    s_px[i] = s.px[base + i];
-    s_py[i] = s.py[base + i];
-    s_pz[i] = s.pz[base + i];
+    if constexpr (DIRECT) {
+      creg[j] = c;
+    } else {
+      s_cell[i] = c;
+      s_dx[i] = s.dx[base + i];
+      s_dy[i] = s.dy[base + i];
+      s_px[i] = s.px[base + i];
+      s_py[i] = s.py[base + i];
+      s_pz[i] = s.pz[base + i];
+    }
     const int ix = cell_ix(c), iy = cell_iy(c);
     xmn = min(xmn, ix);
     xmx = max(xmx, ix);
     ymn = min(ymn, iy);
     ymx = max(ymx, iy);
   }
+  const auto cell_of = [&](int j) -> int32_t {
+    if constexpr (DIRECT) return creg[j];
+    else return s_cell[tid + NT * j];
+  };
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     xmn = min(xmn, __shfl_xor(xmn, off));
@@ -506,7 +523,7 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
     // only the particles within RBOX cells of the chunk's middle particle;
     // the others miss the window and deposit with global atomics.
     constexpr int RBOX = 16;
-    const int32_t mc = s_cell[cnt / 2];
+    const int32_t mc = DIRECT ? s.cell[base + cnt / 2] : s_cell[DIRECT ? 0 : cnt / 2];
     const int mx = cell_ix(mc), my = cell_iy(mc);
     __syncthreads();  // everyone has read the full box
     if (tid == 0) {
@@ -517,8 +534,10 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
     }
     __syncthreads();
     int a0 = INT_MAX, a1 = INT_MIN, b0 = INT_MAX, b1 = INT_MIN;
-    for (int i = tid; i < cnt; i += NT) {
-      const int32_t c = s_cell[i];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      if (tid + NT * j >= cnt) break;
+      const int32_t c = cell_of(j);
       const int ix = cell_ix(c), iy = cell_iy(c);
       if (abs(ix - mx) <= RBOX && abs(iy - my) <= RBOX) {
         a0 = min(a0, ix);
@@ -556,8 +575,10 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
       lds_ptr<int> cntg = (lds_ptr<int>)s_acc;
       for (int i = tid; i < k.wh; i += NT) cntg[i] = 0;
       __syncthreads();
-      for (int i = tid; i < cnt; i += NT) {
-        const int32_t c = s_cell[i];
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        if (tid + NT * j >= cnt) break;
+        const int32_t c = cell_of(j);
         const int cx = cell_ix(c) - k.x0, cy = cell_iy(c) - k.y0;
         if (cx >= 0 && cx < k.w && cy >= 0 && cy < k.h)  // particles outside deposit globally
           __hip_atomic_fetch_add(cntg + cy * k.w + cx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -599,33 +620,58 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   }();
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
-    const int p = ((tid + NT * j) * 33) & (K - 1);
-    if (p < cnt) {
-      const int32_t c = s_cell[p];
-      int ix = cell_ix(c), iy = cell_iy(c);
-      T dx = s_dx[p], dy = s_dy[p], px = s_px[p], py = s_py[p], pz = s_pz[p];
-      if (PUSH) {
-        boris(ix, iy, dx, dy, px, py, pz, fields, g, s.alpha, s.beta, bad);
-        if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
-        s_py[p] = py;
-        s_pz[p] = pz;
+    if constexpr (DIRECT) {
+      // the sort interleaves each tile's particles (k_tile_pass), so
+      // consecutive particles, hence lanes, sit in different cells
+      const int i = tid + NT * j;
+      if (i < cnt) {
+        const int64_t q = base + i;
+        const int32_t c = creg[j];
+        int ix = cell_ix(c), iy = cell_iy(c);
+        T dx = s.dx[q], dy = s.dy[q], px = s.px[q], py = s.py[q], pz = s.pz[q];
+        if (PUSH) {
+          boris(ix, iy, dx, dy, px, py, pz, fields, g, s.alpha, s.beta, bad);
+          if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
+          s.py[q] = py;
+          s.pz[q] = pz;
+        }
+        move_and_deposit(ix, iy, dx, dy, px, py, pz, s.charge, g, k, bad);
+        s.cell[q] = make_cell(ix, iy);
+        s.dx[q] = dx;
+        s.dy[q] = dy;
+        s.px[q] = px;
       }
-      move_and_deposit(ix, iy, dx, dy, px, py, pz, s.charge, g, k, bad);
-      s_cell[p] = make_cell(ix, iy);
-      s_dx[p] = dx;
-      s_dy[p] = dy;
-      s_px[p] = px;
+    } else {
+      const int p = ((tid + NT * j) * 33) & (K - 1);
+      if (p < cnt) {
+        const int32_t c = s_cell[p];
+        int ix = cell_ix(c), iy = cell_iy(c);
+        T dx = s_dx[p], dy = s_dy[p], px = s_px[p], py = s_py[p], pz = s_pz[p];
+        if (PUSH) {
+          boris(ix, iy, dx, dy, px, py, pz, fields, g, s.alpha, s.beta, bad);
+          if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
+          s_py[p] = py;
+          s_pz[p] = pz;
+        }
+        move_and_deposit(ix, iy, dx, dy, px, py, pz, s.charge, g, k, bad);
+        s_cell[p] = make_cell(ix, iy);
+        s_dx[p] = dx;
+        s_dy[p] = dy;
+        s_px[p] = px;
+      }
     }
   }
   __syncthreads();
-  for (int i = tid; i < cnt; i += NT) {
-    s.cell[base + i] = s_cell[i];
-    s.dx[base + i] = s_dx[i];
-    s.dy[base + i] = s_dy[i];
-    s.px[base + i] = s_px[i];
-    if (PUSH) {
-      s.py[base + i] = s_py[i];
-      s.pz[base + i] = s_pz[i];
+  if constexpr (!DIRECT) {
+    for (int i = tid; i < cnt; i += NT) {
+      s.cell[base + i] = s_cell[i];
+      s.dx[base + i] = s_dx[i];
+      s.dy[base + i] = s_dy[i];
+      s.px[base + i] = s_px[i];
+      if (PUSH) {
+        s.py[base + i] = s_py[i];
+        s.pz[base + i] = s_pz[i];
+      }
     }
   }
   for (int i = tid; i < k.wh; i += NT) {
@@ -778,11 +824,39 @@ __device__ __forceinline__ void table_add(lds_ptr<int> keys, lds_ptr<int> cnt, i
   }
 }
 
-// Counts (HIST) or places (!HIST) one workgroup's particles.
+// Inverse of a modulo n (gcd(a, n) = 1), extended Euclid.
+__device__ __forceinline__ int inv_mod(int a, int n) {
+  int t = 0, nt = 1, r = n, nr = a % n;
+  while (nr != 0) {
+    const int q = r / nr;
+    int x = t - q * nt;
+    t = nt;
+    nt = x;
+    x = r - q * nr;
+    r = nr;
+    nr = x;
+  }
+  return t < 0 ? t + n : t;
+}
+__device__ __forceinline__ int gcd_int(int a, int b) {
+  while (b != 0) {
+    const int t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// Counts (HIST) or places (!HIST) one workgroup's particles.  Placement
+// interleaves each group of n particles that share a tile: rank r goes to
+// position (r * u^-1) mod n for a stride u ~ 33 coprime with n, so position p
+// holds rank (p * u) mod n and consecutive positions hold particles about a
+// cell's worth apart in the input order -- in different cells, so the lanes
+// of a wave of the particle pass rarely hit the same deposit addresses.
 template <class T, bool HIST>
 __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty,
                                                        int* ctr) {
-  __shared__ int s_keys[SORT_HT], s_cnt[SORT_HT];
+  __shared__ int s_keys[SORT_HT], s_cnt[SORT_HT], s_n[HIST ? 1 : SORT_HT];
   lds_ptr<int> keys = (lds_ptr<int>)s_keys, cnt = (lds_ptr<int>)s_cnt;
   for (int i = threadIdx.x; i < SORT_HT; i += SORT_NT) {
     keys[i] = -1;
@@ -819,8 +893,16 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
   for (int sl = threadIdx.x; sl < SORT_HT; sl += SORT_NT) {
     const int key = keys[sl];
     if (key < 0) continue;
-    if (HIST) atomicAdd(ctr + key, (int)cnt[sl]);
-    else cnt[sl] = atomicAdd(ctr + key, (int)cnt[sl]);  // the tile's first output position
+    const int n = cnt[sl];
+    if (HIST) {
+      atomicAdd(ctr + key, n);
+    } else {
+      cnt[sl] = atomicAdd(ctr + key, n);  // the group's first output position
+      int u = 33;
+      while (gcd_int(u, n) != 1) u += 2;
+      s_n[sl] = n;
+      keys[sl] = inv_mod(u % n, n);  // the slot's key is no longer needed
+    }
   }
   if (HIST) return;
   __syncthreads();
@@ -828,7 +910,8 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
   for (int j = 0; j < SORT_PPT; ++j) {
     const int64_t i = base + threadIdx.x + SORT_NT * j;
     if (i >= a.n) continue;
-    const int pos = cnt[slot[j]] + rank[j];
+    const int n = s_n[slot[j]];
+    const int pos = cnt[slot[j]] + (n > 1 ? (rank[j] * keys[slot[j]]) % n : 0);
     b.cell[pos] = c[j];
     b.dx[pos] = v[0][j];
     b.dy[pos] = v[1][j];
@@ -878,8 +961,12 @@ template <class T>
 void launch_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
   if (s.n == 0) return;
   const int K = ChunkShape<T>::K;
-  hipLaunchKernelGGL((k_particles<T, false>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s, g, gs,
-                     flags);
+  if (g.direct)
+    hipLaunchKernelGGL((k_particles<T, false, true>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s,
+                       g, gs, flags);
+  else
+    hipLaunchKernelGGL((k_particles<T, false, false>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s,
+                       g, gs, flags);
 }
 template <class T>
 void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
@@ -891,8 +978,12 @@ void launch_push_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const Gri
                               hipStream_t st) {
   if (s.n == 0) return;
   const int K = ChunkShape<T>::K;
-  hipLaunchKernelGGL((k_particles<T, true>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s, g, gs,
-                     flags);
+  if (g.direct)
+    hipLaunchKernelGGL((k_particles<T, true, true>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s,
+                       g, gs, flags);
+  else
+    hipLaunchKernelGGL((k_particles<T, true, false>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s,
+                       g, gs, flags);
 }
 template <class T>
 void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, double gamma_inj, uint64_t seed,

@@ -28,6 +28,8 @@ template <class T> struct Geo {
   int delta, wall, lwall;  // wall = nx + 1 - delta (src/prtls.rs:153), lwall = delta + 1
   int periodic_x, checked, left_edge;
   int lds_ok;              // dt * c < 1: a particle moves less than one cell per step
+  int direct;              // particles are kept tile-sorted and interleaved: the particle
+                           // pass reads them in place (no LDS staging or lane permutation)
   T dt, c, csqinv;         // csqinv = 1 / (c * c) in Float (src/prtls.rs:265)
   T half_dt;               // 0.5 * dt in Float (src/prtls.rs:103)
 };
