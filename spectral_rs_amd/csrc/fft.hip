@@ -16,6 +16,16 @@
 
 #include "sp_internal.h"
 
+#ifndef SP_FFT_XCD_REMAP
+#define SP_FFT_XCD_REMAP 1
+#endif
+#ifndef SP_FFT_ROWS_MAX
+#define SP_FFT_ROWS_MAX 1
+#endif
+#ifndef SP_FFT_COLS_MAX
+#define SP_FFT_COLS_MAX 8
+#endif
+
 namespace sp {
 
 template <class T> struct V2 {
@@ -220,7 +230,11 @@ __global__ void __launch_bounds__(512) k_fft_cols(FftBatch b, FftDim d, int nx, 
   V2<T>* lds = (V2<T>*)smem;
   const int f = blockIdx.y;
   const int ny = d.n;
-  const int c0 = blockIdx.x * C;
+  // Blocks of C < 16 columns read parts of 128-byte lines that neighbouring
+  // blocks also read: each XCD takes a contiguous run of column blocks, so the
+  // neighbours share its L2.
+  const int bx = SP_FFT_XCD_REMAP ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int c0 = bx * C;
   const int cols = min(C, nx - c0);
   const int E = cols * ny;
   const int nt = blockDim.x;
@@ -253,12 +267,6 @@ template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st) {
   hipLaunchKernelGGL(k_twiddles<T>, dim3((d.n + 255) / 256), dim3(256), 0, st, (V2<T>*)d.tw, d.n);
 }
 
-#ifndef SP_FFT_ROWS_MAX
-#define SP_FFT_ROWS_MAX 1
-#endif
-#ifndef SP_FFT_COLS_MAX
-#define SP_FFT_COLS_MAX 4
-#endif
 static int round_threads(int e) {
   int t = (e + EPT - 1) / EPT;
   t = ((t + 63) / 64) * 64;

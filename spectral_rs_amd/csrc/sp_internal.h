@@ -46,6 +46,15 @@ struct GridSet {  // device pointers of one precision, indexed by sp_grid
   void* g[11];
 };
 
+// Workgroups are dealt to the MI355X's 8 XCDs round-robin (workgroup b runs on
+// XCD b % 8), each XCD with its own L2.  This maps b to a logical index such
+// that every XCD walks one contiguous run of logical indices (a bijection of
+// [0, n) for any n), so neighbouring work shares an L2.
+__device__ inline int xcd_remap(int b, int n) {
+  const int x = b & 7, q = b >> 3, full = n >> 3, rem = n & 7;
+  return x * full + (x < rem ? x : rem) + q;
+}
+
 __host__ __device__ inline int cell_ix(int32_t c) { return c & 0xFFFF; }
 __host__ __device__ inline int cell_iy(int32_t c) { return c >> 16; }
 __host__ __device__ inline int32_t make_cell(int ix, int iy) { return (iy << 16) | (ix & 0xFFFF); }
