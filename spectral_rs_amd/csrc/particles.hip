@@ -856,14 +856,19 @@ __device__ __forceinline__ int gcd_int(int a, int b) {
 template <class T, bool HIST>
 __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty,
                                                        int* ctr) {
+  constexpr int CH = SORT_NT * SORT_PPT;
+  constexpr int OUT = HIST ? 1 : CH;
   __shared__ int s_keys[SORT_HT], s_cnt[SORT_HT], s_n[HIST ? 1 : SORT_HT];
+  __shared__ int s_gpos[OUT], s_val[OUT];  // placement: output position and value by local index
+  __shared__ int s_total;
   lds_ptr<int> keys = (lds_ptr<int>)s_keys, cnt = (lds_ptr<int>)s_cnt;
   for (int i = threadIdx.x; i < SORT_HT; i += SORT_NT) {
     keys[i] = -1;
     cnt[i] = 0;
   }
+  if (threadIdx.x == 0) s_total = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * (SORT_NT * SORT_PPT);
+  const int64_t base = (int64_t)blockIdx.x * CH;
   // all loads first, so that they are in flight together
   int32_t c[SORT_PPT], tg[SORT_PPT];
   T v[5][SORT_PPT];
@@ -900,25 +905,63 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
       cnt[sl] = atomicAdd(ctr + key, n);  // the group's first output position
       int u = 33;
       while (gcd_int(u, n) != 1) u += 2;
-      s_n[sl] = n;
-      keys[sl] = inv_mod(u % n, n);  // the slot's key is no longer needed
+      s_n[sl] = n | (inv_mod(u % n, n) << 16);  // n, u^-1 mod n  (both < 2^12)
+      keys[sl] = atomicAdd(&s_total, n);        // the group's first local index
     }
   }
   if (HIST) return;
   __syncthreads();
+  // local index of each particle: groups contiguous, interleaved inside
 #pragma unroll
   for (int j = 0; j < SORT_PPT; ++j) {
     const int64_t i = base + threadIdx.x + SORT_NT * j;
     if (i >= a.n) continue;
-    const int n = s_n[slot[j]];
-    const int pos = cnt[slot[j]] + (n > 1 ? (rank[j] * keys[slot[j]]) % n : 0);
-    b.cell[pos] = c[j];
-    b.dx[pos] = v[0][j];
-    b.dy[pos] = v[1][j];
-    b.px[pos] = v[2][j];
-    b.py[pos] = v[3][j];
-    b.pz[pos] = v[4][j];
-    b.tag[pos] = tg[j];
+    const int nu = s_n[slot[j]], n = nu & 0xFFFF, inv = nu >> 16;
+    const int r = n > 1 ? (rank[j] * inv) % n : 0;
+    const int loc = keys[slot[j]] + r;
+    s_gpos[loc] = cnt[slot[j]] + r;
+    rank[j] = loc;  // from here on: the local index
+  }
+  const int m = (int)min((int64_t)CH, a.n - base);
+  // each value array goes through LDS so that the global writes follow the
+  // output order (contiguous within a group)
+#pragma unroll
+  for (int f = 0; f < 7; ++f) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SORT_PPT; ++j) {
+      const int64_t i = base + threadIdx.x + SORT_NT * j;
+      if (i >= a.n) continue;
+      int x = 0;
+      if (f == 0) x = c[j];
+      else if (f == 6) x = tg[j];
+      else if constexpr (sizeof(T) == 4) x = __builtin_bit_cast(int, (float)v[f >= 1 && f <= 5 ? f - 1 : 0][j]);
+      s_val[rank[j]] = x;
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < m; l += SORT_NT) {
+      const int g = s_gpos[l], x = s_val[l];
+      if (f == 0) b.cell[g] = x;
+      else if (f == 6) b.tag[g] = x;
+      else if constexpr (sizeof(T) == 4) {
+        T* dst = f == 1 ? b.dx : f == 2 ? b.dy : f == 3 ? b.px : f == 4 ? b.py : b.pz;
+        dst[g] = __builtin_bit_cast(float, x);
+      }
+    }
+  }
+  if constexpr (sizeof(T) != 4) {
+    // FP64 values do not fit the 32-bit staging: written directly
+#pragma unroll
+    for (int j = 0; j < SORT_PPT; ++j) {
+      const int64_t i = base + threadIdx.x + SORT_NT * j;
+      if (i >= a.n) continue;
+      const int g = s_gpos[rank[j]];
+      b.dx[g] = v[0][j];
+      b.dy[g] = v[1][j];
+      b.px[g] = v[2][j];
+      b.py[g] = v[3][j];
+      b.pz[g] = v[4][j];
+    }
   }
 }
 
