@@ -165,6 +165,14 @@ int sp_sync(sp_ctx* ctx);
  * the last reset; reset != 0 clears them. */
 int sp_set_timing(sp_ctx* ctx, int on);
 int sp_get_timers(sp_ctx* ctx, double* ms4, int reset);
+/* Step-loop capture (no reference counterpart; the loop of src/lib.rs:155-199
+ * is what gets captured).  With it on and timing off, sp_step runs whole
+ * periods of 2 * sort_interval steps (2 without sorting), starting on a
+ * period boundary with a push pending, as one replayed hipGraph: the buffer
+ * swaps of a period cancel, so one captured graph serves every period.  It is
+ * captured again when any buffer changed (species added, a stand-alone filter
+ * call).  Results are those of the eager steps. */
+int sp_set_graph(sp_ctx* ctx, int on);
 /* FP64 diagnostics (no reference counterpart): energies, momentum, Gauss
  * residuals. */
 int sp_get_diag(sp_ctx* ctx, sp_diag* out);

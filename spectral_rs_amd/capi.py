@@ -24,7 +24,8 @@ SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_spe
            "sp_species_count", "sp_get_species", "sp_get_grid", "sp_set_grid", "sp_get_spectrum",
            "sp_set_spectrum", "sp_zero_currents", "sp_move_and_deposit", "sp_deposit_ghosts",
            "sp_fields_update", "sp_boris_push", "sp_filter", "sp_fft2d", "sp_spectral_update",
-           "sp_prestep", "sp_step", "sp_sync", "sp_set_timing", "sp_get_timers", "sp_get_diag"]
+           "sp_prestep", "sp_step", "sp_sync", "sp_set_timing", "sp_get_timers", "sp_set_graph",
+           "sp_get_diag"]
 
 
 class SpParams(ct.Structure):
@@ -72,7 +73,7 @@ def load(path=LIB_PATH):
     for n in ["sp_zero_currents", "sp_deposit_ghosts", "sp_fields_update", "sp_spectral_update",
               "sp_prestep", "sp_sync"]:
         getattr(L, n).argtypes = [vp]
-    for n in ["sp_move_and_deposit", "sp_boris_push", "sp_step", "sp_set_timing"]:
+    for n in ["sp_move_and_deposit", "sp_boris_push", "sp_step", "sp_set_timing", "sp_set_graph"]:
         getattr(L, n).argtypes = [vp, i32]
     L.sp_filter.argtypes = [vp, i32, i32]
     L.sp_fft2d.argtypes = [vp, i32, i32]
@@ -189,6 +190,8 @@ class GpuSim:
     def step(self, n=1): self._chk(load().sp_step(self.h, n))
     def sync(self): self._chk(load().sp_sync(self.h))
     def set_timing(self, on=True): self._chk(load().sp_set_timing(self.h, int(on)))
+
+    def set_graph(self, on=True): self._chk(load().sp_set_graph(self.h, int(on)))
 
     def timers(self, reset=False):
         t = np.zeros(4, np.float64)

@@ -59,6 +59,10 @@ struct sp_ctx {
   int* tile_count = nullptr;   // per-tile particle counts of the tile sort
   int* tile_cursor = nullptr;  // per-tile write positions
   double* gauss_d = nullptr;   // Gauss-law sums of the last solve + block partials
+  // step-loop graph (sp_set_graph): the captured period and the buffers it uses
+  bool graph_on = false;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<const void*> gsig;
   double* diag_d = nullptr;    // energy / momentum sums of sp_get_diag
   std::vector<SpeciesBuf> species;
   std::string err;
@@ -263,6 +267,62 @@ template <class T> void diag_sums(sp_ctx* c) {
     launch_kinetic<T>(sdev<T>(c, c->species[s]), csqinv, c->diag_d + DIAG_FIXED + 4 * s, c->st);
 }
 
+// ---- step-loop graph ----
+// A period of steps whose buffer swaps cancel: the filter swaps grid and work
+// buffers once a step, a sort swaps a species' SoA buffers, so 2 sort
+// intervals (2 steps without sorting) return every pointer to where it was.
+int graph_period(const sp_ctx* c) { return c->p.sort_interval > 0 ? 2 * c->p.sort_interval : 2; }
+
+std::vector<const void*> graph_signature(const sp_ctx* c) {
+  std::vector<const void*> v(c->grid, c->grid + 11);
+  v.insert(v.end(), c->work, c->work + 4);
+  for (const auto& s : c->species) {
+    v.push_back(s.cur.cell);
+    v.push_back(s.alt.cell);
+    v.push_back((const void*)(intptr_t)s.n);
+  }
+  return v;
+}
+
+void drop_graph(sp_ctx* c) {
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  c->gsig.clear();
+}
+
+template <class T> void one_step(sp_ctx* c);
+
+// Runs one period through the graph, capturing it first when there is none
+// for the current buffers.  Capture advances the host-side state (step count,
+// swaps) while recording; a replay advances only the step count, its swaps
+// cancelling.  The fills inside the step are kernels (no memset nodes).
+template <class T> int graph_period_run(sp_ctx* c) {
+  const int P = graph_period(c);
+  const std::vector<const void*> sig = graph_signature(c);
+  if (!c->gexec || sig != c->gsig) {
+    drop_graph(c);
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < P; ++i) one_step<T>(c);
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamEndCapture(c->st, &graph));
+    const hipError_t e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      c->gexec = nullptr;
+      return fail(c, SP_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    if (graph_signature(c) != sig) {  // cannot happen by the period's construction
+      drop_graph(c);
+      return fail(c, SP_EHIP, "captured period does not return the buffers to their start");
+    }
+    c->gsig = sig;
+  } else {
+    c->steps += P;
+  }
+  HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+  return SP_OK;
+}
+
 // Tile-sorts every species (particle order only; no value changes).
 template <class T> void sort_species(sp_ctx* c) {
   for (auto& s : c->species) {
@@ -447,6 +507,7 @@ int sp_create(const sp_params* params, sp_ctx** out) {
 int sp_destroy(sp_ctx* c) {
   if (!c) return SP_OK;
   if (c->st) hipStreamSynchronize(c->st);
+  drop_graph(c);
   for (auto& s : c->species) free_species(s);
   for (auto g : c->grid) hipFree(g);
   for (auto g : c->work) hipFree(g);
@@ -709,11 +770,26 @@ int sp_prestep(sp_ctx* c) {
 int sp_step(sp_ctx* c, int n) {
   if (!c) return SP_EINVAL;
   if (n < 0) return fail(c, SP_EINVAL, "negative step count");
-  for (int i = 0; i < n; ++i) {
-    if (c->f64) one_step<double>(c);
-    else one_step<float>(c);
+  for (int i = 0; i < n;) {
+    const int P = graph_period(c);
+    if (c->graph_on && !c->timing && c->push_pending && c->steps % P == 0 && n - i >= P) {
+      const int rc = c->f64 ? graph_period_run<double>(c) : graph_period_run<float>(c);
+      if (rc) return rc;
+      i += P;
+    } else {
+      if (c->f64) one_step<double>(c);
+      else one_step<float>(c);
+      ++i;
+    }
   }
   HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_set_graph(sp_ctx* c, int on) {
+  if (!c) return SP_EINVAL;
+  c->graph_on = on != 0;
+  if (!c->graph_on) drop_graph(c);
   return SP_OK;
 }
 
