@@ -177,6 +177,41 @@ int sp_set_graph(sp_ctx* ctx, int on);
  * residuals. */
 int sp_get_diag(sp_ctx* ctx, sp_diag* out);
 
+/* ---- Learned current denoiser (no reference counterpart) ----
+ * A 3-layer 3x3 convolutional network (4 -> 16 -> 16 -> 4 channels, ReLU) on
+ * BF16 MFMA with FP32 accumulation.  Input x: Jx Jy Jz rho after the binomial
+ * filter, each divided by dens and a channel scale; output x + r.  Parameters,
+ * one FP32 vector of SP_DN_NPARAMS: w1 [16][4*9], b1 [16], w2 [16][16*9],
+ * b2 [16], w3 [4][16*9], b3 [4], scales [4] (w[co][ci*9 + ky*3 + kx]); the
+ * first SP_DN_NTRAIN are trained. */
+#define SP_DN_NTRAIN 3492
+#define SP_DN_NPARAMS 3496
+typedef struct sp_dn sp_dn;
+/* A training context for batches of `batch` images of h x w, 4 planes each
+ * ([batch][4][h][w], zero outside the image). */
+int sp_dn_create(int batch, int h, int w, sp_dn** out);
+int sp_dn_destroy(sp_dn* dn);
+const char* sp_dn_last_error(const sp_dn* dn);
+int sp_dn_set_params(sp_dn* dn, const float* params, int n);
+int sp_dn_get_params(sp_dn* dn, float* params, int n);
+/* y = x + r(x) for one batch (host arrays). */
+int sp_dn_forward(sp_dn* dn, const float* x, float* y);
+/* Mean squared error of x + r(x) against t and its gradient (SP_DN_NTRAIN
+ * entries) for one batch. */
+int sp_dn_gradient(sp_dn* dn, const float* x, const float* t, float* grad, double* loss);
+/* `iters` Adam steps (lr, betas 0.9/0.999) on batches drawn with `seed` from
+ * nsamples uploaded [4][h][w] pairs (xs, ts; NULL keeps the last upload);
+ * loss_hist (iters entries, may be NULL) gets each batch's mean loss. */
+int sp_dn_train(sp_dn* dn, const float* xs, const float* ts, int nsamples, int iters, float lr, uint64_t seed,
+                double* loss_hist);
+/* Apply a trained network after the binomial filter in every field solve of
+ * the context (FP32 only).  Fallback to the binomial-filtered currents when
+ * |r| > max_rel |x| or r is not finite, or when the previous solve's relative
+ * pre-correction Gauss residual exceeds max_gauss.  params NULL disables. */
+int sp_set_denoiser(sp_ctx* ctx, const float* params, int n, float max_rel, float max_gauss);
+/* Solves that applied the network / fell back since the denoiser was set. */
+int sp_get_denoiser_stats(sp_ctx* ctx, int* applied, int* fallbacks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,7 +14,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libspectral_rs_hip.so")
 BINDIR = os.path.join(HERE, "bin")
 HOST = os.path.join(BINDIR, "spectral_rs")
-SOURCES = ["api.hip", "particles.hip", "fields.hip", "fft.hip", "diag.hip"]
+SOURCES = ["api.hip", "particles.hip", "fields.hip", "fft.hip", "diag.hip", "denoise.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

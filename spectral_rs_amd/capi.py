@@ -25,7 +25,10 @@ SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_spe
            "sp_set_spectrum", "sp_zero_currents", "sp_move_and_deposit", "sp_deposit_ghosts",
            "sp_fields_update", "sp_boris_push", "sp_filter", "sp_fft2d", "sp_spectral_update",
            "sp_prestep", "sp_step", "sp_sync", "sp_set_timing", "sp_get_timers", "sp_set_graph",
-           "sp_get_diag"]
+           "sp_get_diag", "sp_dn_create", "sp_dn_destroy", "sp_dn_last_error", "sp_dn_set_params",
+           "sp_dn_get_params", "sp_dn_forward", "sp_dn_gradient", "sp_dn_train", "sp_set_denoiser",
+           "sp_get_denoiser_stats"]
+DN_NTRAIN, DN_NPARAMS = 3492, 3496
 
 
 class SpParams(ct.Structure):
@@ -79,6 +82,17 @@ def load(path=LIB_PATH):
     L.sp_fft2d.argtypes = [vp, i32, i32]
     L.sp_get_timers.argtypes = [vp, vp, i32]
     L.sp_get_diag.argtypes = [vp, ct.POINTER(SpDiag)]
+    L.sp_dn_create.argtypes = [i32, i32, i32, ct.POINTER(vp)]
+    L.sp_dn_destroy.argtypes = [vp]
+    L.sp_dn_last_error.argtypes = [vp]
+    L.sp_dn_last_error.restype = ct.c_char_p
+    L.sp_dn_set_params.argtypes = [vp, vp, i32]
+    L.sp_dn_get_params.argtypes = [vp, vp, i32]
+    L.sp_dn_forward.argtypes = [vp, vp, vp]
+    L.sp_dn_gradient.argtypes = [vp, vp, vp, vp, ct.POINTER(ct.c_double)]
+    L.sp_dn_train.argtypes = [vp, vp, vp, i32, i32, ct.c_float, ct.c_uint64, vp]
+    L.sp_set_denoiser.argtypes = [vp, vp, i32, ct.c_float, ct.c_float]
+    L.sp_get_denoiser_stats.argtypes = [vp, ct.POINTER(i32), ct.POINTER(i32)]
     for n in SYMBOLS:
         if n not in ("sp_last_error",):
             getattr(L, n).restype = ct.c_int
@@ -208,3 +222,79 @@ class GpuSim:
                 "momentum": [d.momentum[k] for k in range(3)],
                 "gauss_pre": d.gauss_pre, "gauss_post": d.gauss_post, "gauss_post_max": d.gauss_post_max,
                 "steps": d.steps}
+
+    def set_denoiser(self, params, max_rel=0.5, max_gauss=1.0):
+        """Apply a trained denoiser after the binomial filter (params None disables)."""
+        if params is None:
+            self._chk(load().sp_set_denoiser(self.h, None, 0, 0.0, 0.0))
+            return
+        p = np.ascontiguousarray(params, np.float32)
+        self._chk(load().sp_set_denoiser(self.h, _p(p), p.size, max_rel, max_gauss))
+
+    def denoiser_stats(self):
+        a, f = ct.c_int32(), ct.c_int32()
+        self._chk(load().sp_get_denoiser_stats(self.h, ct.byref(a), ct.byref(f)))
+        return a.value, f.value
+
+
+class Denoiser:
+    """Training context of the learned current denoiser (sp_dn_*): batches of
+    `batch` images of h x w with 4 planes (Jx Jy Jz rho)."""
+
+    def __init__(self, batch, h, w):
+        L = load()
+        self.batch, self.h, self.w = batch, h, w
+        d = ct.c_void_p()
+        rc = L.sp_dn_create(batch, h, w, ct.byref(d))
+        if rc != SP_OK:
+            raise SpError(rc, L.sp_dn_last_error(None).decode())
+        self.d = d
+
+    def close(self):
+        if getattr(self, "d", None):
+            load().sp_dn_destroy(self.d)
+            self.d = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != SP_OK:
+            raise SpError(rc, load().sp_dn_last_error(self.d).decode())
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._chk(load().sp_dn_set_params(self.d, _p(p), p.size))
+
+    def params(self):
+        p = np.zeros(DN_NPARAMS, np.float32)
+        self._chk(load().sp_dn_get_params(self.d, _p(p), p.size))
+        return p
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty_like(x)
+        self._chk(load().sp_dn_forward(self.d, _p(x), _p(y)))
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(DN_NTRAIN, np.float32)
+        loss = ct.c_double()
+        self._chk(load().sp_dn_gradient(self.d, _p(x), _p(t), _p(g), ct.byref(loss)))
+        return loss.value, g
+
+    def train(self, xs, ts, iters, lr, seed=0):
+        """Adam on random batches of the pairs (xs, ts) [n][4][h][w]; xs None reuses the last upload."""
+        hist = np.zeros(iters, np.float64)
+        if xs is None:
+            self._chk(load().sp_dn_train(self.d, None, None, 0, iters, lr, seed, _p(hist)))
+        else:
+            xs = np.ascontiguousarray(xs, np.float32)
+            ts = np.ascontiguousarray(ts, np.float32)
+            self._chk(load().sp_dn_train(self.d, _p(xs), _p(ts), xs.shape[0], iters, lr, seed, _p(hist)))
+        return hist

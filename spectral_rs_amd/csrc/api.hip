@@ -59,6 +59,12 @@ struct sp_ctx {
   int* tile_count = nullptr;   // per-tile particle counts of the tile sort
   int* tile_cursor = nullptr;  // per-tile write positions
   double* gauss_d = nullptr;   // Gauss-law sums of the last solve + block partials
+  // learned denoiser (sp_set_denoiser): parameters, activations, sums, counters
+  bool dn_on = false;
+  float* dn_buf = nullptr;  // parameters + x (4) + a1 (16) + a2 (16) + r (4) planes
+  double* dn_sums = nullptr;
+  int* dn_stats = nullptr;
+  float dn_tol = 0, dn_gtol = 0;
   // step-loop graph (sp_set_graph): the captured period and the buffers it uses
   bool graph_on = false;
   hipGraphExec_t gexec = nullptr;
@@ -216,11 +222,21 @@ template <class T> void flush_push(sp_ctx* c) {
   c->push_pending = false;
 }
 
+DnTensors dn_tensors(const sp_ctx* c);
+
 template <class T> void fields_update(sp_ctx* c) {
   const int nx = c->p.size_x, ny = c->p.size_y;
   // mod.rs:90-94 (Jx Jy Jz rho are consecutive grid ids)
   static_assert(SP_JY == SP_JX + 1 && SP_JZ == SP_JX + 2 && SP_RHO == SP_JX + 3, "grid ids");
   launch_binomial<T>(&c->grid[SP_JX], c->work, 4, nx, ny, c->p.n_pass, c->st);
+  if constexpr (sizeof(T) == 4) {
+    if (c->dn_on) {  // learned correction of the filtered currents, or the fallback
+      float* g4[4] = {(float*)c->grid[SP_JX], (float*)c->grid[SP_JY], (float*)c->grid[SP_JZ], (float*)c->grid[SP_RHO]};
+      dn_apply_sim(dn_params(c->dn_buf), dn_tensors(c), g4, nx, ny, c->p.dens, c->dn_sums, c->gauss_d, c->dn_tol,
+                   c->dn_gtol, c->dn_stats, c->st);
+      launch_update_ghosts<float>(g4, 4, nx, ny, c->st);
+    }
+  }
   {  // mod.rs:97-103, 122-124: 7 forward transforms, spatial->spectral fused
     FftBatch b{};
     const int ids[7] = {SP_JX, SP_JY, SP_JZ, SP_EX, SP_EY, SP_EZ, SP_RHO};
@@ -265,6 +281,17 @@ template <class T> void diag_sums(sp_ctx* c) {
   const double csqinv = 1.0 / ((double)c->p.c * c->p.c);
   for (size_t s = 0; s < c->species.size(); ++s)
     launch_kinetic<T>(sdev<T>(c, c->species[s]), csqinv, c->diag_d + DIAG_FIXED + 4 * s, c->st);
+}
+
+// Activation planes of the in-loop denoiser, after the parameters in dn_buf.
+DnTensors dn_tensors(const sp_ctx* c) {
+  const size_t n = c->nspec_elems();
+  DnTensors t{};
+  t.x = c->dn_buf + DN_NPARAMS;
+  t.a1 = t.x + 4 * n;
+  t.a2 = t.a1 + 16 * n;
+  t.r = t.a2 + 16 * n;
+  return t;
 }
 
 // ---- step-loop graph ----
@@ -508,6 +535,9 @@ int sp_destroy(sp_ctx* c) {
   if (!c) return SP_OK;
   if (c->st) hipStreamSynchronize(c->st);
   drop_graph(c);
+  hipFree(c->dn_buf);
+  hipFree(c->dn_sums);
+  hipFree(c->dn_stats);
   for (auto& s : c->species) free_species(s);
   for (auto g : c->grid) hipFree(g);
   for (auto g : c->work) hipFree(g);
@@ -783,6 +813,42 @@ int sp_step(sp_ctx* c, int n) {
     }
   }
   HIPCHK(c, hipGetLastError());
+  return SP_OK;
+}
+
+int sp_set_denoiser(sp_ctx* c, const float* params, int n, float max_rel, float max_gauss) {
+  ENTRY_PRELUDE(c);
+  if (!params) {
+    c->dn_on = false;
+    return SP_OK;
+  }
+  if (c->f64) return fail(c, SP_EINVAL, "the denoiser runs in FP32 contexts only");
+  if (n != DN_NPARAMS) return fail(c, SP_EINVAL, "expected SP_DN_NPARAMS parameters");
+  if (!(max_rel > 0) || !(max_gauss > 0)) return fail(c, SP_EINVAL, "tolerances must be positive");
+  if (!c->dn_buf) {
+    const size_t planes = 40 * c->nspec_elems();
+    HIPCHK(c, hipMalloc(&c->dn_buf, (DN_NPARAMS + planes) * sizeof(float)));
+    HIPCHK(c, hipMalloc(&c->dn_sums, 2 * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->dn_stats, 2 * sizeof(int)));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->dn_buf, params, DN_NPARAMS * sizeof(float), hipMemcpyHostToDevice, c->st));
+  fill<int>(c->dn_stats, 0, 2, c->st);
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->dn_tol = max_rel;
+  c->dn_gtol = max_gauss;
+  c->dn_on = true;
+  return SP_OK;
+}
+
+int sp_get_denoiser_stats(sp_ctx* c, int* applied, int* fallbacks) {
+  if (!c || !applied || !fallbacks) return SP_EINVAL;
+  int h[2] = {0, 0};
+  if (c->dn_stats) {
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(h, c->dn_stats, sizeof(h), hipMemcpyDeviceToHost));
+  }
+  *applied = h[0];
+  *fallbacks = h[1];
   return SP_OK;
 }
 

@@ -1,0 +1,533 @@
+// denoise.hip -- learned current denoiser (no reference counterpart): a small
+// 2D convolutional network on BF16 MFMA with FP32 accumulation, trained and
+// run by the kernels below, applied after the binomial filter at the point of
+// src/flds/mod.rs:90-94.
+//
+// Network: x (4 channels: Jx Jy Jz rho after the binomial filter, each divided
+// by dens and by a per-channel scale s_c) -> conv3x3(4->16) + b, ReLU ->
+// conv3x3(16->16) + b, ReLU -> conv3x3(16->4) + b = r; output x + r, times
+// s_c dens.  "same" 3x3 convolutions; outside the image the input is zero,
+// except that in the simulation y is periodic.
+//
+// Every convolution is an implicit GEMM on v_mfma_f32_16x16x16bf16_1k: a
+// workgroup owns 8 x 32 output pixels, stages its input tile (+1 halo) and the
+// weights in LDS as BF16, and each wave computes 16-pixel x 16-channel blocks,
+// the K dimension being (input channel, 3x3 tap) padded to 16.  Operand
+// layouts (checked on gfx950): A row = lane % 16, k = 4 (lane / 16) + i;
+// B column = lane % 16, same k; D rows 4 (lane / 16) + i, column lane % 16.
+// The weight gradient is the same MFMA with pixels as the K dimension.
+#include "sp_internal.h"
+
+namespace sp {
+
+typedef __attribute__((ext_vector_type(4))) short bf16x4;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ short bf16_bits(float f) {
+  const unsigned u = __builtin_bit_cast(unsigned, f);
+  return (short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+constexpr int DN_TY = 8, DN_TX = 32, DN_NT = 256;  // output pixels of a workgroup: 8 rows x 32 columns
+
+template <int CIN> struct ConvShape {
+  static constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16, LY = DN_TY + 2, LX = DN_TX + 2;
+};
+
+// Stages rows y0-1 .. y0+DN_TY and columns x0-1 .. x0+DN_TX of CIN planes of
+// one image as BF16.  PERIODIC_Y wraps rows; columns outside are zero.
+template <int CIN, bool PERIODIC_Y>
+__device__ __forceinline__ void stage_tile(short* s_in, const float* __restrict__ img, int H, int W, int y0, int x0) {
+  using S = ConvShape<CIN>;
+  for (int i = threadIdx.x; i < CIN * S::LY * S::LX; i += DN_NT) {
+    const int ci = i / (S::LY * S::LX), rc = i - ci * (S::LY * S::LX);
+    const int r = rc / S::LX, c = rc - r * S::LX;
+    int y = y0 + r - 1;
+    const int x = x0 + c - 1;
+    if (PERIODIC_Y) y = y < 0 ? y + H : (y >= H ? y - H : y);
+    float v = 0.0f;
+    if (y >= 0 && y < H && x >= 0 && x < W) v = img[((size_t)ci * H + y) * W + x];
+    s_in[i] = bf16_bits(v);
+  }
+}
+
+template <int CIN> __device__ __forceinline__ void stage_offsets(int* s_off) {
+  using S = ConvShape<CIN>;
+  for (int kk = threadIdx.x; kk < S::KP; kk += DN_NT) {
+    int o = 0;  // padded taps read any valid element: their weight is zero
+    if (kk < S::K) {
+      const int ci = kk / 9, t = kk - ci * 9;
+      o = ci * S::LY * S::LX + (t / 3) * S::LX + (t % 3);
+    }
+    s_off[kk] = o;
+  }
+}
+
+// out[b][n][y][x] = post(bias[n] + sum_{ci,ky,kx} w[n][ci*9+ky*3+kx] in[b][ci][y+ky-1][x+kx-1]).
+// POST: 0 none, 1 ReLU, 2 times (mask > 0) -- the ReLU derivative when the
+// layer is a backward (transposed) convolution and mask the forward activation.
+template <int CIN, int COUT, int POST, bool PERIODIC_Y>
+__global__ void __launch_bounds__(DN_NT) k_conv3x3(const float* __restrict__ in, const float* __restrict__ w,
+                                                   const float* __restrict__ bias, const float* __restrict__ mask,
+                                                   float* __restrict__ out, int H, int W) {
+  using S = ConvShape<CIN>;
+  __shared__ short s_in[CIN * S::LY * S::LX];
+  __shared__ short s_w[16 * S::KP];
+  __shared__ int s_off[S::KP];
+  const int b = blockIdx.z, y0 = blockIdx.y * DN_TY, x0 = blockIdx.x * DN_TX;
+  stage_tile<CIN, PERIODIC_Y>(s_in, in + (size_t)b * CIN * H * W, H, W, y0, x0);
+  for (int i = threadIdx.x; i < 16 * S::KP; i += DN_NT) {
+    const int n = i / S::KP, kk = i - n * S::KP;
+    s_w[i] = (n < COUT && kk < S::K) ? bf16_bits(w[n * S::K + kk]) : (short)0;
+  }
+  stage_offsets<CIN>(s_off);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = (lane >> 4) * 4;
+  for (int t = wave; t < (DN_TY * DN_TX) / 16; t += DN_NT / 64) {
+    const int py = t / (DN_TX / 16), px = (t % (DN_TX / 16)) * 16;
+    const int pbase = py * S::LX + px + m;  // window corner of pixel (py, px + m) in the staged tile
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < S::KP; kc += 16) {
+      bf16x4 a, bv;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int kk = kc + kq + i;
+        a[i] = s_in[pbase + s_off[kk]];
+        bv[i] = s_w[m * S::KP + kk];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+    }
+    const int n = lane & 15;
+    if (n < COUT) {
+      const float bn = bias ? bias[n] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int y = y0 + py, x = x0 + px + kq + i;
+        if (y < H && x < W) {
+          const size_t o = (((size_t)b * COUT + n) * H + y) * W + x;
+          float v = acc[i] + bn;
+          if (POST == 1) v = fmaxf(v, 0.0f);
+          if (POST == 2) v = mask[o] > 0.0f ? v : 0.0f;
+          out[o] = v;
+        }
+      }
+    }
+  }
+}
+
+// dw[n][kk] += sum_{b,pixels} dy[b][n][p] im2col(in)[p][kk]; db[n] += sum dy[b][n][p].
+// Pixels are the K dimension: A = dy^T (16 channels x 16 pixels), B = im2col
+// (16 pixels x 16 taps).  Zero boundary (training images).
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(DN_NT) k_conv3x3_wgrad(const float* __restrict__ in, const float* __restrict__ dy,
+                                                         float* __restrict__ dw, float* __restrict__ db, int H,
+                                                         int W) {
+  using S = ConvShape<CIN>;
+  constexpr int NP = DN_TY * DN_TX;
+  __shared__ short s_in[CIN * S::LY * S::LX];
+  __shared__ short s_dy[16 * NP];
+  __shared__ int s_off[S::KP];
+  const int b = blockIdx.z, y0 = blockIdx.y * DN_TY, x0 = blockIdx.x * DN_TX;
+  stage_tile<CIN, false>(s_in, in + (size_t)b * CIN * H * W, H, W, y0, x0);
+  for (int i = threadIdx.x; i < 16 * NP; i += DN_NT) {
+    const int n = i / NP, p = i - n * NP;
+    const int y = y0 + p / DN_TX, x = x0 + p % DN_TX;
+    float v = 0.0f;
+    if (n < COUT && y < H && x < W) v = dy[(((size_t)b * COUT + n) * H + y) * W + x];
+    s_dy[i] = bf16_bits(v);
+  }
+  stage_offsets<CIN>(s_off);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = (lane >> 4) * 4;
+  for (int kc = wave * 16; kc < S::KP; kc += 16 * (DN_NT / 64)) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int off = s_off[kc + m];  // this lane's tap (column of B)
+    for (int pc = 0; pc < NP; pc += 16) {
+      bf16x4 a, bv;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int p = pc + kq + i;  // pixel (k index)
+        a[i] = s_dy[m * NP + p];
+        bv[i] = s_in[(p / DN_TX) * S::LX + (p % DN_TX) + off];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+    }
+    // D rows = output channel 4 (lane / 16) + i, column = tap kc + lane % 16
+    const int kk = kc + m;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int n = kq + i;
+      if (n < COUT && kk < S::K) atomicAdd(dw + n * S::K + kk, acc[i]);
+    }
+  }
+  if (threadIdx.x < COUT * 16) {  // bias: 16 threads per channel, FP32 sums of dy
+    const int n = threadIdx.x / 16, part = threadIdx.x % 16;
+    float s = 0.0f;
+    for (int p = part; p < NP; p += 16) {
+      const int y = y0 + p / DN_TX, x = x0 + p % DN_TX;
+      if (y < H && x < W) s += dy[(((size_t)b * COUT + n) * H + y) * W + x];
+    }
+    atomicAdd(db + n, s);
+  }
+}
+
+// Backward-data weights: wt[ci][co*9 + 8 - t] = w[co][ci*9 + t] (the
+// transposed, flipped kernel of a "same" convolution with zero boundary).
+__global__ void k_flip_weights(const float* __restrict__ w, float* __restrict__ wt, int cin, int cout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cin * cout * 9) return;
+  const int co = i / (cin * 9), r = i - co * cin * 9, ci = r / 9, t = r - ci * 9;
+  wt[ci * cout * 9 + co * 9 + (8 - t)] = w[i];
+}
+
+// pred = x + r; loss += sum (pred - t)^2 (FP64); dr = 2 (pred - t) / n.
+__global__ void k_mse(const float* __restrict__ x, const float* __restrict__ r, const float* __restrict__ t,
+                      float* __restrict__ dr, double* loss, int64_t n) {
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float d = x[i] + r[i] - t[i];
+    dr[i] = 2.0f * d / (float)n;
+    acc += (double)d * d;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss, acc);
+}
+
+// Adam (beta1 0.9, beta2 0.999, eps 1e-8) on n parameters; step count t >= 1.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, int n, float lr, int t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.0f - b1) * gi;
+  const float v = b2 * m2[i] + (1.0f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = v;
+  const float ah = a / (1.0f - powf(b1, (float)t)), vh = v / (1.0f - powf(b2, (float)t));
+  p[i] -= lr * ah / (sqrtf(vh) + 1e-8f);
+}
+
+// ---- simulation side: pack the 4 filtered grids, apply the correction ----
+struct Grids4 {
+  float* g[4];
+};
+
+// x[c][y][x] = grid_c(interior) / (dens s_c); sums |x|^2 into sums[0].
+__global__ void k_dn_pack(Grids4 gr, const float* __restrict__ scale, float inv_dens, float* __restrict__ x, int nx,
+                          int ny, double* sums) {
+  const int64_t n = (int64_t)nx * ny;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i / n);
+    const int64_t p = i - c * n;
+    const int yy = (int)(p / nx), xx = (int)(p - (int64_t)yy * nx);
+    const float v = gr.g[c][(int64_t)(yy + 1) * (nx + 2) + xx + 1] * inv_dens / scale[c];
+    x[i] = v;
+    acc += (double)v * v;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd(sums, acc);
+}
+
+__global__ void k_dn_sumsq(const float* __restrict__ r, int64_t n, double* sums) {
+  double acc = 0.0;
+  bool finite = true;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = r[i];
+    finite = finite && isfinite(v);
+    acc += (double)v * v;
+  }
+  if (!finite) acc = __builtin_inf();
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd(sums + 1, acc);
+}
+
+// Writes dens s_c (x + r) into the interior of the grids when the correction
+// passes the checks: |r| <= tol |x| (finite), and the previous solve's
+// pre-correction Gauss residual (gauss[0] / gauss[2], relative) is below
+// gauss_tol.  Otherwise the binomial-filtered grids stay (the fallback).
+// stats[0] counts applications, stats[1] fallbacks.
+__global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float dens, const float* __restrict__ x,
+                           const float* __restrict__ r, int nx, int ny, const double* __restrict__ sums,
+                           const double* __restrict__ gauss, float tol, float gauss_tol, int* stats) {
+  const double ratio2 = sums[0] > 0 ? sums[1] / sums[0] : 0.0;
+  const double g2 = gauss[2] > 0 ? gauss[0] / gauss[2] : 0.0;
+  const bool ok = ratio2 <= (double)tol * tol && g2 <= (double)gauss_tol * gauss_tol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stats + (ok ? 0 : 1), 1);
+  if (!ok) return;
+  const int64_t n = (int64_t)nx * ny;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i / n);
+    const int64_t p = i - c * n;
+    const int yy = (int)(p / nx), xx = (int)(p - (int64_t)yy * nx);
+    gr.g[c][(int64_t)(yy + 1) * (nx + 2) + xx + 1] = dens * scale[c] * (x[i] + r[i]);
+  }
+}
+
+// ---- launchers ----
+template <int CIN, int COUT, int POST, bool PY>
+static void conv(const float* in, const float* w, const float* bias, const float* mask, float* out, int B, int H,
+                 int W, hipStream_t st) {
+  dim3 grid((W + DN_TX - 1) / DN_TX, (H + DN_TY - 1) / DN_TY, B);
+  hipLaunchKernelGGL((k_conv3x3<CIN, COUT, POST, PY>), grid, dim3(DN_NT), 0, st, in, w, bias, mask, out, H, W);
+}
+template <int CIN, int COUT>
+static void wgrad(const float* in, const float* dy, float* dw, float* db, int B, int H, int W, hipStream_t st) {
+  dim3 grid((W + DN_TX - 1) / DN_TX, (H + DN_TY - 1) / DN_TY, B);
+  hipLaunchKernelGGL((k_conv3x3_wgrad<CIN, COUT>), grid, dim3(DN_NT), 0, st, in, dy, dw, db, H, W);
+}
+static unsigned nblk(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return (unsigned)(b > 4096 ? 4096 : (b < 1 ? 1 : b));
+}
+
+void dn_forward(const DnParams& P, const DnTensors& T, int B, int H, int W, bool periodic_y, hipStream_t st) {
+  if (periodic_y) {
+    conv<4, 16, 1, true>(T.x, P.w1, P.b1, nullptr, T.a1, B, H, W, st);
+    conv<16, 16, 1, true>(T.a1, P.w2, P.b2, nullptr, T.a2, B, H, W, st);
+    conv<16, 4, 0, true>(T.a2, P.w3, P.b3, nullptr, T.r, B, H, W, st);
+  } else {
+    conv<4, 16, 1, false>(T.x, P.w1, P.b1, nullptr, T.a1, B, H, W, st);
+    conv<16, 16, 1, false>(T.a1, P.w2, P.b2, nullptr, T.a2, B, H, W, st);
+    conv<16, 4, 0, false>(T.a2, P.w3, P.b3, nullptr, T.r, B, H, W, st);
+  }
+}
+
+// Loss and gradients of one batch (zero boundary): T.x, T.t set; grad has
+// DN_NTRAIN entries in the parameter layout; loss[0] gets sum (pred - t)^2.
+void dn_backward(const DnParams& P, DnParams G, const DnTensors& T, int B, int H, int W, double* loss,
+                 hipStream_t st) {
+  const int64_t n4 = (int64_t)B * 4 * H * W;
+  hipLaunchKernelGGL(k_mse, dim3(nblk(n4)), dim3(256), 0, st, T.x, T.r, T.t, T.dr, loss, n4);
+  // layer 3: r = conv(a2, w3)
+  wgrad<16, 4>(T.a2, T.dr, G.w3, G.b3, B, H, W, st);
+  hipLaunchKernelGGL(k_flip_weights, dim3((16 * 4 * 9 + 255) / 256), dim3(256), 0, st, P.w3, T.wt3, 16, 4);
+  conv<4, 16, 2, false>(T.dr, T.wt3, nullptr, T.a2, T.da2, B, H, W, st);
+  // layer 2: a2 = relu(conv(a1, w2))
+  wgrad<16, 16>(T.a1, T.da2, G.w2, G.b2, B, H, W, st);
+  hipLaunchKernelGGL(k_flip_weights, dim3((16 * 16 * 9 + 255) / 256), dim3(256), 0, st, P.w2, T.wt2, 16, 16);
+  conv<16, 16, 2, false>(T.da2, T.wt2, nullptr, T.a1, T.da1, B, H, W, st);
+  // layer 1: a1 = relu(conv(x, w1))
+  wgrad<4, 16>(T.x, T.da1, G.w1, G.b1, B, H, W, st);
+}
+
+void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, int t, hipStream_t st) {
+  hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, p, g, m1, m2, n, lr, t);
+}
+
+void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, int nx, int ny, int dens, double* sums,
+                  const double* gauss, float tol, float gauss_tol, int* stats, hipStream_t st) {
+  Grids4 gr;
+  for (int c = 0; c < 4; ++c) gr.g[c] = grids[c];
+  const int64_t n4 = (int64_t)4 * nx * ny;
+  fill<double>(sums, 0.0, 2, st);
+  hipLaunchKernelGGL(k_dn_pack, dim3(nblk(n4)), dim3(256), 0, st, gr, P.scale, 1.0f / (float)dens, T.x, nx, ny,
+                     sums);
+  dn_forward(P, T, 1, ny, nx, true, st);
+  hipLaunchKernelGGL(k_dn_sumsq, dim3(nblk(n4)), dim3(256), 0, st, T.r, n4, sums);
+  hipLaunchKernelGGL(k_dn_apply, dim3(nblk(n4)), dim3(256), 0, st, gr, P.scale, (float)dens, T.x, T.r, nx, ny, sums,
+                     gauss, tol, gauss_tol, stats);
+}
+
+}  // namespace sp
+
+// ------------------------------------------------------------ C ABI
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+struct sp_dn {
+  int B = 0, H = 0, W = 0;
+  hipStream_t st = nullptr;
+  float *params = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
+  sp::DnTensors T{};
+  double* loss = nullptr;
+  float *xs = nullptr, *ts = nullptr;  // uploaded training samples
+  int64_t nsamples = 0;
+  int adam_t = 0;
+  std::vector<void*> allocs;
+  std::string err;
+};
+
+namespace {
+std::string g_dn_create_error;
+int dn_fail(sp_dn* d, int code, const std::string& m) {
+  if (d) d->err = m;
+  else g_dn_create_error = m;
+  return code;
+}
+#define DNCHK(d, call)                                                                                     \
+  do {                                                                                                     \
+    hipError_t e_ = (call);                                                                                \
+    if (e_ != hipSuccess) return dn_fail(d, SP_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+int dn_alloc(sp_dn* d, float** p, size_t n) {
+  void* q = nullptr;
+  DNCHK(d, hipMalloc(&q, n * sizeof(float)));
+  d->allocs.push_back(q);
+  *p = (float*)q;
+  return SP_OK;
+}
+// forward + backward on the batch in d->T.x / d->T.t; gradients in d->grad,
+// the sum of squared errors in d->loss[0]
+int dn_step_grad(sp_dn* d) {
+  sp::fill<float>(d->grad, 0.0f, sp::DN_NPARAMS, d->st);
+  sp::fill<double>(d->loss, 0.0, 1, d->st);
+  const sp::DnParams P = sp::dn_params(d->params);
+  sp::dn_forward(P, d->T, d->B, d->H, d->W, false, d->st);
+  sp::dn_backward(P, sp::dn_params(d->grad), d->T, d->B, d->H, d->W, d->loss, d->st);
+  DNCHK(d, hipGetLastError());
+  return SP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sp_dn_create(int batch, int h, int w, sp_dn** out) {
+  if (!out || batch < 1 || h < 1 || w < 1) return dn_fail(nullptr, SP_EINVAL, "bad denoiser shape");
+  sp_dn* d = new sp_dn();
+  d->B = batch;
+  d->H = h;
+  d->W = w;
+  auto bail = [&](int rc) {
+    g_dn_create_error = d->err;
+    for (void* p : d->allocs) hipFree(p);
+    if (d->st) hipStreamDestroy(d->st);
+    delete d;
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking) != hipSuccess)
+    return bail(dn_fail(d, SP_EHIP, "hipStreamCreateWithFlags failed"));
+  const size_t px = (size_t)batch * h * w;
+  int rc = 0;
+  rc |= dn_alloc(d, &d->params, sp::DN_NPARAMS);
+  rc |= dn_alloc(d, &d->grad, sp::DN_NPARAMS);
+  rc |= dn_alloc(d, &d->m1, sp::DN_NPARAMS);
+  rc |= dn_alloc(d, &d->m2, sp::DN_NPARAMS);
+  rc |= dn_alloc(d, &d->T.x, 4 * px);
+  rc |= dn_alloc(d, &d->T.t, 4 * px);
+  rc |= dn_alloc(d, &d->T.r, 4 * px);
+  rc |= dn_alloc(d, &d->T.dr, 4 * px);
+  rc |= dn_alloc(d, &d->T.a1, 16 * px);
+  rc |= dn_alloc(d, &d->T.a2, 16 * px);
+  rc |= dn_alloc(d, &d->T.da1, 16 * px);
+  rc |= dn_alloc(d, &d->T.da2, 16 * px);
+  rc |= dn_alloc(d, &d->T.wt2, 16 * 144);
+  rc |= dn_alloc(d, &d->T.wt3, 16 * 36);
+  float* lossf = nullptr;
+  rc |= dn_alloc(d, &lossf, 2);
+  if (rc) return bail(rc);
+  d->loss = (double*)lossf;
+  sp::fill<float>(d->params, 0.0f, sp::DN_NPARAMS, d->st);
+  sp::fill<float>(d->m1, 0.0f, sp::DN_NPARAMS, d->st);
+  sp::fill<float>(d->m2, 0.0f, sp::DN_NPARAMS, d->st);
+  if (hipStreamSynchronize(d->st) != hipSuccess) return bail(dn_fail(d, SP_EHIP, "initialisation failed"));
+  *out = d;
+  return SP_OK;
+}
+
+int sp_dn_destroy(sp_dn* d) {
+  if (!d) return SP_OK;
+  if (d->st) hipStreamSynchronize(d->st);
+  for (void* p : d->allocs) hipFree(p);
+  if (d->st) hipStreamDestroy(d->st);
+  delete d;
+  return SP_OK;
+}
+
+const char* sp_dn_last_error(const sp_dn* d) { return d ? d->err.c_str() : g_dn_create_error.c_str(); }
+
+int sp_dn_set_params(sp_dn* d, const float* p, int n) {
+  if (!d || !p || n != sp::DN_NPARAMS) return dn_fail(d, SP_EINVAL, "expected SP_DN_NPARAMS parameters");
+  DNCHK(d, hipMemcpy(d->params, p, n * sizeof(float), hipMemcpyHostToDevice));
+  d->adam_t = 0;
+  sp::fill<float>(d->m1, 0.0f, n, d->st);
+  sp::fill<float>(d->m2, 0.0f, n, d->st);
+  DNCHK(d, hipStreamSynchronize(d->st));
+  return SP_OK;
+}
+
+int sp_dn_get_params(sp_dn* d, float* p, int n) {
+  if (!d || !p || n != sp::DN_NPARAMS) return dn_fail(d, SP_EINVAL, "expected SP_DN_NPARAMS parameters");
+  DNCHK(d, hipStreamSynchronize(d->st));
+  DNCHK(d, hipMemcpy(p, d->params, n * sizeof(float), hipMemcpyDeviceToHost));
+  return SP_OK;
+}
+
+int sp_dn_forward(sp_dn* d, const float* x, float* y) {
+  if (!d || !x || !y) return SP_EINVAL;
+  const size_t n4 = (size_t)d->B * 4 * d->H * d->W;
+  DNCHK(d, hipMemcpyAsync(d->T.x, x, n4 * sizeof(float), hipMemcpyHostToDevice, d->st));
+  sp::dn_forward(sp::dn_params(d->params), d->T, d->B, d->H, d->W, false, d->st);
+  DNCHK(d, hipGetLastError());
+  std::vector<float> r(n4);
+  DNCHK(d, hipMemcpyAsync(r.data(), d->T.r, n4 * sizeof(float), hipMemcpyDeviceToHost, d->st));
+  DNCHK(d, hipStreamSynchronize(d->st));
+  for (size_t i = 0; i < n4; ++i) y[i] = x[i] + r[i];
+  return SP_OK;
+}
+
+int sp_dn_gradient(sp_dn* d, const float* x, const float* t, float* grad, double* loss) {
+  if (!d || !x || !t || !grad || !loss) return SP_EINVAL;
+  const size_t n4 = (size_t)d->B * 4 * d->H * d->W;
+  DNCHK(d, hipMemcpyAsync(d->T.x, x, n4 * sizeof(float), hipMemcpyHostToDevice, d->st));
+  DNCHK(d, hipMemcpyAsync(d->T.t, t, n4 * sizeof(float), hipMemcpyHostToDevice, d->st));
+  const int rc = dn_step_grad(d);
+  if (rc) return rc;
+  DNCHK(d, hipMemcpyAsync(grad, d->grad, sp::DN_NTRAIN * sizeof(float), hipMemcpyDeviceToHost, d->st));
+  double l = 0;
+  DNCHK(d, hipMemcpyAsync(&l, d->loss, sizeof(double), hipMemcpyDeviceToHost, d->st));
+  DNCHK(d, hipStreamSynchronize(d->st));
+  *loss = l / (double)n4;
+  return SP_OK;
+}
+
+int sp_dn_train(sp_dn* d, const float* xs, const float* ts, int nsamples, int iters, float lr, uint64_t seed,
+                double* loss_hist) {
+  if (!d || iters < 0 || !(lr > 0)) return SP_EINVAL;
+  const size_t one = (size_t)4 * d->H * d->W;
+  if (xs && ts) {  // (re)upload the training set
+    if (nsamples < 1) return dn_fail(d, SP_EINVAL, "empty training set");
+    float *nx = nullptr, *nt = nullptr;
+    DNCHK(d, hipMalloc(&nx, nsamples * one * sizeof(float)));
+    d->allocs.push_back(nx);
+    DNCHK(d, hipMalloc(&nt, nsamples * one * sizeof(float)));
+    d->allocs.push_back(nt);
+    DNCHK(d, hipMemcpy(nx, xs, nsamples * one * sizeof(float), hipMemcpyHostToDevice));
+    DNCHK(d, hipMemcpy(nt, ts, nsamples * one * sizeof(float), hipMemcpyHostToDevice));
+    d->xs = nx;
+    d->ts = nt;
+    d->nsamples = nsamples;
+  }
+  if (!d->xs || d->nsamples < 1) return dn_fail(d, SP_EINVAL, "no training set uploaded");
+  std::mt19937_64 rng(seed);
+  std::vector<double> hist((size_t)iters);
+  for (int it = 0; it < iters; ++it) {
+    for (int b = 0; b < d->B; ++b) {
+      const int64_t k = (int64_t)(rng() % (uint64_t)d->nsamples);
+      DNCHK(d, hipMemcpyAsync(d->T.x + b * one, d->xs + k * one, one * sizeof(float), hipMemcpyDeviceToDevice,
+                              d->st));
+      DNCHK(d, hipMemcpyAsync(d->T.t + b * one, d->ts + k * one, one * sizeof(float), hipMemcpyDeviceToDevice,
+                              d->st));
+    }
+    const int rc = dn_step_grad(d);
+    if (rc) return rc;
+    sp::dn_adam(d->params, d->grad, d->m1, d->m2, sp::DN_NTRAIN, lr, ++d->adam_t, d->st);
+    if (loss_hist) DNCHK(d, hipMemcpyAsync(&hist[it], d->loss, sizeof(double), hipMemcpyDeviceToHost, d->st));
+  }
+  DNCHK(d, hipStreamSynchronize(d->st));
+  if (loss_hist)
+    for (int it = 0; it < iters; ++it) loss_hist[it] = hist[it] / (double)(d->B * one);
+  return SP_OK;
+}
+
+}  // extern "C"

@@ -128,4 +128,35 @@ template <class T> void launch_fft_cols(const FftBatch& b, const FftDim& dy, int
 // Largest transform length one block holds in LDS.
 template <class T> constexpr int fft_max_len() { return (int)(65536 / (2 * sizeof(T))); }
 
+// Learned current denoiser (denoise.hip).  Parameters, one FP32 vector:
+// w1 [16][4*9], b1 [16], w2 [16][16*9], b2 [16], w3 [4][16*9], b3 [4], then
+// the 4 channel scales (not trained).  w[co][ci*9 + ky*3 + kx].
+constexpr int DN_NTRAIN = 16 * 36 + 16 + 16 * 144 + 16 + 4 * 144 + 4;  // 3492
+constexpr int DN_NPARAMS = DN_NTRAIN + 4;
+struct DnParams {
+  float *w1, *b1, *w2, *b2, *w3, *b3, *scale;
+};
+inline DnParams dn_params(float* p) {
+  DnParams d;
+  d.w1 = p;
+  d.b1 = d.w1 + 16 * 36;
+  d.w2 = d.b1 + 16;
+  d.b2 = d.w2 + 16 * 144;
+  d.w3 = d.b2 + 16;
+  d.b3 = d.w3 + 4 * 144;
+  d.scale = d.b3 + 4;
+  return d;
+}
+// Activations of B images of H x W: x, t (targets), r (4 channels); a1, a2,
+// da1, da2 (16); dr (4); wt2, wt3 the flipped backward weights.
+struct DnTensors {
+  float *x, *t, *a1, *a2, *r, *dr, *da2, *da1, *wt2, *wt3;
+};
+void dn_forward(const DnParams& P, const DnTensors& T, int B, int H, int W, bool periodic_y, hipStream_t st);
+void dn_backward(const DnParams& P, DnParams G, const DnTensors& T, int B, int H, int W, double* loss,
+                 hipStream_t st);
+void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, int t, hipStream_t st);
+void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, int nx, int ny, int dens, double* sums,
+                  const double* gauss, float tol, float gauss_tol, int* stats, hipStream_t st);
+
 }  // namespace sp

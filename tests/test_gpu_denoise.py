@@ -1,0 +1,123 @@
+"""Learned current denoiser on the GPU (BF16 MFMA kernels), marked gpu:
+forward pass and gradients against the FP64 numpy restatement with the same
+BF16 rounding points (tests/dn_numpy.py, itself pinned by finite
+differences), training on a synthetic denoising task, and the in-loop use
+with its fallback."""
+import numpy as np
+import pytest
+
+from spectral_rs_amd import capi
+from tests import dn_numpy as D
+from tests.helpers import shock_species
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def sample(rng, B, H, W):
+    x = rng.standard_normal((B, 4, H, W)).astype(np.float32)
+    t = (0.5 * x + 0.1 * rng.standard_normal((B, 4, H, W))).astype(np.float32)
+    return x, t
+
+
+# image sizes that are not multiples of the 8 x 32 workgroup tile exercise the edges
+@pytest.mark.parametrize("shape", [(2, 13, 40), (1, 8, 32)])
+def test_forward_matches_numpy(shape):
+    B, H, W = shape
+    rng = np.random.default_rng(4)
+    p = D.init_params(5)
+    p[:D.NTRAIN] += (0.02 * rng.standard_normal(D.NTRAIN)).astype(np.float32)
+    x, _ = sample(rng, B, H, W)
+    dn = capi.Denoiser(B, H, W)
+    dn.set_params(p)
+    y = dn.forward(x)
+    yr, (_, _, r) = D.forward(p, x, emulate=True)
+    # BF16 operands, FP32 sums: the correction r agrees to FP32 accumulation and
+    # the occasional one-ulp BF16 rounding difference of an FP32 activation
+    assert rel(y - x, r) < 3e-3
+    assert rel(y, yr) < 1e-3
+    dn.close()
+
+
+def test_gradient_matches_numpy():
+    B, H, W = 2, 13, 40
+    rng = np.random.default_rng(6)
+    p = D.init_params(7)
+    p[:D.NTRAIN] += (0.02 * rng.standard_normal(D.NTRAIN)).astype(np.float32)
+    x, t = sample(rng, B, H, W)
+    dn = capi.Denoiser(B, H, W)
+    dn.set_params(p)
+    loss, g = dn.gradient(x, t)
+    lr, gr = D.loss_grad(p, x, t, emulate=True)
+    assert loss == pytest.approx(lr, rel=1e-3)
+    o = 0
+    for name, shp in D.SHAPES[:-1]:
+        n = int(np.prod(shp))
+        assert rel(g[o:o + n], gr[o:o + n]) < 1e-2, name
+        o += n
+    dn.close()
+
+
+def test_training_reduces_loss_on_a_denoising_task():
+    # targets: smooth fields; inputs: targets + white noise.  The network has
+    # to learn a low-pass filter; the loss must fall well below that of the
+    # identity (the noise variance) within a few hundred Adam steps.
+    rng = np.random.default_rng(8)
+    n, H, W = 32, 32, 64
+    yy, xx = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    ts = np.zeros((n, 4, H, W), np.float32)
+    for i in range(n):
+        for c in range(4):
+            kx, ky, ph = rng.integers(1, 4), rng.integers(1, 3), rng.random() * 6.28
+            ts[i, c] = np.sin(2 * np.pi * (kx * xx / W + ky * yy / H) + ph)
+    xs = (ts + 0.3 * rng.standard_normal(ts.shape)).astype(np.float32)
+    dn = capi.Denoiser(8, H, W)
+    dn.set_params(D.init_params(9))
+    hist = dn.train(xs, ts, iters=300, lr=3e-3, seed=1)
+    identity = 0.3 ** 2
+    assert np.all(np.isfinite(hist))
+    assert hist[:10].mean() > 0.5 * identity
+    assert hist[-20:].mean() < 0.5 * identity, (hist[:5], hist[-5:])
+    # a held-out batch
+    y = dn.forward(xs[:8])
+    assert np.mean((y - ts[:8]) ** 2) < 0.5 * identity
+    dn.close()
+
+
+def shock(denoise=None, max_rel=0.5):
+    kw = dict(size_x=128, size_y=32, delta=8, dens=4, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0, left_edge=1)
+    g = capi.GpuSim(prec=4, sort_interval=5, **kw)
+    for q, m, vth, sd in [(1.0, 16.0, 0.0025, 3), (-1.0, 1.0, 0.01, 4)]:
+        g.add_species(q, m, **shock_species(128, 32, 8, 4, 10.0, 10.0, vth, seed=sd))
+    if denoise is not None:
+        g.set_denoiser(denoise, max_rel=max_rel)
+    g.prestep()
+    g.step(10)
+    g.sync()
+    return g
+
+
+def test_in_loop_identity_network_and_fallback():
+    plain = shock()
+    # an all-zero network corrects nothing: the currents go through
+    # (J / (dens s)) (dens s) and the run follows the plain one
+    p0 = np.zeros(D.NPARAMS, np.float32)
+    p0[-4:] = 1.0
+    ident = shock(p0)
+    assert ident.denoiser_stats() == (10, 0)
+    for name in ["jx", "rho", "ex", "bz"]:
+        assert rel(ident.grid(name), plain.grid(name)) < 1e-4, name
+    # a network whose correction is huge trips the |r| <= max_rel |x| check:
+    # every solve falls back to the binomial filter, i.e. the plain run (up to
+    # the order of the deposit's global float atomics between two runs)
+    pb = p0.copy()
+    pb[D.NTRAIN - 4:D.NTRAIN] = 1e3  # b3
+    fb = shock(pb, max_rel=0.5)
+    assert fb.denoiser_stats() == (0, 10)
+    for name in ["jx", "rho", "ex", "bz"]:
+        assert rel(fb.grid(name), plain.grid(name)) < 1e-4, name
+    for s in (plain, ident, fb):
+        s.close()
