@@ -194,8 +194,13 @@ int sp_dn_destroy(sp_dn* dn);
 const char* sp_dn_last_error(const sp_dn* dn);
 int sp_dn_set_params(sp_dn* dn, const float* params, int n);
 int sp_dn_get_params(sp_dn* dn, float* params, int n);
-/* y = x + r(x) for one batch (host arrays). */
-int sp_dn_forward(sp_dn* dn, const float* x, float* y);
+/* Loss weight of each of the 4 channels (default 1): the loss is
+ * sum_c w_c mean((x + r - t)_c^2). */
+int sp_dn_set_loss_weights(sp_dn* dn, const float* w4);
+/* y = x + r(x) for one batch (host arrays).  mode 0: zero outside the
+ * images (training); 1: rows periodic; 2: rows periodic through the fused
+ * one-launch kernel the simulation uses. */
+int sp_dn_forward(sp_dn* dn, const float* x, float* y, int mode);
 /* Mean squared error of x + r(x) against t and its gradient (SP_DN_NTRAIN
  * entries) for one batch. */
 int sp_dn_gradient(sp_dn* dn, const float* x, const float* t, float* grad, double* loss);

@@ -26,7 +26,8 @@ SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_spe
            "sp_fields_update", "sp_boris_push", "sp_filter", "sp_fft2d", "sp_spectral_update",
            "sp_prestep", "sp_step", "sp_sync", "sp_set_timing", "sp_get_timers", "sp_set_graph",
            "sp_get_diag", "sp_dn_create", "sp_dn_destroy", "sp_dn_last_error", "sp_dn_set_params",
-           "sp_dn_get_params", "sp_dn_forward", "sp_dn_gradient", "sp_dn_train", "sp_set_denoiser",
+           "sp_dn_get_params", "sp_dn_set_loss_weights", "sp_dn_forward", "sp_dn_gradient", "sp_dn_train",
+           "sp_set_denoiser",
            "sp_get_denoiser_stats"]
 DN_NTRAIN, DN_NPARAMS = 3492, 3496
 
@@ -88,7 +89,8 @@ def load(path=LIB_PATH):
     L.sp_dn_last_error.restype = ct.c_char_p
     L.sp_dn_set_params.argtypes = [vp, vp, i32]
     L.sp_dn_get_params.argtypes = [vp, vp, i32]
-    L.sp_dn_forward.argtypes = [vp, vp, vp]
+    L.sp_dn_set_loss_weights.argtypes = [vp, vp]
+    L.sp_dn_forward.argtypes = [vp, vp, vp, i32]
     L.sp_dn_gradient.argtypes = [vp, vp, vp, vp, ct.POINTER(ct.c_double)]
     L.sp_dn_train.argtypes = [vp, vp, vp, i32, i32, ct.c_float, ct.c_uint64, vp]
     L.sp_set_denoiser.argtypes = [vp, vp, i32, ct.c_float, ct.c_float]
@@ -269,15 +271,20 @@ class Denoiser:
         p = np.ascontiguousarray(p, np.float32)
         self._chk(load().sp_dn_set_params(self.d, _p(p), p.size))
 
+    def set_loss_weights(self, w4):
+        w = np.ascontiguousarray(w4, np.float32)
+        self._chk(load().sp_dn_set_loss_weights(self.d, _p(w)))
+
     def params(self):
         p = np.zeros(DN_NPARAMS, np.float32)
         self._chk(load().sp_dn_get_params(self.d, _p(p), p.size))
         return p
 
-    def forward(self, x):
+    def forward(self, x, mode=0):
+        """x + r(x); mode 0 zero boundary, 1 periodic rows, 2 periodic rows (fused kernel)."""
         x = np.ascontiguousarray(x, np.float32)
         y = np.empty_like(x)
-        self._chk(load().sp_dn_forward(self.d, _p(x), _p(y)))
+        self._chk(load().sp_dn_forward(self.d, _p(x), _p(y), int(mode)))
         return y
 
     def gradient(self, x, t):

@@ -184,14 +184,16 @@ __global__ void k_flip_weights(const float* __restrict__ w, float* __restrict__ 
   wt[ci * cout * 9 + co * 9 + (8 - t)] = w[i];
 }
 
-// pred = x + r; loss += sum (pred - t)^2 (FP64); dr = 2 (pred - t) / n.
+// pred = x + r; loss += sum lw_c (pred - t)^2 (FP64); dr = 2 lw_c (pred - t) / n,
+// with lw_c the weight of channel c = (i / hw) % 4.
 __global__ void k_mse(const float* __restrict__ x, const float* __restrict__ r, const float* __restrict__ t,
-                      float* __restrict__ dr, double* loss, int64_t n) {
+                      const float* __restrict__ lw, float* __restrict__ dr, double* loss, int64_t n, int64_t hw) {
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float w = lw[(i / hw) & 3];
     const float d = x[i] + r[i] - t[i];
-    dr[i] = 2.0f * d / (float)n;
-    acc += (double)d * d;
+    dr[i] = 2.0f * w * d / (float)n;
+    acc += (double)w * d * d;
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
@@ -218,8 +220,18 @@ struct Grids4 {
   float* g[4];
 };
 
+// Sum of v over the workgroup, added to *out by one atomic (256 threads).
+__device__ __forceinline__ void block_add(double v, double* out) {
+  __shared__ double red[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, ((red[0] + red[1]) + red[2]) + red[3]);
+}
+
 // x[c][y][x] = grid_c(interior) / (dens s_c); sums |x|^2 into sums[0].
-__global__ void k_dn_pack(Grids4 gr, const float* __restrict__ scale, float inv_dens, float* __restrict__ x, int nx,
+__global__ void __launch_bounds__(256) k_dn_pack(Grids4 gr, const float* __restrict__ scale, float inv_dens, float* __restrict__ x, int nx,
                           int ny, double* sums) {
   const int64_t n = (int64_t)nx * ny;
   double acc = 0.0;
@@ -231,12 +243,10 @@ __global__ void k_dn_pack(Grids4 gr, const float* __restrict__ scale, float inv_
     x[i] = v;
     acc += (double)v * v;
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-  if ((threadIdx.x & 63) == 0) atomicAdd(sums, acc);
+  block_add(acc, sums);
 }
 
-__global__ void k_dn_sumsq(const float* __restrict__ r, int64_t n, double* sums) {
+__global__ void __launch_bounds__(256) k_dn_sumsq(const float* __restrict__ r, int64_t n, double* sums) {
   double acc = 0.0;
   bool finite = true;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -245,9 +255,7 @@ __global__ void k_dn_sumsq(const float* __restrict__ r, int64_t n, double* sums)
     acc += (double)v * v;
   }
   if (!finite) acc = __builtin_inf();
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-  if ((threadIdx.x & 63) == 0) atomicAdd(sums + 1, acc);
+  block_add(acc, sums + 1);
 }
 
 // Writes dens s_c (x + r) into the interior of the grids when the correction
@@ -272,6 +280,120 @@ __global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float den
   }
 }
 
+// ---- fused inference: the three layers of one output tile in LDS ----
+// A workgroup owns FY x FX output pixels.  Layer 1 is evaluated on the tile
+// plus a 2-pixel ring, layer 2 on the tile plus 1, layer 3 on the tile; the
+// activations stay in LDS as BF16 (the precision the next layer's MFMA reads
+// anyway).  Activations of pixels outside the image are zero, as in the
+// layer-by-layer path (columns outside; rows wrap periodically), so both
+// paths compute the same numbers.
+constexpr int FY = 8, FX = 32;
+
+// One conv of the fused chain: output region oh x ow (top-left at offset 0),
+// input region (oh+2) x (ow+2) in s_src (planes of CIN), weights s_w [16][KP].
+// Output pixel (qy, qx) of the region is image pixel (y0 + qy - ring,
+// x0 + qx - ring).
+// s_off[kk]: offset of tap kk in the input region (0 for padded taps).
+template <int CIN> __device__ __forceinline__ void fused_offsets(int* s_off, int ih, int iw) {
+  constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16;
+  for (int kk = threadIdx.x; kk < KP; kk += DN_NT) {
+    int o = 0;
+    if (kk < K) {
+      const int ci = kk / 9, tp = kk - ci * 9;
+      o = ci * ih * iw + (tp / 3) * iw + (tp % 3);
+    }
+    s_off[kk] = o;
+  }
+}
+
+template <int CIN, int COUT, bool RELU, bool LAST>
+__device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w, const int* s_off,
+                                            const float* __restrict__ bias, short* s_dst, float* __restrict__ out,
+                                            int oh, int ow, int ring, int y0, int x0, int H, int W) {
+  constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16;
+  const int iw = ow + 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = (lane >> 4) * 4;
+  const int npx = oh * ow, ntile = (npx + 15) / 16;
+  for (int t = wave; t < ntile; t += DN_NT / 64) {
+    const int q = min(t * 16 + m, npx - 1);  // this lane's A-row pixel (clamped; its row is discarded)
+    const int qy = q / ow, qx = q - qy * ow;
+    const int pbase = qy * iw + qx;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < KP; kc += 16) {
+      bf16x4 a, bv;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int kk = kc + kq + i;
+        a[i] = s_src[pbase + s_off[kk]];
+        bv[i] = s_w[m * KP + kk];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+    }
+    const int n = lane & 15;
+    if (n < COUT) {
+      const float bn = bias[n];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int p = t * 16 + kq + i;
+        if (p >= npx) continue;
+        const int py = p / ow, px = p - py * ow;
+        const int x = x0 + px - ring;
+        float v = acc[i] + bn;
+        if (RELU) v = fmaxf(v, 0.0f);
+        if (LAST) {
+          const int y = y0 + py - ring;
+          if (y < H && x >= 0 && x < W) out[((size_t)n * H + y) * W + x] = v;
+        } else {
+          if (x < 0 || x >= W) v = 0.0f;  // outside the image (columns); rows wrap
+          s_dst[n * npx + p] = bf16_bits(v);
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(DN_NT) k_dn_fused(const float* __restrict__ x, const float* __restrict__ params,
+                                                    float* __restrict__ r, int H, int W) {
+  constexpr int H0 = FY + 6, W0 = FX + 6;  // input region: tile + 3 rings
+  constexpr int H1 = FY + 4, W1 = FX + 4, H2 = FY + 2, W2 = FX + 2;
+  __shared__ short s_x[4 * H0 * W0];
+  __shared__ short s_a1[16 * H1 * W1];
+  __shared__ short s_a2[16 * H2 * W2];
+  __shared__ short s_w1[16 * 48], s_w2[16 * 144], s_w3[16 * 144];
+  __shared__ int s_off1[48], s_off2[144], s_off3[144];
+  const DnParams P = dn_params(const_cast<float*>(params));
+  const int y0 = blockIdx.y * FY, x0 = blockIdx.x * FX;
+  for (int i = threadIdx.x; i < 4 * H0 * W0; i += DN_NT) {
+    const int c = i / (H0 * W0), rc = i - c * (H0 * W0);
+    const int ry = rc / W0, rx = rc - ry * W0;
+    int y = y0 + ry - 3;
+    const int xx = x0 + rx - 3;
+    y = ((y % H) + H) % H;
+    s_x[i] = bf16_bits(xx >= 0 && xx < W ? x[((size_t)c * H + y) * W + xx] : 0.0f);
+  }
+  for (int i = threadIdx.x; i < 16 * 144; i += DN_NT) {
+    const int n = i / 144, kk = i - n * 144;
+    if (kk < 48) s_w1[n * 48 + kk] = kk < 36 ? bf16_bits(P.w1[n * 36 + kk]) : (short)0;
+    s_w2[i] = bf16_bits(P.w2[n * 144 + kk]);
+    s_w3[i] = n < 4 ? bf16_bits(P.w3[n * 144 + kk]) : (short)0;
+  }
+  fused_offsets<4>(s_off1, H0, W0);
+  fused_offsets<16>(s_off2, H1, W1);
+  fused_offsets<16>(s_off3, H2, W2);
+  __syncthreads();
+  fused_layer<4, 16, true, false>(s_x, s_w1, s_off1, P.b1, s_a1, nullptr, H1, W1, 2, y0, x0, H, W);
+  __syncthreads();
+  fused_layer<16, 16, true, false>(s_a1, s_w2, s_off2, P.b2, s_a2, nullptr, H2, W2, 1, y0, x0, H, W);
+  __syncthreads();
+  fused_layer<16, 4, false, true>(s_a2, s_w3, s_off3, P.b3, nullptr, r, FY, FX, 0, y0, x0, H, W);
+}
+
+void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipStream_t st) {
+  dim3 grid((W + FX - 1) / FX, (H + FY - 1) / FY);
+  hipLaunchKernelGGL(k_dn_fused, grid, dim3(DN_NT), 0, st, T.x, P.w1, T.r, H, W);
+}
+
 // ---- launchers ----
 template <int CIN, int COUT, int POST, bool PY>
 static void conv(const float* in, const float* w, const float* bias, const float* mask, float* out, int B, int H,
@@ -284,9 +406,9 @@ static void wgrad(const float* in, const float* dy, float* dw, float* db, int B,
   dim3 grid((W + DN_TX - 1) / DN_TX, (H + DN_TY - 1) / DN_TY, B);
   hipLaunchKernelGGL((k_conv3x3_wgrad<CIN, COUT>), grid, dim3(DN_NT), 0, st, in, dy, dw, db, H, W);
 }
-static unsigned nblk(int64_t n) {
+static unsigned nblk(int64_t n, int64_t cap = 4096) {
   int64_t b = (n + 255) / 256;
-  return (unsigned)(b > 4096 ? 4096 : (b < 1 ? 1 : b));
+  return (unsigned)(b > cap ? cap : (b < 1 ? 1 : b));
 }
 
 void dn_forward(const DnParams& P, const DnTensors& T, int B, int H, int W, bool periodic_y, hipStream_t st) {
@@ -303,10 +425,10 @@ void dn_forward(const DnParams& P, const DnTensors& T, int B, int H, int W, bool
 
 // Loss and gradients of one batch (zero boundary): T.x, T.t set; grad has
 // DN_NTRAIN entries in the parameter layout; loss[0] gets sum (pred - t)^2.
-void dn_backward(const DnParams& P, DnParams G, const DnTensors& T, int B, int H, int W, double* loss,
-                 hipStream_t st) {
+void dn_backward(const DnParams& P, DnParams G, const DnTensors& T, const float* lw, int B, int H, int W,
+                 double* loss, hipStream_t st) {
   const int64_t n4 = (int64_t)B * 4 * H * W;
-  hipLaunchKernelGGL(k_mse, dim3(nblk(n4)), dim3(256), 0, st, T.x, T.r, T.t, T.dr, loss, n4);
+  hipLaunchKernelGGL(k_mse, dim3(nblk(n4)), dim3(256), 0, st, T.x, T.r, T.t, lw, T.dr, loss, n4, (int64_t)H * W);
   // layer 3: r = conv(a2, w3)
   wgrad<16, 4>(T.a2, T.dr, G.w3, G.b3, B, H, W, st);
   hipLaunchKernelGGL(k_flip_weights, dim3((16 * 4 * 9 + 255) / 256), dim3(256), 0, st, P.w3, T.wt3, 16, 4);
@@ -329,10 +451,11 @@ void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, in
   for (int c = 0; c < 4; ++c) gr.g[c] = grids[c];
   const int64_t n4 = (int64_t)4 * nx * ny;
   fill<double>(sums, 0.0, 2, st);
-  hipLaunchKernelGGL(k_dn_pack, dim3(nblk(n4)), dim3(256), 0, st, gr, P.scale, 1.0f / (float)dens, T.x, nx, ny,
-                     sums);
-  dn_forward(P, T, 1, ny, nx, true, st);
-  hipLaunchKernelGGL(k_dn_sumsq, dim3(nblk(n4)), dim3(256), 0, st, T.r, n4, sums);
+  // reductions: 1024 workgroups, one FP64 atomic each
+  hipLaunchKernelGGL(k_dn_pack, dim3(nblk(n4, 1024)), dim3(256), 0, st, gr, P.scale, 1.0f / (float)dens, T.x, nx,
+                     ny, sums);
+  dn_forward_fused(P, T, ny, nx, st);
+  hipLaunchKernelGGL(k_dn_sumsq, dim3(nblk(n4, 1024)), dim3(256), 0, st, T.r, n4, sums);
   hipLaunchKernelGGL(k_dn_apply, dim3(nblk(n4)), dim3(256), 0, st, gr, P.scale, (float)dens, T.x, T.r, nx, ny, sums,
                      gauss, tol, gauss_tol, stats);
 }
@@ -352,6 +475,7 @@ struct sp_dn {
   sp::DnTensors T{};
   double* loss = nullptr;
   float *xs = nullptr, *ts = nullptr;  // uploaded training samples
+  float* lw = nullptr;                  // loss weight per channel (4)
   int64_t nsamples = 0;
   int adam_t = 0;
   std::vector<void*> allocs;
@@ -384,7 +508,7 @@ int dn_step_grad(sp_dn* d) {
   sp::fill<double>(d->loss, 0.0, 1, d->st);
   const sp::DnParams P = sp::dn_params(d->params);
   sp::dn_forward(P, d->T, d->B, d->H, d->W, false, d->st);
-  sp::dn_backward(P, sp::dn_params(d->grad), d->T, d->B, d->H, d->W, d->loss, d->st);
+  sp::dn_backward(P, sp::dn_params(d->grad), d->T, d->lw, d->B, d->H, d->W, d->loss, d->st);
   DNCHK(d, hipGetLastError());
   return SP_OK;
 }
@@ -425,11 +549,13 @@ int sp_dn_create(int batch, int h, int w, sp_dn** out) {
   rc |= dn_alloc(d, &d->T.wt3, 16 * 36);
   float* lossf = nullptr;
   rc |= dn_alloc(d, &lossf, 2);
+  rc |= dn_alloc(d, &d->lw, 4);
   if (rc) return bail(rc);
   d->loss = (double*)lossf;
   sp::fill<float>(d->params, 0.0f, sp::DN_NPARAMS, d->st);
   sp::fill<float>(d->m1, 0.0f, sp::DN_NPARAMS, d->st);
   sp::fill<float>(d->m2, 0.0f, sp::DN_NPARAMS, d->st);
+  sp::fill<float>(d->lw, 1.0f, 4, d->st);
   if (hipStreamSynchronize(d->st) != hipSuccess) return bail(dn_fail(d, SP_EHIP, "initialisation failed"));
   *out = d;
   return SP_OK;
@@ -456,6 +582,12 @@ int sp_dn_set_params(sp_dn* d, const float* p, int n) {
   return SP_OK;
 }
 
+int sp_dn_set_loss_weights(sp_dn* d, const float* w4) {
+  if (!d || !w4) return SP_EINVAL;
+  DNCHK(d, hipMemcpy(d->lw, w4, 4 * sizeof(float), hipMemcpyHostToDevice));
+  return SP_OK;
+}
+
 int sp_dn_get_params(sp_dn* d, float* p, int n) {
   if (!d || !p || n != sp::DN_NPARAMS) return dn_fail(d, SP_EINVAL, "expected SP_DN_NPARAMS parameters");
   DNCHK(d, hipStreamSynchronize(d->st));
@@ -463,11 +595,21 @@ int sp_dn_get_params(sp_dn* d, float* p, int n) {
   return SP_OK;
 }
 
-int sp_dn_forward(sp_dn* d, const float* x, float* y) {
-  if (!d || !x || !y) return SP_EINVAL;
+int sp_dn_forward(sp_dn* d, const float* x, float* y, int mode) {
+  if (!d || !x || !y || mode < 0 || mode > 2) return SP_EINVAL;
   const size_t n4 = (size_t)d->B * 4 * d->H * d->W;
   DNCHK(d, hipMemcpyAsync(d->T.x, x, n4 * sizeof(float), hipMemcpyHostToDevice, d->st));
-  sp::dn_forward(sp::dn_params(d->params), d->T, d->B, d->H, d->W, false, d->st);
+  const sp::DnParams P = sp::dn_params(d->params);
+  if (mode < 2) {
+    sp::dn_forward(P, d->T, d->B, d->H, d->W, mode == 1, d->st);
+  } else {
+    for (int b = 0; b < d->B; ++b) {  // the simulation's path, one image at a time
+      sp::DnTensors t = d->T;
+      t.x += (size_t)b * 4 * d->H * d->W;
+      t.r += (size_t)b * 4 * d->H * d->W;
+      sp::dn_forward_fused(P, t, d->H, d->W, d->st);
+    }
+  }
   DNCHK(d, hipGetLastError());
   std::vector<float> r(n4);
   DNCHK(d, hipMemcpyAsync(r.data(), d->T.r, n4 * sizeof(float), hipMemcpyDeviceToHost, d->st));

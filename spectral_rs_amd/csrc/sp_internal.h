@@ -136,7 +136,7 @@ constexpr int DN_NPARAMS = DN_NTRAIN + 4;
 struct DnParams {
   float *w1, *b1, *w2, *b2, *w3, *b3, *scale;
 };
-inline DnParams dn_params(float* p) {
+__host__ __device__ inline DnParams dn_params(float* p) {
   DnParams d;
   d.w1 = p;
   d.b1 = d.w1 + 16 * 36;
@@ -153,9 +153,12 @@ struct DnTensors {
   float *x, *t, *a1, *a2, *r, *dr, *da2, *da1, *wt2, *wt3;
 };
 void dn_forward(const DnParams& P, const DnTensors& T, int B, int H, int W, bool periodic_y, hipStream_t st);
-void dn_backward(const DnParams& P, DnParams G, const DnTensors& T, int B, int H, int W, double* loss,
-                 hipStream_t st);
+void dn_backward(const DnParams& P, DnParams G, const DnTensors& T, const float* lw, int B, int H, int W,
+                 double* loss, hipStream_t st);
 void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, int t, hipStream_t st);
+// The three layers of one image [4][H][W] in one launch, y periodic (the
+// simulation's inference path); same results as dn_forward(periodic_y).
+void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipStream_t st);
 void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, int nx, int ny, int dens, double* sums,
                   const double* gauss, float tol, float gauss_tol, int* stats, hipStream_t st);
 

@@ -38,11 +38,15 @@ def bf16(a):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def conv(x, w, b=None):
-    """out[b,o,y,x] = b[o] + sum_{i,ky,kx} w[o,i,ky,kx] x[b,i,y+ky-1,x+kx-1], zero outside."""
+def conv(x, w, b=None, periodic_y=False):
+    """out[b,o,y,x] = b[o] + sum_{i,ky,kx} w[o,i,ky,kx] x[b,i,y+ky-1,x+kx-1], zero outside
+    (rows wrap when periodic_y)."""
     B, C, H, W = x.shape
     xp = np.zeros((B, C, H + 2, W + 2))
     xp[:, :, 1:-1, 1:-1] = x
+    if periodic_y:
+        xp[:, :, 0, 1:-1] = x[:, :, -1]
+        xp[:, :, -1, 1:-1] = x[:, :, 0]
     out = np.zeros((B, w.shape[0], H, W))
     for ky in range(3):
         for kx in range(3):
@@ -70,12 +74,12 @@ def wgrad(x, d):
     return g
 
 
-def forward(p, x, emulate=True):
+def forward(p, x, emulate=True, periodic_y=False):
     R = bf16 if emulate else (lambda a: np.asarray(a, np.float64))
     q = unpack(p)
-    a1 = np.maximum(conv(R(x), R(q["w1"]), q["b1"]), 0)
-    a2 = np.maximum(conv(R(a1), R(q["w2"]), q["b2"]), 0)
-    r = conv(R(a2), R(q["w3"]), q["b3"])
+    a1 = np.maximum(conv(R(x), R(q["w1"]), q["b1"], periodic_y), 0)
+    a2 = np.maximum(conv(R(a1), R(q["w2"]), q["b2"], periodic_y), 0)
+    r = conv(R(a2), R(q["w3"]), q["b3"], periodic_y)
     return np.asarray(x, np.float64) + r, (a1, a2, r)
 
 

@@ -42,6 +42,25 @@ def test_forward_matches_numpy(shape):
     dn.close()
 
 
+def test_periodic_and_fused_forward():
+    # the simulation's path: rows periodic, all three layers in one launch; it
+    # keeps the same BF16 rounding points and sum order as the layer-by-layer
+    # kernels, so the two agree to the last bit
+    B, H, W = 2, 21, 70
+    rng = np.random.default_rng(11)
+    p = D.init_params(12)
+    p[:D.NTRAIN] += (0.02 * rng.standard_normal(D.NTRAIN)).astype(np.float32)
+    x, _ = sample(rng, B, H, W)
+    dn = capi.Denoiser(B, H, W)
+    dn.set_params(p)
+    y1 = dn.forward(x, mode=1)
+    y2 = dn.forward(x, mode=2)
+    _, (_, _, r) = D.forward(p, x, emulate=True, periodic_y=True)
+    assert rel(y1 - x, r) < 3e-3
+    assert np.array_equal(y1, y2)
+    dn.close()
+
+
 def test_gradient_matches_numpy():
     B, H, W = 2, 13, 40
     rng = np.random.default_rng(6)
@@ -101,23 +120,46 @@ def shock(denoise=None, max_rel=0.5):
 
 
 def test_in_loop_identity_network_and_fallback():
-    plain = shock()
+    # Two plain runs differ by the order of the deposit's global float atomics;
+    # that difference is the noise floor the other runs are held to.
+    plain, plain2 = shock(), shock()
+    floor = {n: rel(plain2.grid(n), plain.grid(n)) for n in ["jx", "rho", "ex", "bz"]}
     # an all-zero network corrects nothing: the currents go through
     # (J / (dens s)) (dens s) and the run follows the plain one
     p0 = np.zeros(D.NPARAMS, np.float32)
     p0[-4:] = 1.0
     ident = shock(p0)
     assert ident.denoiser_stats() == (10, 0)
-    for name in ["jx", "rho", "ex", "bz"]:
-        assert rel(ident.grid(name), plain.grid(name)) < 1e-4, name
+    for name, f in floor.items():
+        assert rel(ident.grid(name), plain.grid(name)) <= 3 * f + 1e-6, (name, f)
     # a network whose correction is huge trips the |r| <= max_rel |x| check:
-    # every solve falls back to the binomial filter, i.e. the plain run (up to
-    # the order of the deposit's global float atomics between two runs)
+    # every solve falls back to the binomial filter, i.e. the plain run
     pb = p0.copy()
     pb[D.NTRAIN - 4:D.NTRAIN] = 1e3  # b3
     fb = shock(pb, max_rel=0.5)
     assert fb.denoiser_stats() == (0, 10)
-    for name in ["jx", "rho", "ex", "bz"]:
-        assert rel(fb.grid(name), plain.grid(name)) < 1e-4, name
-    for s in (plain, ident, fb):
+    for name, f in floor.items():
+        assert rel(fb.grid(name), plain.grid(name)) <= 3 * f + 1e-6, (name, f)
+    for s in (plain, plain2, ident, fb):
         s.close()
+
+
+def test_trained_network_gains_over_the_binomial_filter():
+    # Fresh 4-ppc / 64-ppc pairs from a run with another seed than the
+    # training runs (seeds 0 and 1), sampled over the run as in training
+    # (steps 100 to 1000): the committed network brings the filtered 4-ppc
+    # currents closer to the 64-ppc ones than the binomial filter alone on
+    # every channel (Jz, almost pure noise, by far).  Before the shock forms
+    # (steps below ~600) the currents are weak and the network's Jy can be
+    # worse than the filter's; the relative errors here are summed over the
+    # whole run, as the training loss is.
+    import os
+    from spectral_rs_amd import learned
+    p = np.load(os.path.join(os.path.dirname(capi.__file__), "denoiser_shock.npy"))
+    X, T = learned.pairs(512, 128, n=10, t0=100, every=100, seed=5)
+    ev = learned.evaluate(p, X, T)
+    b, l = ev["binomial_rel_mse"], ev["learned_rel_mse"]
+    assert l[0] < 0.95 * b[0], ev       # jx
+    assert l[1] < 1.0 * b[1], ev        # jy
+    assert l[2] < 0.2 * b[2], ev        # jz
+    assert l[3] < 0.95 * b[3], ev       # rho
