@@ -66,20 +66,138 @@ def cpu_baseline(seconds_hint=15.0):
                       f"{el:.1f} s; serial mover/deposit, threaded push/filter/FFT as the reference"}
 
 
+def ensemble_runs(count=64, seed=0):
+    """The batched-ensemble workload: `count` independent shock runs drawn with
+    `seed` -- grid from {128,256,512,1024} x {32,64,128,256}, ppc from
+    {4,8,16,32}, gamma0 uniform in [2,20], temperature T log-uniform in
+    [1e-5,1e-3] (electron vth = sqrt(T), ion vth = sqrt(T/16))."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    runs = []
+    for _ in range(count):
+        nx = int(rng.choice([128, 256, 512, 1024]))
+        ny = int(rng.choice([32, 64, 128, 256]))
+        ppc = int(rng.choice([4, 8, 16, 32]))
+        g0 = float(rng.uniform(2.0, 20.0))
+        temp = float(np.exp(rng.uniform(np.log(1e-5), np.log(1e-3))))
+        runs.append(dict(size_x=nx, size_y=ny, delta=8 if nx <= 256 else 16, dens=ppc, n_pass=4, dt=0.045,
+                         c=10.0, gamma_inj=g0, temp=temp))
+    return runs
+
+
+def bench_ensemble(args):
+    """64 independent runs of uneven size stepped together: one hipGraph per
+    2 steps of every member, 4 concurrent branches, tunings from the
+    autotuner.  Also times the same ensemble eagerly and with default tunings."""
+    from spectral_rs_amd import capi
+
+    steps = args.steps if args.steps_given else 1000
+    runs = ensemble_runs()
+    sims = []
+    n = 0
+    cells = 0
+    for r in runs:
+        kw = {k: v for k, v in r.items() if k != "temp"}
+        s = capi.GpuSim(prec=4, sort_interval=args.sort_interval, left_edge=capi.SP_LEFT_REFLECT, **kw)
+        for q, m in [(1.0, 16.0), (-1.0, 1.0)]:
+            s.init_species(q, m, (r["temp"] / m) ** 0.5, seed=0)
+        n += 2 * r["dens"] * (r["size_x"] - 2 * r["delta"]) * r["size_y"]
+        cells += (r["size_x"] + 2) * (r["size_y"] + 2)
+        sims.append(s)
+    print(f"bench: ensemble of {len(sims)} runs, {n} particles, {cells} cells", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    tunings = [s.autotune(trials=3, use_cache=True) for s in sims]
+    t_tune = time.perf_counter() - t0
+    shapes = {}
+    for r, t in zip(runs, tunings):
+        shapes[f"{r['size_x']}x{r['size_y']}/{r['dens']}ppc"] = t
+    print(f"bench: autotuned {len(shapes)} distinct configurations in {t_tune:.1f} s", file=sys.stderr, flush=True)
+    for s in sims:
+        s.prestep()
+    ens = capi.Ensemble(sims, branches=4, unit=2)
+    ens.step(max(args.warmup, 2))
+    ens.sync()
+
+    def timed(k):
+        ens.sync()
+        t0 = time.perf_counter()
+        ens.step(k)
+        ens.sync()
+        return time.perf_counter() - t0
+
+    cmp_steps = 100
+    el = timed(steps)
+    ens.set_graph(False)
+    el_eager = timed(cmp_steps)
+    ens.set_graph(True)
+    ens.close()
+    for s in sims:
+        s.set_tuning(chunk=2048, tile_w_log=3, tile_h_log=3, fft_rows=1, fft_cols=8, fft_radix_x=8, fft_radix_y=8)
+    ens = capi.Ensemble(sims, branches=4, unit=2)
+    ens.step(2)
+    el_default = timed(cmp_steps)
+    st = ens.stats()
+    ens.close()
+    for s in sims:
+        s.close()
+    value = n * steps / el
+    ms = el * 1e3 / steps
+    # particle-pass algorithmic bytes of the whole ensemble per step over the
+    # whole step's time: a lower bound on the rate of the passes themselves
+    alg = 48.0 * n + 2 * 44.0 * cells
+    out = {
+        "metric": "particle-steps/s on 1 MI355X (batched ensemble)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(args.warmup, 2),
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (GPU-generated shock initial states, seed 0)",
+        "config": {"workload": "ensemble of 64 independent FP32 shock runs, grids {128..1024}x{32..256}, "
+                               "ppc {4..32}, gamma0 in [2,20], T log-uniform in [1e-5,1e-3], 4 binomial passes; "
+                               "one hipGraph per 2 steps of all runs, 4 branches, autotuned launch shapes",
+                   "runs": len(runs), "particles": n, "cells": cells},
+        "modes_ms_per_step": {"graph_tuned": ms, "eager_tuned": el_eager * 1e3 / cmp_steps,
+                              "graph_default_tuning": el_default * 1e3 / cmp_steps},
+        "autotune_s": t_tune,
+        "tunings": shapes,
+        "graphs": st,
+        "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "particle passes of all runs, algorithmic bytes per whole step"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--size", default=None, help="override grid, e.g. 1024x256 (not the metric workload)")
     ap.add_argument("--sort-interval", type=int, default=20, help="steps between tile sorts of the particles")
     ap.add_argument("--trace", type=int, default=0, help="print phase times every N steps (diagnostic)")
+    ap.add_argument("--autotune", action="store_true", help="shock workload: autotune the launch shapes first")
+    ap.add_argument("--workload", choices=["shock", "ensemble"], default="shock",
+                    help="shock: the 4096x1024 run (the metric); ensemble: 64 uneven runs stepped together")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 2000
+    if args.workload == "ensemble":
+        bench_ensemble(args)
+        return
 
     from spectral_rs_amd import capi
 
@@ -97,6 +215,11 @@ def main():
     sim = capi.GpuSim(prec=4, sort_interval=args.sort_interval, left_edge=capi.SP_LEFT_REFLECT, **cfg)
     for k, (q, m, vth) in enumerate(SPECIES):
         sim.init_species(q, m, vth, seed=0)
+    tuning = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tuning = sim.autotune(trials=3, use_cache=False)
+        print(f"bench: autotuned in {time.perf_counter() - t0:.1f} s: {tuning}", file=sys.stderr, flush=True)
     sim.prestep()
     sim.step(args.warmup)
     sim.sync()
@@ -159,7 +282,8 @@ def main():
         "data": "synthetic (GPU-generated shock initial state, seed 0)",
         "config": {"workload": f"plain FP32 unmagnetized shock {cfg['size_x']}x{cfg['size_y']}, 32 ppc/species, "
                                f"delta 16, gamma0 10, mass ratio 16, 4 binomial passes",
-                   "particles": n, "grid": [cfg["size_x"], cfg["size_y"]]},
+                   "particles": n, "grid": [cfg["size_x"], cfg["size_y"]],
+                   "tuning": tuning or sim.tuning()},
         "phases_ms_per_step": {k: v / args.steps for k, v in tm.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -92,6 +92,7 @@ typedef struct sp_diag {
 } sp_diag;
 
 typedef struct sp_ctx sp_ctx;
+typedef struct sp_ensemble sp_ensemble;
 
 /* Flds::new + Sim::new (src/flds/mod.rs:40-65, src/lib.rs:247-264): validates
  * the parameters like Config::new (src/lib.rs:68-117) and allocates every grid,
@@ -177,15 +178,67 @@ int sp_set_graph(sp_ctx* ctx, int on);
  * residuals. */
 int sp_get_diag(sp_ctx* ctx, sp_diag* out);
 
+/* ---- Launch shapes and the autotuner (no reference counterpart) ----
+ * The knobs of one context's kernels.  They change speed only: cells, offsets
+ * and momenta are the same for every tuning, and J/rho/dens only by the
+ * summation order of the deposit. */
+typedef struct sp_tuning {
+  int32_t chunk;        /* particles per workgroup of the particle pass:
+                           512, 1024 or 2048 (SP_F64: 1024) */
+  int32_t tile_w_log;   /* sort tiles of 2^tile_w_log x 2^tile_h_log cells, */
+  int32_t tile_h_log;   /*   each log in 1..5 (the deposit windows follow them) */
+  int32_t fft_rows;     /* row transforms per workgroup, 1..64 (capped by LDS) */
+  int32_t fft_cols;     /* column transforms per workgroup, 1..64 (capped by LDS) */
+  int32_t fft_radix_x;  /* largest radix of the x / y transform plans: 8 or 4 */
+  int32_t fft_radix_y;
+  int32_t reserved;
+} sp_tuning;
+/* Defaults: chunk 2048 (SP_F64: 1024), 8x8 tiles, 1 row and 8 columns per
+ * workgroup, radix 8. */
+int sp_set_tuning(sp_ctx* ctx, const sp_tuning* t);
+int sp_get_tuning(const sp_ctx* ctx, sp_tuning* t);
+/* Times the candidates of every knob on a scratch copy of the context (same
+ * parameters, particles and fields; the context itself is not advanced),
+ * `trials` timed repetitions each, and sets the fastest: chunk x tile shape
+ * by the particle pass, radix x rows per workgroup by the row transforms,
+ * radix x columns per workgroup by the column transforms.  The result is
+ * kept per (grid size, dens, species count, precision, sort interval) for
+ * the process; use_cache != 0 takes a kept result without measuring.
+ * `out` may be NULL.  Needs at least one species. */
+int sp_autotune(sp_ctx* ctx, int trials, int use_cache, sp_tuning* out);
+
+/* ---- Ensembles of independent runs (no reference counterpart) ----
+ * n contexts of any sizes stepped together on one GPU.  Each context joins
+ * one of `branches` (1..16) concurrent branches, balanced by particle count;
+ * every step of the ensemble forks into the branches and joins again, and
+ * with the graph on (the default) `unit` steps of every member are captured
+ * once as one hipGraph and replayed (captured again when a member's buffers,
+ * push state, sort phase or tuning differ from every kept capture).  Members
+ * keep their own results, getters and errors; they are launched on the
+ * ensemble's stream while they belong to it, must outlive it (sp_destroy of
+ * a member fails until then), and must have timing and their own graph off. */
+int sp_ensemble_create(sp_ctx* const* members, int n, int branches, int unit, sp_ensemble** out);
+int sp_ensemble_destroy(sp_ensemble* e);
+const char* sp_ensemble_last_error(const sp_ensemble* e);
+int sp_ensemble_set_graph(sp_ensemble* e, int on);
+/* nsteps iterations of run()'s loop body for every member (as sp_step). */
+int sp_ensemble_step(sp_ensemble* e, int nsteps);
+/* Waits for the ensemble; reports the first member's deferred invariant
+ * violation (its index in *bad_member, -1 when none; may be NULL). */
+int sp_ensemble_sync(sp_ensemble* e, int* bad_member);
+/* Captured graphs kept and captures made so far. */
+int sp_ensemble_stats(const sp_ensemble* e, int* kept, int* captures);
+
 /* ---- Learned current denoiser (no reference counterpart) ----
- * A 3-layer 3x3 convolutional network (4 -> 16 -> 16 -> 4 channels, ReLU) on
- * BF16 MFMA with FP32 accumulation.  Input x: Jx Jy Jz rho after the binomial
- * filter, each divided by dens and a channel scale; output x + r.  Parameters,
- * one FP32 vector of SP_DN_NPARAMS: w1 [16][4*9], b1 [16], w2 [16][16*9],
- * b2 [16], w3 [4][16*9], b3 [4], scales [4] (w[co][ci*9 + ky*3 + kx]); the
- * first SP_DN_NTRAIN are trained. */
-#define SP_DN_NTRAIN 3492
-#define SP_DN_NPARAMS 3496
+ * Five 3x3 convolutional layers (4 -> 32 -> 32 -> 32 -> 32 -> 4 channels,
+ * GELU after each but the last) on BF16 MFMA with FP32 accumulation.  Input
+ * x: Jx Jy Jz rho after the binomial filter, each divided by dens and a
+ * channel scale; output x + r (residual).  Parameters, one FP32 vector of
+ * SP_DN_NPARAMS: for each layer l = 0..4, w_l [cout][cin*9] then b_l [cout];
+ * then scales [4] (w[co][ci*9 + ky*3 + kx]); the first SP_DN_NTRAIN are
+ * trained. */
+#define SP_DN_NTRAIN 30084
+#define SP_DN_NPARAMS 30088
 typedef struct sp_dn sp_dn;
 /* A training context for batches of `batch` images of h x w, 4 planes each
  * ([batch][4][h][w], zero outside the image). */

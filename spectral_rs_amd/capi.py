@@ -25,11 +25,14 @@ SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_spe
            "sp_set_spectrum", "sp_zero_currents", "sp_move_and_deposit", "sp_deposit_ghosts",
            "sp_fields_update", "sp_boris_push", "sp_filter", "sp_fft2d", "sp_spectral_update",
            "sp_prestep", "sp_step", "sp_sync", "sp_set_timing", "sp_get_timers", "sp_set_graph",
-           "sp_get_diag", "sp_dn_create", "sp_dn_destroy", "sp_dn_last_error", "sp_dn_set_params",
+           "sp_get_diag", "sp_set_tuning", "sp_get_tuning", "sp_autotune", "sp_ensemble_create",
+           "sp_ensemble_destroy", "sp_ensemble_last_error", "sp_ensemble_set_graph", "sp_ensemble_step",
+           "sp_ensemble_sync", "sp_ensemble_stats", "sp_dn_create", "sp_dn_destroy", "sp_dn_last_error", "sp_dn_set_params",
            "sp_dn_get_params", "sp_dn_set_loss_weights", "sp_dn_forward", "sp_dn_gradient", "sp_dn_train",
            "sp_set_denoiser",
            "sp_get_denoiser_stats"]
-DN_NTRAIN, DN_NPARAMS = 3492, 3496
+DN_NTRAIN, DN_NPARAMS = 30084, 30088
+DN_LAYERS, DN_CH = 5, 32  # the denoiser: 5 convolutions, 32 hidden channels
 
 
 class SpParams(ct.Structure):
@@ -44,6 +47,15 @@ class SpDiag(ct.Structure):
     _fields_ = [("energy_e", ct.c_double), ("energy_b", ct.c_double), ("energy_kin", ct.c_double * 8),
                 ("momentum", ct.c_double * 3), ("gauss_pre", ct.c_double), ("gauss_post", ct.c_double),
                 ("gauss_post_max", ct.c_double), ("steps", ct.c_int64)]
+
+
+class SpTuning(ct.Structure):
+    _fields_ = [("chunk", ct.c_int32), ("tile_w_log", ct.c_int32), ("tile_h_log", ct.c_int32),
+                ("fft_rows", ct.c_int32), ("fft_cols", ct.c_int32), ("fft_radix_x", ct.c_int32),
+                ("fft_radix_y", ct.c_int32), ("reserved", ct.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class SpError(RuntimeError):
@@ -95,8 +107,19 @@ def load(path=LIB_PATH):
     L.sp_dn_train.argtypes = [vp, vp, vp, i32, i32, ct.c_float, ct.c_uint64, vp]
     L.sp_set_denoiser.argtypes = [vp, vp, i32, ct.c_float, ct.c_float]
     L.sp_get_denoiser_stats.argtypes = [vp, ct.POINTER(i32), ct.POINTER(i32)]
+    L.sp_set_tuning.argtypes = [vp, ct.POINTER(SpTuning)]
+    L.sp_get_tuning.argtypes = [vp, ct.POINTER(SpTuning)]
+    L.sp_autotune.argtypes = [vp, i32, i32, ct.POINTER(SpTuning)]
+    L.sp_ensemble_create.argtypes = [ct.POINTER(vp), i32, i32, i32, ct.POINTER(vp)]
+    L.sp_ensemble_destroy.argtypes = [vp]
+    L.sp_ensemble_last_error.argtypes = [vp]
+    L.sp_ensemble_last_error.restype = ct.c_char_p
+    L.sp_ensemble_set_graph.argtypes = [vp, i32]
+    L.sp_ensemble_step.argtypes = [vp, i32]
+    L.sp_ensemble_sync.argtypes = [vp, ct.POINTER(i32)]
+    L.sp_ensemble_stats.argtypes = [vp, ct.POINTER(i32), ct.POINTER(i32)]
     for n in SYMBOLS:
-        if n not in ("sp_last_error",):
+        if not n.endswith("_last_error"):
             getattr(L, n).restype = ct.c_int
     _lib = L
     return L
@@ -128,8 +151,8 @@ class GpuSim:
 
     def close(self):
         if getattr(self, "h", None):
-            load().sp_destroy(self.h)
-            self.h = None
+            if load().sp_destroy(self.h) == SP_OK:  # fails while the context is in an ensemble
+                self.h = None
 
     def __del__(self):
         try:
@@ -237,6 +260,69 @@ class GpuSim:
         a, f = ct.c_int32(), ct.c_int32()
         self._chk(load().sp_get_denoiser_stats(self.h, ct.byref(a), ct.byref(f)))
         return a.value, f.value
+
+    # ---- launch shapes ----
+    def tuning(self):
+        t = SpTuning()
+        self._chk(load().sp_get_tuning(self.h, ct.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        """Change some knobs of sp_tuning (the others keep their values)."""
+        t = SpTuning()
+        self._chk(load().sp_get_tuning(self.h, ct.byref(t)))
+        for k, v in kw.items():
+            if k not in t.as_dict():
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        self._chk(load().sp_set_tuning(self.h, ct.byref(t)))
+
+    def autotune(self, trials=3, use_cache=True):
+        t = SpTuning()
+        self._chk(load().sp_autotune(self.h, trials, int(use_cache), ct.byref(t)))
+        return t.as_dict()
+
+
+class Ensemble:
+    """Independent runs stepped together (sp_ensemble_*): the members keep
+    their own state and getters; they must outlive the ensemble."""
+
+    def __init__(self, sims, branches=4, unit=2):
+        L = load()
+        self.sims = list(sims)
+        arr = (ct.c_void_p * len(self.sims))(*[s.h for s in self.sims])
+        e = ct.c_void_p()
+        rc = L.sp_ensemble_create(arr, len(self.sims), branches, unit, ct.byref(e))
+        if rc != SP_OK:
+            raise SpError(rc, L.sp_ensemble_last_error(None).decode())
+        self.e = e
+
+    def close(self):
+        if getattr(self, "e", None):
+            load().sp_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != SP_OK:
+            raise SpError(rc, load().sp_ensemble_last_error(self.e).decode())
+
+    def set_graph(self, on=True): self._chk(load().sp_ensemble_set_graph(self.e, int(on)))
+    def step(self, n=1): self._chk(load().sp_ensemble_step(self.e, n))
+
+    def sync(self):
+        bad = ct.c_int32(-1)
+        self._chk(load().sp_ensemble_sync(self.e, ct.byref(bad)))
+
+    def stats(self):
+        k, c = ct.c_int32(), ct.c_int32()
+        self._chk(load().sp_ensemble_stats(self.e, ct.byref(k), ct.byref(c)))
+        return dict(kept=k.value, captures=c.value)
 
 
 class Denoiser:

@@ -13,7 +13,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -48,7 +51,11 @@ struct sp_ctx {
   sp_params p{};
   bool f64 = false;
   size_t esz = 4;
-  hipStream_t st = nullptr;
+  hipStream_t st = nullptr;      // the stream every launch of this context goes to
+  hipStream_t own_st = nullptr;  // created by sp_create; st differs from it inside an ensemble
+  sp_ensemble* ens = nullptr;    // the ensemble this context belongs to, if any
+  sp_tuning tune{};              // launch shapes (sp_set_tuning / sp_autotune)
+  int64_t tune_gen = 0;          // bumped on every tuning change (graph signatures)
   int device = 0;
   void* grid[11] = {};
   void* work[4] = {};
@@ -59,9 +66,9 @@ struct sp_ctx {
   int* tile_count = nullptr;   // per-tile particle counts of the tile sort
   int* tile_cursor = nullptr;  // per-tile write positions
   double* gauss_d = nullptr;   // Gauss-law sums of the last solve + block partials
-  // learned denoiser (sp_set_denoiser): parameters, activations, sums, counters
+  // learned denoiser (sp_set_denoiser): parameters, input/correction planes, sums, counters
   bool dn_on = false;
-  float* dn_buf = nullptr;  // parameters + x (4) + a1 (16) + a2 (16) + r (4) planes
+  float* dn_buf = nullptr;  // parameters + x (4 planes) + r (4 planes)
   double* dn_sums = nullptr;
   int* dn_stats = nullptr;
   float dn_tol = 0, dn_gtol = 0;
@@ -84,6 +91,18 @@ struct sp_ctx {
 };
 
 namespace {
+
+sp_tuning default_tuning(bool f64) {
+  sp_tuning t{};
+  t.chunk = f64 ? 1024 : 2048;
+  t.tile_w_log = 3;
+  t.tile_h_log = 3;
+  t.fft_rows = 1;
+  t.fft_cols = 8;
+  t.fft_radix_x = 8;
+  t.fft_radix_y = 8;
+  return t;
+}
 
 int fail(sp_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -111,6 +130,7 @@ template <class T> Geo<T> geo(const sp_ctx* c) {
   g.left_edge = c->p.left_edge;
   g.lds_ok = c->p.dt * c->p.c < 1.0;
   g.direct = c->p.sort_interval > 0;
+  g.chunk = c->tune.chunk;
   g.dt = (T)c->p.dt;
   g.c = (T)c->p.c;
   g.csqinv = T(1.) / (g.c * g.c);
@@ -143,7 +163,9 @@ GridSet gset(const sp_ctx* c) {
   return g;
 }
 
-bool plan_dim(FftDim& d, int n) {
+// Radix plan of one FFT dimension: the largest radices first, at most
+// `maxr` (8 or 4), then 2 and 3.
+bool plan_dim(FftDim& d, int n, int maxr = 8) {
   d.n = n;
   d.nrad = 0;
   d.radix_code = 0;
@@ -153,7 +175,7 @@ bool plan_dim(FftDim& d, int n) {
     d.nrad++;
     m /= r;
   };
-  while (m % 8 == 0) push(8);
+  while (maxr >= 8 && m % 8 == 0) push(8);
   while (m % 4 == 0) push(4);
   while (m % 2 == 0) push(2);
   while (m % 3 == 0) push(3);
@@ -283,14 +305,13 @@ template <class T> void diag_sums(sp_ctx* c) {
     launch_kinetic<T>(sdev<T>(c, c->species[s]), csqinv, c->diag_d + DIAG_FIXED + 4 * s, c->st);
 }
 
-// Activation planes of the in-loop denoiser, after the parameters in dn_buf.
+// Planes of the in-loop denoiser after the parameters in dn_buf: its input x
+// and correction r (4 channels each; the fused kernel keeps the rest in LDS).
 DnTensors dn_tensors(const sp_ctx* c) {
   const size_t n = c->nspec_elems();
   DnTensors t{};
   t.x = c->dn_buf + DN_NPARAMS;
-  t.a1 = t.x + 4 * n;
-  t.a2 = t.a1 + 16 * n;
-  t.r = t.a2 + 16 * n;
+  t.r = t.x + DN_IN * n;
   return t;
 }
 
@@ -353,8 +374,8 @@ template <class T> int graph_period_run(sp_ctx* c) {
 // Tile-sorts every species (particle order only; no value changes).
 template <class T> void sort_species(sp_ctx* c) {
   for (auto& s : c->species) {
-    launch_tile_sort<T>(sdev<T>(c, s, s.cur), sdev<T>(c, s, s.alt), c->p.size_x, c->p.size_y, c->tile_count,
-                        c->tile_cursor, c->st);
+    launch_tile_sort<T>(sdev<T>(c, s, s.cur), sdev<T>(c, s, s.alt), c->p.size_x, c->p.size_y, c->tune.tile_w_log,
+                        c->tune.tile_h_log, c->tile_count, c->tile_cursor, c->st);
     std::swap(s.cur, s.alt);
   }
 }
@@ -445,6 +466,307 @@ void free_species(SpeciesBuf& s) {
 
 }  // namespace
 
+namespace {
+
+// ---- tuning and the autotuner ----
+int check_tuning(sp_ctx* c, const sp_tuning& t) {
+  if (c->f64 ? t.chunk != 1024 : (t.chunk != 512 && t.chunk != 1024 && t.chunk != 2048))
+    return fail(c, SP_EINVAL, c->f64 ? "chunk must be 1024 in FP64" : "chunk must be 512, 1024 or 2048");
+  if (t.tile_w_log < TILE_LOG_MIN || t.tile_w_log > TILE_LOG_MAX || t.tile_h_log < TILE_LOG_MIN ||
+      t.tile_h_log > TILE_LOG_MAX)
+    return fail(c, SP_EINVAL, "tile logs must be in 1..5");
+  if (t.fft_rows < 1 || t.fft_rows > 64 || t.fft_cols < 1 || t.fft_cols > 64)
+    return fail(c, SP_EINVAL, "fft_rows and fft_cols must be in 1..64");
+  if ((t.fft_radix_x != 8 && t.fft_radix_x != 4) || (t.fft_radix_y != 8 && t.fft_radix_y != 4))
+    return fail(c, SP_EINVAL, "FFT radix caps must be 8 or 4");
+  return SP_OK;
+}
+
+void apply_tuning(sp_ctx* c, const sp_tuning& t) {
+  c->tune = t;
+  plan_dim(c->fx, c->p.size_x, t.fft_radix_x);  // keeps the twiddle table: it depends on n only
+  plan_dim(c->fy, c->p.size_y, t.fft_radix_y);
+  c->fx.per_block = t.fft_rows;
+  c->fy.per_block = t.fft_cols;
+  c->tune_gen++;
+  drop_graph(c);
+}
+
+// A scratch context with the parameters, particles and fields of c.
+int clone_ctx(sp_ctx* c, sp_ctx** out) {
+  sp_ctx* t = nullptr;
+  if (sp_create(&c->p, &t) != SP_OK) return fail(c, SP_EHIP, std::string("scratch context: ") + sp_last_error(nullptr));
+  auto bail = [&](const std::string& m) {
+    sp_destroy(t);
+    return fail(c, SP_EHIP, "scratch context: " + m);
+  };
+  for (const auto& s : c->species) {
+    SpeciesBuf d;
+    d.charge = s.charge;
+    d.mass = s.mass;
+    if (alloc_species(t, d, s.n) != SP_OK) {
+      free_species(d);
+      return bail(t->err);
+    }
+    t->species.push_back(d);
+    if (s.n == 0) continue;
+    const size_t n = (size_t)s.n;
+    hipMemcpyAsync(d.cur.cell, s.cur.cell, n * 4, hipMemcpyDeviceToDevice, c->st);
+    hipMemcpyAsync(d.cur.tag, s.cur.tag, n * 4, hipMemcpyDeviceToDevice, c->st);
+    const void* src[5] = {s.cur.dx, s.cur.dy, s.cur.px, s.cur.py, s.cur.pz};
+    void* dst[5] = {d.cur.dx, d.cur.dy, d.cur.px, d.cur.py, d.cur.pz};
+    for (int k = 0; k < 5; ++k) hipMemcpyAsync(dst[k], src[k], n * c->esz, hipMemcpyDeviceToDevice, c->st);
+  }
+  for (int i = 0; i < 11; ++i)
+    hipMemcpyAsync(t->grid[i], c->grid[i], c->ngrid_elems() * c->esz, hipMemcpyDeviceToDevice, c->st);
+  for (int i = 0; i < 14; ++i)
+    hipMemcpyAsync(t->spec[i], c->spec[i], c->nspec_elems() * 2 * c->esz, hipMemcpyDeviceToDevice, c->st);
+  const hipError_t e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess) return bail(hipGetErrorString(e));
+  *out = t;
+  return SP_OK;
+}
+
+// Mean time of `trials` runs of fn on t's stream, after one untimed run.
+template <class F> float time_ms(sp_ctx* t, int trials, F&& fn) {
+  fn();
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return 1e30f;
+  if (hipEventCreate(&b) != hipSuccess) {
+    hipEventDestroy(a);
+    return 1e30f;
+  }
+  hipEventRecord(a, t->st);
+  for (int i = 0; i < trials; ++i) fn();
+  hipEventRecord(b, t->st);
+  float ms = 1e30f;
+  if (hipEventSynchronize(b) == hipSuccess && hipGetLastError() == hipSuccess) hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return ms / (float)trials;
+}
+
+template <class T> void particle_trial(sp_ctx* t) {
+  zero_currents<T>(t);
+  const Geo<T> g = geo<T>(t);
+  const GridSet gs = gset(t);
+  for (auto& s : t->species) launch_push_move_deposit<T>(sdev<T>(t, s), g, gs, t->flags_d, t->st);
+}
+
+template <class T> sp_tuning autotune_on(sp_ctx* t, int trials) {
+  sp_tuning best = t->tune;
+  // chunk x sort tile: the particle pass after a sort with that tile
+  {
+    std::vector<int> chunks = sizeof(T) == 8 ? std::vector<int>{1024} : std::vector<int>{512, 1024, 2048};
+    std::vector<std::pair<int, int>> tiles = {{2, 2}, {3, 2}, {3, 3}, {4, 3}, {4, 4}, {5, 3}};
+    if (t->p.sort_interval <= 0) tiles = {{t->tune.tile_w_log, t->tune.tile_h_log}};
+    float bt = 1e30f;
+    sp_tuning cand = best;
+    for (int k : chunks)
+      for (const auto& tl : tiles) {
+        cand.chunk = k;
+        cand.tile_w_log = tl.first;
+        cand.tile_h_log = tl.second;
+        apply_tuning(t, cand);
+        if (t->p.sort_interval > 0) sort_species<T>(t);
+        const float ms = time_ms(t, trials, [&] { particle_trial<T>(t); });
+        if (ms < bt) {
+          bt = ms;
+          best.chunk = k;
+          best.tile_w_log = tl.first;
+          best.tile_h_log = tl.second;
+        }
+      }
+  }
+  const int maxlen = fft_max_len<T>();
+  // radix x rows per workgroup, then radix x columns per workgroup: the field solve
+  {
+    float bt = 1e30f;
+    sp_tuning cand = best;
+    for (int r : {8, 4})
+      for (int rows = 1; rows <= 64 && rows * t->p.size_x <= maxlen && rows <= t->p.size_y; rows *= 2) {
+        cand.fft_radix_x = r;
+        cand.fft_rows = rows;
+        apply_tuning(t, cand);
+        const float ms = time_ms(t, trials, [&] { fields_update<T>(t); });
+        if (ms < bt) {
+          bt = ms;
+          best.fft_radix_x = r;
+          best.fft_rows = rows;
+        }
+      }
+  }
+  {
+    float bt = 1e30f;
+    sp_tuning cand = best;
+    for (int r : {8, 4})
+      for (int cols = 1; cols <= 64 && cols * t->p.size_y <= maxlen && cols <= t->p.size_x; cols *= 2) {
+        cand.fft_radix_y = r;
+        cand.fft_cols = cols;
+        apply_tuning(t, cand);
+        const float ms = time_ms(t, trials, [&] { fields_update<T>(t); });
+        if (ms < bt) {
+          bt = ms;
+          best.fft_radix_y = r;
+          best.fft_cols = cols;
+        }
+      }
+  }
+  return best;
+}
+
+std::mutex g_tune_mu;
+std::map<std::vector<int64_t>, sp_tuning> g_tune_cache;
+
+std::vector<int64_t> tune_key(const sp_ctx* c) {
+  return {c->p.size_x, c->p.size_y, c->p.delta, c->p.dens, (int64_t)c->species.size(), c->f64 ? 1 : 0,
+          c->p.sort_interval, c->p.n_pass};
+}
+
+}  // namespace
+
+// ---- ensembles ----
+namespace {
+// The host-side state of a context that one step changes: grid/work pointers
+// (the filter swaps them), species buffers (a sort swaps them), push state.
+struct MemberState {
+  void* grid[11];
+  void* work[4];
+  std::vector<std::pair<SoA, SoA>> sp;
+  bool push_pending;
+};
+MemberState save_state(const sp_ctx* c) {
+  MemberState m;
+  std::memcpy(m.grid, c->grid, sizeof(m.grid));
+  std::memcpy(m.work, c->work, sizeof(m.work));
+  for (const auto& s : c->species) m.sp.emplace_back(s.cur, s.alt);
+  m.push_pending = c->push_pending;
+  return m;
+}
+void load_state(sp_ctx* c, const MemberState& m) {
+  std::memcpy(c->grid, m.grid, sizeof(m.grid));
+  std::memcpy(c->work, m.work, sizeof(m.work));
+  for (size_t i = 0; i < c->species.size(); ++i) {
+    c->species[i].cur = m.sp[i].first;
+    c->species[i].alt = m.sp[i].second;
+  }
+  c->push_pending = m.push_pending;
+}
+}  // namespace
+
+struct sp_ensemble {
+  std::vector<sp_ctx*> m;
+  std::vector<int> branch_of;
+  int nb = 1, unit = 2;
+  bool graph_on = true;
+  hipStream_t st = nullptr;
+  std::vector<hipStream_t> bst;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  struct Entry {
+    std::vector<int64_t> sig;
+    std::vector<MemberState> post;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<Entry> kept;
+  int captures = 0;
+  std::string err;
+};
+
+namespace {
+
+int efail(sp_ensemble* e, int code, const std::string& msg) {
+  e->err = msg;
+  return code;
+}
+
+void drop_kept(sp_ensemble* e) {
+  for (auto& k : e->kept)
+    if (k.exec) hipGraphExecDestroy(k.exec);
+  e->kept.clear();
+}
+
+// Everything the launches of the next `unit` steps depend on.
+std::vector<int64_t> ens_signature(const sp_ensemble* e) {
+  std::vector<int64_t> v;
+  for (const sp_ctx* c : e->m) {
+    for (const void* p : graph_signature(c)) v.push_back((int64_t)(intptr_t)p);
+    v.push_back(c->push_pending);
+    v.push_back(c->tune_gen);
+    v.push_back(c->dn_on);
+    for (int j = 0; j < e->unit; ++j)
+      v.push_back(c->p.sort_interval > 0 && (c->steps + j) % c->p.sort_interval == 0);
+  }
+  return v;
+}
+
+// One step of every member: fork into the branch streams, each member's
+// launches on its branch, join back into the ensemble stream.
+void ens_step_forked(sp_ensemble* e) {
+  hipEventRecord(e->fork, e->st);
+  for (int b = 0; b < e->nb; ++b) hipStreamWaitEvent(e->bst[b], e->fork, 0);
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    sp_ctx* c = e->m[i];
+    c->st = e->bst[e->branch_of[i]];
+    if (c->f64) one_step<double>(c);
+    else one_step<float>(c);
+    c->st = e->st;
+  }
+  for (int b = 0; b < e->nb; ++b) {
+    hipEventRecord(e->join[b], e->bst[b]);
+    hipStreamWaitEvent(e->st, e->join[b], 0);
+  }
+}
+
+// `unit` steps of every member as one graph launch; captured when no kept
+// graph matches the members' state.
+int ens_graph_unit(sp_ensemble* e) {
+  const std::vector<int64_t> sig = ens_signature(e);
+  for (const auto& k : e->kept) {
+    if (k.sig != sig) continue;
+    const hipError_t r = hipGraphLaunch(k.exec, e->st);
+    if (r != hipSuccess) return efail(e, SP_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(r));
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      load_state(e->m[i], k.post[i]);
+      e->m[i]->steps += e->unit;
+    }
+    return SP_OK;
+  }
+  if (e->kept.size() >= 16) drop_kept(e);
+  std::vector<MemberState> pre;
+  std::vector<int64_t> steps0;
+  for (sp_ctx* c : e->m) {
+    pre.push_back(save_state(c));
+    steps0.push_back(c->steps);
+  }
+  auto undo = [&](const std::string& what, hipError_t r) {
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      load_state(e->m[i], pre[i]);
+      e->m[i]->steps = steps0[i];
+    }
+    return efail(e, SP_EHIP, what + ": " + hipGetErrorString(r));
+  };
+  hipError_t r = hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal);
+  if (r != hipSuccess) return undo("hipStreamBeginCapture", r);
+  for (int u = 0; u < e->unit; ++u) ens_step_forked(e);
+  hipGraph_t graph = nullptr;
+  r = hipStreamEndCapture(e->st, &graph);
+  if (r != hipSuccess) return undo("hipStreamEndCapture", r);
+  sp_ensemble::Entry k;
+  k.sig = sig;
+  r = hipGraphInstantiate(&k.exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (r != hipSuccess) return undo("hipGraphInstantiate", r);
+  for (sp_ctx* c : e->m) k.post.push_back(save_state(c));
+  e->captures++;
+  r = hipGraphLaunch(k.exec, e->st);
+  e->kept.push_back(std::move(k));
+  if (r != hipSuccess) return efail(e, SP_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(r));
+  return SP_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int sp_create(const sp_params* params, sp_ctx** out) {
@@ -493,7 +815,11 @@ int sp_create(const sp_params* params, sp_ctx** out) {
     }                                                                                \
   } while (0)
   CK(hipGetDevice(&c->device));
-  CK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  CK(hipStreamCreateWithFlags(&c->own_st, hipStreamNonBlocking));
+  c->st = c->own_st;
+  c->tune = default_tuning(c->f64);
+  c->fx.per_block = c->tune.fft_rows;
+  c->fy.per_block = c->tune.fft_cols;
   const size_t gb = c->ngrid_elems() * c->esz, sb = c->nspec_elems() * 2 * c->esz;
   for (auto& g : c->grid) CK(hipMalloc(&g, gb));
   for (auto& g : c->work) CK(hipMalloc(&g, gb));
@@ -511,7 +837,9 @@ int sp_create(const sp_params* params, sp_ctx** out) {
     fft_make_twiddles<float>(c->fy, c->st);
   }
   {
-    const size_t ntiles = (size_t)((p.size_x + TILE_W - 1) / TILE_W) * ((p.size_y + TILE_H - 1) / TILE_H);
+    // room for the smallest sort tile a tuning may pick
+    const int tmin = 1 << TILE_LOG_MIN;
+    const size_t ntiles = (size_t)((p.size_x + tmin - 1) / tmin) * ((p.size_y + tmin - 1) / tmin);
     CK(hipMalloc(&c->tile_count, ntiles * sizeof(int)));
     CK(hipMalloc(&c->tile_cursor, ntiles * sizeof(int)));
   }
@@ -533,6 +861,7 @@ int sp_create(const sp_params* params, sp_ctx** out) {
 
 int sp_destroy(sp_ctx* c) {
   if (!c) return SP_OK;
+  if (c->ens) return fail(c, SP_EINVAL, "the context belongs to an ensemble: destroy the ensemble first");
   if (c->st) hipStreamSynchronize(c->st);
   drop_graph(c);
   hipFree(c->dn_buf);
@@ -552,7 +881,7 @@ int sp_destroy(sp_ctx* c) {
   if (c->flags_h) hipHostFree(c->flags_h);
   for (auto& tp : c->open_pairs) { hipEventDestroy(tp.a); hipEventDestroy(tp.b); }
   for (auto e : c->free_events) hipEventDestroy(e);
-  if (c->st) hipStreamDestroy(c->st);
+  if (c->own_st) hipStreamDestroy(c->own_st);
   delete c;
   return SP_OK;
 }
@@ -826,7 +1155,7 @@ int sp_set_denoiser(sp_ctx* c, const float* params, int n, float max_rel, float 
   if (n != DN_NPARAMS) return fail(c, SP_EINVAL, "expected SP_DN_NPARAMS parameters");
   if (!(max_rel > 0) || !(max_gauss > 0)) return fail(c, SP_EINVAL, "tolerances must be positive");
   if (!c->dn_buf) {
-    const size_t planes = 40 * c->nspec_elems();
+    const size_t planes = 2 * DN_IN * c->nspec_elems();
     HIPCHK(c, hipMalloc(&c->dn_buf, (DN_NPARAMS + planes) * sizeof(float)));
     HIPCHK(c, hipMalloc(&c->dn_sums, 2 * sizeof(double)));
     HIPCHK(c, hipMalloc(&c->dn_stats, 2 * sizeof(int)));
@@ -854,6 +1183,7 @@ int sp_get_denoiser_stats(sp_ctx* c, int* applied, int* fallbacks) {
 
 int sp_set_graph(sp_ctx* c, int on) {
   if (!c) return SP_EINVAL;
+  if (on && c->ens) return fail(c, SP_EINVAL, "the ensemble captures its members' steps");
   c->graph_on = on != 0;
   if (!c->graph_on) drop_graph(c);
   return SP_OK;
@@ -868,6 +1198,7 @@ int sp_sync(sp_ctx* c) {
 
 int sp_set_timing(sp_ctx* c, int on) {
   if (!c) return SP_EINVAL;
+  if (on && c->ens) return fail(c, SP_EINVAL, "no phase timers inside an ensemble");
   c->timing = on != 0;
   return SP_OK;
 }
@@ -912,6 +1243,191 @@ int sp_get_diag(sp_ctx* c, sp_diag* out) {
   out->gauss_post = rho2 > 0 ? std::sqrt(g[1] / rho2) : std::sqrt(g[1]);
   out->gauss_post_max = g[3];
   out->steps = c->steps;
+  return SP_OK;
+}
+
+
+int sp_set_tuning(sp_ctx* c, const sp_tuning* t) {
+  if (!c || !t) return SP_EINVAL;
+  int rc = check_tuning(c, *t);
+  if (rc) return rc;
+  rc = flush(c);
+  if (rc) return rc;
+  apply_tuning(c, *t);
+  return SP_OK;
+}
+
+int sp_get_tuning(const sp_ctx* c, sp_tuning* t) {
+  if (!c || !t) return SP_EINVAL;
+  *t = c->tune;
+  return SP_OK;
+}
+
+int sp_autotune(sp_ctx* c, int trials, int use_cache, sp_tuning* out) {
+  ENTRY_PRELUDE(c);
+  if (trials < 1 || trials > 100) return fail(c, SP_EINVAL, "trials must be in 1..100");
+  if (c->species.empty()) return fail(c, SP_EINVAL, "the autotuner needs the context's species");
+  const std::vector<int64_t> key = tune_key(c);
+  if (use_cache) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      apply_tuning(c, it->second);
+      if (out) *out = c->tune;
+      return SP_OK;
+    }
+  }
+  sp_ctx* t = nullptr;
+  int rc = clone_ctx(c, &t);
+  if (rc) return rc;
+  t->tune = c->tune;
+  const sp_tuning best = c->f64 ? autotune_on<double>(t, trials) : autotune_on<float>(t, trials);
+  const hipError_t e = hipStreamSynchronize(t->st);
+  sp_destroy(t);
+  if (e != hipSuccess) return fail(c, SP_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+  rc = check_tuning(c, best);
+  if (rc) return rc;
+  apply_tuning(c, best);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  if (out) *out = best;
+  return SP_OK;
+}
+
+int sp_ensemble_create(sp_ctx* const* members, int n, int branches, int unit, sp_ensemble** out) {
+  g_create_error.clear();
+  if (!members || !out || n < 1) return fail(nullptr, SP_EINVAL, "null argument or empty ensemble");
+  if (branches < 1 || branches > 16) return fail(nullptr, SP_EINVAL, "branches must be in 1..16");
+  if (unit < 1 || unit > 64) return fail(nullptr, SP_EINVAL, "unit must be in 1..64");
+  for (int i = 0; i < n; ++i) {
+    sp_ctx* c = members[i];
+    if (!c) return fail(nullptr, SP_EINVAL, "null member");
+    if (c->ens) return fail(nullptr, SP_EINVAL, "a member already belongs to an ensemble");
+    if (c->timing || c->graph_on) return fail(nullptr, SP_EINVAL, "members must have timing and graph off");
+    if (c->device != members[0]->device) return fail(nullptr, SP_EINVAL, "members must share one device");
+    for (int j = 0; j < i; ++j)
+      if (members[j] == c) return fail(nullptr, SP_EINVAL, "a context appears twice");
+  }
+  sp_ensemble* e = new sp_ensemble();
+  e->m.assign(members, members + n);
+  e->nb = std::min(branches, n);
+  e->unit = unit;
+  auto bail = [&](const char* what, hipError_t r) {
+    for (auto s : e->bst) hipStreamDestroy(s);
+    for (auto v : e->join) hipEventDestroy(v);
+    if (e->fork) hipEventDestroy(e->fork);
+    if (e->st) hipStreamDestroy(e->st);
+    delete e;
+    return fail(nullptr, SP_EHIP, std::string(what) + ": " + hipGetErrorString(r));
+  };
+  hipError_t r = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking);
+  if (r != hipSuccess) return bail("hipStreamCreateWithFlags", r);
+  for (int b = 0; b < e->nb; ++b) {
+    hipStream_t s;
+    hipEvent_t v;
+    r = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (r != hipSuccess) return bail("hipStreamCreateWithFlags", r);
+    e->bst.push_back(s);
+    r = hipEventCreateWithFlags(&v, hipEventDisableTiming);
+    if (r != hipSuccess) return bail("hipEventCreateWithFlags", r);
+    e->join.push_back(v);
+  }
+  r = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+  if (r != hipSuccess) return bail("hipEventCreateWithFlags", r);
+  // branches balanced by particle count: largest member to the lightest branch
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  auto load = [&](int i) {
+    int64_t k = 0;
+    for (const auto& s : members[i]->species) k += s.n;
+    return k + (int64_t)members[i]->ngrid_elems();
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return load(a) > load(b); });
+  std::vector<int64_t> w(e->nb, 0);
+  e->branch_of.assign(n, 0);
+  for (int i : order) {
+    const int b = (int)(std::min_element(w.begin(), w.end()) - w.begin());
+    e->branch_of[i] = b;
+    w[b] += load(i);
+  }
+  for (sp_ctx* c : e->m) {
+    r = hipStreamSynchronize(c->st);
+    if (r != hipSuccess) return bail("hipStreamSynchronize", r);
+  }
+  for (sp_ctx* c : e->m) {
+    c->st = e->st;
+    c->ens = e;
+  }
+  *out = e;
+  return SP_OK;
+}
+
+int sp_ensemble_destroy(sp_ensemble* e) {
+  if (!e) return SP_OK;
+  hipStreamSynchronize(e->st);
+  drop_kept(e);
+  for (sp_ctx* c : e->m) {
+    c->st = c->own_st;
+    c->ens = nullptr;
+  }
+  for (auto s : e->bst) hipStreamDestroy(s);
+  for (auto v : e->join) hipEventDestroy(v);
+  hipEventDestroy(e->fork);
+  hipStreamDestroy(e->st);
+  delete e;
+  return SP_OK;
+}
+
+const char* sp_ensemble_last_error(const sp_ensemble* e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
+int sp_ensemble_set_graph(sp_ensemble* e, int on) {
+  if (!e) return SP_EINVAL;
+  e->graph_on = on != 0;
+  if (!e->graph_on) drop_kept(e);
+  return SP_OK;
+}
+
+int sp_ensemble_step(sp_ensemble* e, int n) {
+  if (!e) return SP_EINVAL;
+  if (n < 0) return efail(e, SP_EINVAL, "negative step count");
+  for (sp_ctx* c : e->m)
+    if (c->timing || c->graph_on) return efail(e, SP_EINVAL, "members must have timing and graph off");
+  for (int i = 0; i < n;) {
+    if (e->graph_on && n - i >= e->unit) {
+      const int rc = ens_graph_unit(e);
+      if (rc) return rc;
+      i += e->unit;
+    } else {
+      ens_step_forked(e);
+      ++i;
+    }
+  }
+  const hipError_t r = hipGetLastError();
+  if (r != hipSuccess) return efail(e, SP_EHIP, hipGetErrorString(r));
+  return SP_OK;
+}
+
+int sp_ensemble_sync(sp_ensemble* e, int* bad_member) {
+  if (!e) return SP_EINVAL;
+  if (bad_member) *bad_member = -1;
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    sp_ctx* c = e->m[i];
+    int rc = flush(c);
+    if (!rc) rc = sync_check(c);
+    if (rc) {
+      if (bad_member) *bad_member = (int)i;
+      return efail(e, rc, "member " + std::to_string(i) + ": " + c->err);
+    }
+  }
+  return SP_OK;
+}
+
+int sp_ensemble_stats(const sp_ensemble* e, int* kept, int* captures) {
+  if (!e) return SP_EINVAL;
+  if (kept) *kept = (int)e->kept.size();
+  if (captures) *captures = e->captures;
   return SP_OK;
 }
 

@@ -3,20 +3,27 @@
 // run by the kernels below, applied after the binomial filter at the point of
 // src/flds/mod.rs:90-94.
 //
-// Network: x (4 channels: Jx Jy Jz rho after the binomial filter, each divided
-// by dens and by a per-channel scale s_c) -> conv3x3(4->16) + b, ReLU ->
-// conv3x3(16->16) + b, ReLU -> conv3x3(16->4) + b = r; output x + r, times
-// s_c dens.  "same" 3x3 convolutions; outside the image the input is zero,
-// except that in the simulation y is periodic.
+// Network (DN_LAYERS = 5 layers of 3x3 convolutions, DN_CH = 32 channels):
+// x (4 channels: Jx Jy Jz rho after the binomial filter, each divided by dens
+// and by a per-channel scale s_c) -> z0 = conv(4->32) + b, a0 = GELU(z0) ->
+// z_l = conv(32->32)(a_{l-1}) + b, a_l = GELU(z_l) for l = 1..3 ->
+// r = conv(32->4)(a3) + b; output x + r (residual), times s_c dens.  "same"
+// convolutions; outside the image the input is zero, except that in the
+// simulation y is periodic.  GELU(z) = z Phi(z) = z (1 + erf(z / sqrt 2)) / 2.
 //
 // Every convolution is an implicit GEMM on v_mfma_f32_16x16x16bf16_1k: a
 // workgroup owns 8 x 32 output pixels, stages its input tile (+1 halo) and the
-// weights in LDS as BF16, and each wave computes 16-pixel x 16-channel blocks,
-// the K dimension being (input channel, 3x3 tap) padded to 16.  Operand
-// layouts (checked on gfx950): A row = lane % 16, k = 4 (lane / 16) + i;
-// B column = lane % 16, same k; D rows 4 (lane / 16) + i, column lane % 16.
-// The weight gradient is the same MFMA with pixels as the K dimension.
+// weights in LDS as BF16, and each wave computes 16-pixel x 32-channel blocks
+// (two 16-column MFMA tiles sharing the A operand), the K dimension being
+// (input channel, 3x3 tap) padded to 16.  Operand layouts (checked on gfx950):
+// A row = lane % 16, k = 4 (lane / 16) + i; B column = lane % 16, same k;
+// D rows 4 (lane / 16) + i, column lane % 16.  Training keeps the
+// pre-activations z in FP32; a layer's input a = GELU(z) is formed when the
+// next layer stages it (rounded to BF16 there, as the MFMA reads it).  The
+// weight gradient is the same MFMA with pixels as the K dimension.
 #include "sp_internal.h"
+
+static_assert(SP_DN_NTRAIN == sp::DN_NTRAIN && SP_DN_NPARAMS == sp::DN_NPARAMS, "parameter count of the C ABI");
 
 namespace sp {
 
@@ -29,6 +36,12 @@ __device__ __forceinline__ short bf16_bits(float f) {
   return (short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// d GELU / dz = Phi(z) + z phi(z)
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
 constexpr int DN_TY = 8, DN_TX = 32, DN_NT = 256;  // output pixels of a workgroup: 8 rows x 32 columns
 
 template <int CIN> struct ConvShape {
@@ -36,8 +49,9 @@ template <int CIN> struct ConvShape {
 };
 
 // Stages rows y0-1 .. y0+DN_TY and columns x0-1 .. x0+DN_TX of CIN planes of
-// one image as BF16.  PERIODIC_Y wraps rows; columns outside are zero.
-template <int CIN, bool PERIODIC_Y>
+// one image as BF16, GELU applied first when GIN (the planes are the previous
+// layer's pre-activations).  PERIODIC_Y wraps rows; columns outside are zero.
+template <int CIN, bool PERIODIC_Y, bool GIN>
 __device__ __forceinline__ void stage_tile(short* s_in, const float* __restrict__ img, int H, int W, int y0, int x0) {
   using S = ConvShape<CIN>;
   for (int i = threadIdx.x; i < CIN * S::LY * S::LX; i += DN_NT) {
@@ -47,7 +61,10 @@ __device__ __forceinline__ void stage_tile(short* s_in, const float* __restrict_
     const int x = x0 + c - 1;
     if (PERIODIC_Y) y = y < 0 ? y + H : (y >= H ? y - H : y);
     float v = 0.0f;
-    if (y >= 0 && y < H && x >= 0 && x < W) v = img[((size_t)ci * H + y) * W + x];
+    if (y >= 0 && y < H && x >= 0 && x < W) {
+      v = img[((size_t)ci * H + y) * W + x];
+      if (GIN) v = gelu(v);
+    }
     s_in[i] = bf16_bits(v);
   }
 }
@@ -64,20 +81,22 @@ template <int CIN> __device__ __forceinline__ void stage_offsets(int* s_off) {
   }
 }
 
-// out[b][n][y][x] = post(bias[n] + sum_{ci,ky,kx} w[n][ci*9+ky*3+kx] in[b][ci][y+ky-1][x+kx-1]).
-// POST: 0 none, 1 ReLU, 2 times (mask > 0) -- the ReLU derivative when the
-// layer is a backward (transposed) convolution and mask the forward activation.
-template <int CIN, int COUT, int POST, bool PERIODIC_Y>
+// out[b][n][y][x] = post(bias[n] + sum_{ci,ky,kx} w[n][ci*9+ky*3+kx] in'[b][ci][y+ky-1][x+kx-1])
+// with in' = GELU(in) when GIN, else in.  POST: 0 none; 2 times GELU'(mask) --
+// the layer is then a backward (transposed) convolution and mask the forward
+// pre-activation of its output.
+template <int CIN, int COUT, int POST, bool PERIODIC_Y, bool GIN>
 __global__ void __launch_bounds__(DN_NT) k_conv3x3(const float* __restrict__ in, const float* __restrict__ w,
                                                    const float* __restrict__ bias, const float* __restrict__ mask,
                                                    float* __restrict__ out, int H, int W) {
   using S = ConvShape<CIN>;
+  constexpr int NTL = (COUT + 15) / 16;  // 16-channel output tiles
   __shared__ short s_in[CIN * S::LY * S::LX];
-  __shared__ short s_w[16 * S::KP];
+  __shared__ short s_w[NTL * 16 * S::KP];
   __shared__ int s_off[S::KP];
   const int b = blockIdx.z, y0 = blockIdx.y * DN_TY, x0 = blockIdx.x * DN_TX;
-  stage_tile<CIN, PERIODIC_Y>(s_in, in + (size_t)b * CIN * H * W, H, W, y0, x0);
-  for (int i = threadIdx.x; i < 16 * S::KP; i += DN_NT) {
+  stage_tile<CIN, PERIODIC_Y, GIN>(s_in, in + (size_t)b * CIN * H * W, H, W, y0, x0);
+  for (int i = threadIdx.x; i < NTL * 16 * S::KP; i += DN_NT) {
     const int n = i / S::KP, kk = i - n * S::KP;
     s_w[i] = (n < COUT && kk < S::K) ? bf16_bits(w[n * S::K + kk]) : (short)0;
   }
@@ -88,29 +107,34 @@ __global__ void __launch_bounds__(DN_NT) k_conv3x3(const float* __restrict__ in,
   for (int t = wave; t < (DN_TY * DN_TX) / 16; t += DN_NT / 64) {
     const int py = t / (DN_TX / 16), px = (t % (DN_TX / 16)) * 16;
     const int pbase = py * S::LX + px + m;  // window corner of pixel (py, px + m) in the staged tile
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc[NTL];
+#pragma unroll
+    for (int j = 0; j < NTL; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kc = 0; kc < S::KP; kc += 16) {
-      bf16x4 a, bv;
+      bf16x4 a, bv[NTL];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int kk = kc + kq + i;
         a[i] = s_in[pbase + s_off[kk]];
-        bv[i] = s_w[m * S::KP + kk];
+#pragma unroll
+        for (int j = 0; j < NTL; ++j) bv[j][i] = s_w[(j * 16 + m) * S::KP + kk];
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < NTL; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[j], acc[j], 0, 0, 0);
     }
-    const int n = lane & 15;
-    if (n < COUT) {
+#pragma unroll
+    for (int j = 0; j < NTL; ++j) {
+      const int n = j * 16 + (lane & 15);
+      if (n >= COUT) continue;
       const float bn = bias ? bias[n] : 0.0f;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int y = y0 + py, x = x0 + px + kq + i;
         if (y < H && x < W) {
           const size_t o = (((size_t)b * COUT + n) * H + y) * W + x;
-          float v = acc[i] + bn;
-          if (POST == 1) v = fmaxf(v, 0.0f);
-          if (POST == 2) v = mask[o] > 0.0f ? v : 0.0f;
+          float v = acc[j][i] + bn;
+          if (POST == 2) v *= gelu_grad(mask[o]);
           out[o] = v;
         }
       }
@@ -118,21 +142,22 @@ __global__ void __launch_bounds__(DN_NT) k_conv3x3(const float* __restrict__ in,
   }
 }
 
-// dw[n][kk] += sum_{b,pixels} dy[b][n][p] im2col(in)[p][kk]; db[n] += sum dy[b][n][p].
-// Pixels are the K dimension: A = dy^T (16 channels x 16 pixels), B = im2col
-// (16 pixels x 16 taps).  Zero boundary (training images).
-template <int CIN, int COUT>
+// dw[n][kk] += sum_{b,pixels} dy[b][n][p] im2col(in')[p][kk]; db[n] += sum dy[b][n][p]
+// (in' = GELU(in) when GIN).  Pixels are the K dimension: A = dy^T (16
+// channels x 16 pixels), B = im2col (16 pixels x 16 taps).  Zero boundary
+// (training images).
+template <int CIN, int COUT, bool GIN>
 __global__ void __launch_bounds__(DN_NT) k_conv3x3_wgrad(const float* __restrict__ in, const float* __restrict__ dy,
                                                          float* __restrict__ dw, float* __restrict__ db, int H,
                                                          int W) {
   using S = ConvShape<CIN>;
-  constexpr int NP = DN_TY * DN_TX;
+  constexpr int NP = DN_TY * DN_TX, NTL = (COUT + 15) / 16;
   __shared__ short s_in[CIN * S::LY * S::LX];
-  __shared__ short s_dy[16 * NP];
+  __shared__ short s_dy[NTL * 16 * NP];
   __shared__ int s_off[S::KP];
   const int b = blockIdx.z, y0 = blockIdx.y * DN_TY, x0 = blockIdx.x * DN_TX;
-  stage_tile<CIN, false>(s_in, in + (size_t)b * CIN * H * W, H, W, y0, x0);
-  for (int i = threadIdx.x; i < 16 * NP; i += DN_NT) {
+  stage_tile<CIN, false, GIN>(s_in, in + (size_t)b * CIN * H * W, H, W, y0, x0);
+  for (int i = threadIdx.x; i < NTL * 16 * NP; i += DN_NT) {
     const int n = i / NP, p = i - n * NP;
     const int y = y0 + p / DN_TX, x = x0 + p % DN_TX;
     float v = 0.0f;
@@ -143,7 +168,8 @@ __global__ void __launch_bounds__(DN_NT) k_conv3x3_wgrad(const float* __restrict
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = (lane >> 4) * 4;
-  for (int kc = wave * 16; kc < S::KP; kc += 16 * (DN_NT / 64)) {
+  for (int item = wave; item < (S::KP / 16) * NTL; item += DN_NT / 64) {
+    const int kc = (item / NTL) * 16, j = item % NTL;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     const int off = s_off[kc + m];  // this lane's tap (column of B)
     for (int pc = 0; pc < NP; pc += 16) {
@@ -151,21 +177,21 @@ __global__ void __launch_bounds__(DN_NT) k_conv3x3_wgrad(const float* __restrict
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int p = pc + kq + i;  // pixel (k index)
-        a[i] = s_dy[m * NP + p];
+        a[i] = s_dy[(j * 16 + m) * NP + p];
         bv[i] = s_in[(p / DN_TX) * S::LX + (p % DN_TX) + off];
       }
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
     }
-    // D rows = output channel 4 (lane / 16) + i, column = tap kc + lane % 16
+    // D rows = output channel j * 16 + 4 (lane / 16) + i, column = tap kc + lane % 16
     const int kk = kc + m;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int n = kq + i;
+      const int n = j * 16 + kq + i;
       if (n < COUT && kk < S::K) atomicAdd(dw + n * S::K + kk, acc[i]);
     }
   }
-  if (threadIdx.x < COUT * 16) {  // bias: 16 threads per channel, FP32 sums of dy
-    const int n = threadIdx.x / 16, part = threadIdx.x % 16;
+  for (int i = threadIdx.x; i < COUT * 16; i += DN_NT) {  // bias: 16 threads per channel, FP32 sums of dy
+    const int n = i / 16, part = i % 16;
     float s = 0.0f;
     for (int p = part; p < NP; p += 16) {
       const int y = y0 + p / DN_TX, x = x0 + p % DN_TX;
@@ -280,20 +306,17 @@ __global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float den
   }
 }
 
-// ---- fused inference: the three layers of one output tile in LDS ----
-// A workgroup owns FY x FX output pixels.  Layer 1 is evaluated on the tile
-// plus a 2-pixel ring, layer 2 on the tile plus 1, layer 3 on the tile; the
-// activations stay in LDS as BF16 (the precision the next layer's MFMA reads
-// anyway).  Activations of pixels outside the image are zero, as in the
-// layer-by-layer path (columns outside; rows wrap periodically), so both
-// paths compute the same numbers.
-constexpr int FY = 8, FX = 32;
+// ---- fused inference: the five layers of one output tile in LDS ----
+// A workgroup owns FY x FX output pixels.  Layer l (0..4) is evaluated on the
+// tile plus a ring of 4 - l pixels; the activations GELU(z) stay in LDS as
+// BF16 (the precision the next layer's MFMA reads anyway), in two ping-pong
+// buffers, and each layer's weights are staged in turn.  Activations of
+// pixels outside the image are zero, as in the layer-by-layer path (columns
+// outside; rows wrap periodically), so both paths compute the same numbers.
+constexpr int FY = 8, FX = 32, FRING = DN_LAYERS;  // input region: tile + 5 rings
 
-// One conv of the fused chain: output region oh x ow (top-left at offset 0),
-// input region (oh+2) x (ow+2) in s_src (planes of CIN), weights s_w [16][KP].
-// Output pixel (qy, qx) of the region is image pixel (y0 + qy - ring,
-// x0 + qx - ring).
-// s_off[kk]: offset of tap kk in the input region (0 for padded taps).
+// s_off[kk]: offset of tap kk in an input region of ih x iw pixels per plane
+// (0 for padded taps).
 template <int CIN> __device__ __forceinline__ void fused_offsets(int* s_off, int ih, int iw) {
   constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16;
   for (int kk = threadIdx.x; kk < KP; kk += DN_NT) {
@@ -306,11 +329,24 @@ template <int CIN> __device__ __forceinline__ void fused_offsets(int* s_off, int
   }
 }
 
-template <int CIN, int COUT, bool RELU, bool LAST>
+// The weights of one layer, [16 NTL][KP] as BF16, zero-padded.
+template <int CIN, int COUT> __device__ __forceinline__ void fused_weights(short* s_w, const float* __restrict__ w) {
+  constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16, NTL = (COUT + 15) / 16;
+  for (int i = threadIdx.x; i < NTL * 16 * KP; i += DN_NT) {
+    const int n = i / KP, kk = i - n * KP;
+    s_w[i] = (n < COUT && kk < K) ? bf16_bits(w[n * K + kk]) : (short)0;
+  }
+}
+
+// One conv of the fused chain: output region oh x ow, input region
+// (oh+2) x (ow+2) in s_src (planes of CIN).  Output pixel (qy, qx) of the
+// region is image pixel (y0 + qy - ring, x0 + qx - ring).  Not LAST: GELU,
+// zero outside the image's columns, BF16 into s_dst; LAST: r to HBM.
+template <int CIN, int COUT, bool LAST>
 __device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w, const int* s_off,
                                             const float* __restrict__ bias, short* s_dst, float* __restrict__ out,
                                             int oh, int ow, int ring, int y0, int x0, int H, int W) {
-  constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16;
+  constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16, NTL = (COUT + 15) / 16;
   const int iw = ow + 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = (lane >> 4) * 4;
@@ -319,19 +355,25 @@ __device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w
     const int q = min(t * 16 + m, npx - 1);  // this lane's A-row pixel (clamped; its row is discarded)
     const int qy = q / ow, qx = q - qy * ow;
     const int pbase = qy * iw + qx;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc[NTL];
+#pragma unroll
+    for (int j = 0; j < NTL; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int kc = 0; kc < KP; kc += 16) {
-      bf16x4 a, bv;
+      bf16x4 a, bv[NTL];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int kk = kc + kq + i;
         a[i] = s_src[pbase + s_off[kk]];
-        bv[i] = s_w[m * KP + kk];
+#pragma unroll
+        for (int j = 0; j < NTL; ++j) bv[j][i] = s_w[(j * 16 + m) * KP + kk];
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < NTL; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[j], acc[j], 0, 0, 0);
     }
-    const int n = lane & 15;
-    if (n < COUT) {
+#pragma unroll
+    for (int j = 0; j < NTL; ++j) {
+      const int n = j * 16 + (lane & 15);
+      if (n >= COUT) continue;
       const float bn = bias[n];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -339,13 +381,12 @@ __device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w
         if (p >= npx) continue;
         const int py = p / ow, px = p - py * ow;
         const int x = x0 + px - ring;
-        float v = acc[i] + bn;
-        if (RELU) v = fmaxf(v, 0.0f);
+        float v = acc[j][i] + bn;
         if (LAST) {
           const int y = y0 + py - ring;
           if (y < H && x >= 0 && x < W) out[((size_t)n * H + y) * W + x] = v;
         } else {
-          if (x < 0 || x >= W) v = 0.0f;  // outside the image (columns); rows wrap
+          v = (x < 0 || x >= W) ? 0.0f : gelu(v);  // outside the image (columns); rows wrap
           s_dst[n * npx + p] = bf16_bits(v);
         }
       }
@@ -355,90 +396,111 @@ __device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w
 
 __global__ void __launch_bounds__(DN_NT) k_dn_fused(const float* __restrict__ x, const float* __restrict__ params,
                                                     float* __restrict__ r, int H, int W) {
-  constexpr int H0 = FY + 6, W0 = FX + 6;  // input region: tile + 3 rings
-  constexpr int H1 = FY + 4, W1 = FX + 4, H2 = FY + 2, W2 = FX + 2;
-  __shared__ short s_x[4 * H0 * W0];
-  __shared__ short s_a1[16 * H1 * W1];
-  __shared__ short s_a2[16 * H2 * W2];
-  __shared__ short s_w1[16 * 48], s_w2[16 * 144], s_w3[16 * 144];
-  __shared__ int s_off1[48], s_off2[144], s_off3[144];
+  constexpr int H0 = FY + 2 * FRING, W0 = FX + 2 * FRING;
+  // layer l's output region: (FY + 2 (4 - l)) x (FX + 2 (4 - l)); ping-pong
+  // buffers: A holds the outputs of layers 0 and 2, B those of 1 and 3
+  constexpr int NA = (FY + 8) * (FX + 8), NB = (FY + 6) * (FX + 6);
+  __shared__ short s_x[DN_IN * H0 * W0];
+  __shared__ short s_a[DN_CH * NA];
+  __shared__ short s_b[DN_CH * NB];
+  __shared__ short s_w[DN_CH * DN_CH * 9];
+  __shared__ int s_off[DN_CH * 9];
   const DnParams P = dn_params(const_cast<float*>(params));
   const int y0 = blockIdx.y * FY, x0 = blockIdx.x * FX;
-  for (int i = threadIdx.x; i < 4 * H0 * W0; i += DN_NT) {
+  for (int i = threadIdx.x; i < DN_IN * H0 * W0; i += DN_NT) {
     const int c = i / (H0 * W0), rc = i - c * (H0 * W0);
     const int ry = rc / W0, rx = rc - ry * W0;
-    int y = y0 + ry - 3;
-    const int xx = x0 + rx - 3;
+    int y = y0 + ry - FRING;
+    const int xx = x0 + rx - FRING;
     y = ((y % H) + H) % H;
     s_x[i] = bf16_bits(xx >= 0 && xx < W ? x[((size_t)c * H + y) * W + xx] : 0.0f);
   }
-  for (int i = threadIdx.x; i < 16 * 144; i += DN_NT) {
-    const int n = i / 144, kk = i - n * 144;
-    if (kk < 48) s_w1[n * 48 + kk] = kk < 36 ? bf16_bits(P.w1[n * 36 + kk]) : (short)0;
-    s_w2[i] = bf16_bits(P.w2[n * 144 + kk]);
-    s_w3[i] = n < 4 ? bf16_bits(P.w3[n * 144 + kk]) : (short)0;
+  fused_weights<DN_IN, DN_CH>(s_w, P.w[0]);
+  fused_offsets<DN_IN>(s_off, H0, W0);
+  __syncthreads();
+  fused_layer<DN_IN, DN_CH, false>(s_x, s_w, s_off, P.b[0], s_a, nullptr, FY + 8, FX + 8, 4, y0, x0, H, W);
+#pragma unroll
+  for (int l = 1; l < DN_LAYERS - 1; ++l) {
+    const int oh = FY + 2 * (4 - l), ow = FX + 2 * (4 - l);
+    const short* src = (l & 1) ? s_a : s_b;
+    short* dst = (l & 1) ? s_b : s_a;
+    __syncthreads();  // the previous layer is done with s_w
+    fused_weights<DN_CH, DN_CH>(s_w, P.w[l]);
+    fused_offsets<DN_CH>(s_off, oh + 2, ow + 2);
+    __syncthreads();
+    fused_layer<DN_CH, DN_CH, false>(src, s_w, s_off, P.b[l], dst, nullptr, oh, ow, 4 - l, y0, x0, H, W);
   }
-  fused_offsets<4>(s_off1, H0, W0);
-  fused_offsets<16>(s_off2, H1, W1);
-  fused_offsets<16>(s_off3, H2, W2);
   __syncthreads();
-  fused_layer<4, 16, true, false>(s_x, s_w1, s_off1, P.b1, s_a1, nullptr, H1, W1, 2, y0, x0, H, W);
+  fused_weights<DN_CH, DN_IN>(s_w, P.w[DN_LAYERS - 1]);
+  fused_offsets<DN_CH>(s_off, FY + 2, FX + 2);
   __syncthreads();
-  fused_layer<16, 16, true, false>(s_a1, s_w2, s_off2, P.b2, s_a2, nullptr, H2, W2, 1, y0, x0, H, W);
-  __syncthreads();
-  fused_layer<16, 4, false, true>(s_a2, s_w3, s_off3, P.b3, nullptr, r, FY, FX, 0, y0, x0, H, W);
+  fused_layer<DN_CH, DN_IN, true>(s_b, s_w, s_off, P.b[DN_LAYERS - 1], nullptr, r, FY, FX, 0, y0, x0, H, W);
 }
 
 void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipStream_t st) {
   dim3 grid((W + FX - 1) / FX, (H + FY - 1) / FY);
-  hipLaunchKernelGGL(k_dn_fused, grid, dim3(DN_NT), 0, st, T.x, P.w1, T.r, H, W);
+  hipLaunchKernelGGL(k_dn_fused, grid, dim3(DN_NT), 0, st, T.x, P.w[0], T.r, H, W);
 }
 
 // ---- launchers ----
-template <int CIN, int COUT, int POST, bool PY>
+template <int CIN, int COUT, int POST, bool PY, bool GIN>
 static void conv(const float* in, const float* w, const float* bias, const float* mask, float* out, int B, int H,
                  int W, hipStream_t st) {
   dim3 grid((W + DN_TX - 1) / DN_TX, (H + DN_TY - 1) / DN_TY, B);
-  hipLaunchKernelGGL((k_conv3x3<CIN, COUT, POST, PY>), grid, dim3(DN_NT), 0, st, in, w, bias, mask, out, H, W);
+  hipLaunchKernelGGL((k_conv3x3<CIN, COUT, POST, PY, GIN>), grid, dim3(DN_NT), 0, st, in, w, bias, mask, out, H, W);
 }
-template <int CIN, int COUT>
+template <int CIN, int COUT, bool GIN>
 static void wgrad(const float* in, const float* dy, float* dw, float* db, int B, int H, int W, hipStream_t st) {
   dim3 grid((W + DN_TX - 1) / DN_TX, (H + DN_TY - 1) / DN_TY, B);
-  hipLaunchKernelGGL((k_conv3x3_wgrad<CIN, COUT>), grid, dim3(DN_NT), 0, st, in, dy, dw, db, H, W);
+  hipLaunchKernelGGL((k_conv3x3_wgrad<CIN, COUT, GIN>), grid, dim3(DN_NT), 0, st, in, dy, dw, db, H, W);
 }
 static unsigned nblk(int64_t n, int64_t cap = 4096) {
   int64_t b = (n + 255) / 256;
   return (unsigned)(b > cap ? cap : (b < 1 ? 1 : b));
 }
 
-void dn_forward(const DnParams& P, const DnTensors& T, int B, int H, int W, bool periodic_y, hipStream_t st) {
-  if (periodic_y) {
-    conv<4, 16, 1, true>(T.x, P.w1, P.b1, nullptr, T.a1, B, H, W, st);
-    conv<16, 16, 1, true>(T.a1, P.w2, P.b2, nullptr, T.a2, B, H, W, st);
-    conv<16, 4, 0, true>(T.a2, P.w3, P.b3, nullptr, T.r, B, H, W, st);
-  } else {
-    conv<4, 16, 1, false>(T.x, P.w1, P.b1, nullptr, T.a1, B, H, W, st);
-    conv<16, 16, 1, false>(T.a1, P.w2, P.b2, nullptr, T.a2, B, H, W, st);
-    conv<16, 4, 0, false>(T.a2, P.w3, P.b3, nullptr, T.r, B, H, W, st);
-  }
+template <bool PY> static void forward_layers(const DnParams& P, const DnTensors& T, int B, int H, int W,
+                                              hipStream_t st) {
+  conv<DN_IN, DN_CH, 0, PY, false>(T.x, P.w[0], P.b[0], nullptr, T.z[0], B, H, W, st);
+  for (int l = 1; l < DN_LAYERS - 1; ++l)
+    conv<DN_CH, DN_CH, 0, PY, true>(T.z[l - 1], P.w[l], P.b[l], nullptr, T.z[l], B, H, W, st);
+  conv<DN_CH, DN_IN, 0, PY, true>(T.z[DN_LAYERS - 2], P.w[DN_LAYERS - 1], P.b[DN_LAYERS - 1], nullptr, T.r, B, H, W,
+                                  st);
 }
 
-// Loss and gradients of one batch (zero boundary): T.x, T.t set; grad has
-// DN_NTRAIN entries in the parameter layout; loss[0] gets sum (pred - t)^2.
+void dn_forward(const DnParams& P, const DnTensors& T, int B, int H, int W, bool periodic_y, hipStream_t st) {
+  if (periodic_y) forward_layers<true>(P, T, B, H, W, st);
+  else forward_layers<false>(P, T, B, H, W, st);
+}
+
+// Loss and gradients of one batch (zero boundary) after dn_forward: T.x, T.t,
+// T.z, T.r set; G receives the gradients in the parameter layout (DN_NTRAIN
+// entries, accumulated: the caller zeroes them); loss[0] gets the weighted
+// sum of squared errors.
 void dn_backward(const DnParams& P, DnParams G, const DnTensors& T, const float* lw, int B, int H, int W,
                  double* loss, hipStream_t st) {
-  const int64_t n4 = (int64_t)B * 4 * H * W;
+  const int64_t n4 = (int64_t)B * DN_IN * H * W;
   hipLaunchKernelGGL(k_mse, dim3(nblk(n4)), dim3(256), 0, st, T.x, T.r, T.t, lw, T.dr, loss, n4, (int64_t)H * W);
-  // layer 3: r = conv(a2, w3)
-  wgrad<16, 4>(T.a2, T.dr, G.w3, G.b3, B, H, W, st);
-  hipLaunchKernelGGL(k_flip_weights, dim3((16 * 4 * 9 + 255) / 256), dim3(256), 0, st, P.w3, T.wt3, 16, 4);
-  conv<4, 16, 2, false>(T.dr, T.wt3, nullptr, T.a2, T.da2, B, H, W, st);
-  // layer 2: a2 = relu(conv(a1, w2))
-  wgrad<16, 16>(T.a1, T.da2, G.w2, G.b2, B, H, W, st);
-  hipLaunchKernelGGL(k_flip_weights, dim3((16 * 16 * 9 + 255) / 256), dim3(256), 0, st, P.w2, T.wt2, 16, 16);
-  conv<16, 16, 2, false>(T.da2, T.wt2, nullptr, T.a1, T.da1, B, H, W, st);
-  // layer 1: a1 = relu(conv(x, w1))
-  wgrad<4, 16>(T.x, T.da1, G.w1, G.b1, B, H, W, st);
+  constexpr int L = DN_LAYERS;
+  // last layer: r = conv(a3, w4) + b4
+  wgrad<DN_CH, DN_IN, true>(T.z[L - 2], T.dr, G.w[L - 1], G.b[L - 1], B, H, W, st);
+  hipLaunchKernelGGL(k_flip_weights, dim3((DN_CH * DN_IN * 9 + 255) / 256), dim3(256), 0, st, P.w[L - 1], T.wt,
+                     DN_CH, DN_IN);
+  float* dz = T.dza;  // dz of layer L - 2
+  float* dn = T.dzb;
+  conv<DN_IN, DN_CH, 2, false, false>(T.dr, T.wt, nullptr, T.z[L - 2], dz, B, H, W, st);
+  // middle layers l = L-2 .. 1: z_l = conv(a_{l-1}, w_l) + b_l
+  for (int l = L - 2; l >= 1; --l) {
+    wgrad<DN_CH, DN_CH, true>(T.z[l - 1], dz, G.w[l], G.b[l], B, H, W, st);
+    hipLaunchKernelGGL(k_flip_weights, dim3((DN_CH * DN_CH * 9 + 255) / 256), dim3(256), 0, st, P.w[l], T.wt,
+                       DN_CH, DN_CH);
+    conv<DN_CH, DN_CH, 2, false, false>(dz, T.wt, nullptr, T.z[l - 1], dn, B, H, W, st);
+    float* tmp = dz;
+    dz = dn;
+    dn = tmp;
+  }
+  // first layer: z0 = conv(x, w0) + b0
+  wgrad<DN_IN, DN_CH, false>(T.x, dz, G.w[0], G.b[0], B, H, W, st);
 }
 
 void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, int t, hipStream_t st) {
@@ -541,12 +603,10 @@ int sp_dn_create(int batch, int h, int w, sp_dn** out) {
   rc |= dn_alloc(d, &d->T.t, 4 * px);
   rc |= dn_alloc(d, &d->T.r, 4 * px);
   rc |= dn_alloc(d, &d->T.dr, 4 * px);
-  rc |= dn_alloc(d, &d->T.a1, 16 * px);
-  rc |= dn_alloc(d, &d->T.a2, 16 * px);
-  rc |= dn_alloc(d, &d->T.da1, 16 * px);
-  rc |= dn_alloc(d, &d->T.da2, 16 * px);
-  rc |= dn_alloc(d, &d->T.wt2, 16 * 144);
-  rc |= dn_alloc(d, &d->T.wt3, 16 * 36);
+  for (int l = 0; l < sp::DN_LAYERS - 1; ++l) rc |= dn_alloc(d, &d->T.z[l], sp::DN_CH * px);
+  rc |= dn_alloc(d, &d->T.dza, sp::DN_CH * px);
+  rc |= dn_alloc(d, &d->T.dzb, sp::DN_CH * px);
+  rc |= dn_alloc(d, &d->T.wt, sp::DN_CH * sp::DN_CH * 9);
   float* lossf = nullptr;
   rc |= dn_alloc(d, &lossf, 2);
   rc |= dn_alloc(d, &d->lw, 4);

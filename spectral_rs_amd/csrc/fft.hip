@@ -19,12 +19,6 @@
 #ifndef SP_FFT_XCD_REMAP
 #define SP_FFT_XCD_REMAP 1
 #endif
-#ifndef SP_FFT_ROWS_MAX
-#define SP_FFT_ROWS_MAX 1
-#endif
-#ifndef SP_FFT_COLS_MAX
-#define SP_FFT_COLS_MAX 8
-#endif
 
 namespace sp {
 
@@ -277,9 +271,10 @@ template <class T> static int max_elems() { return fft_max_len<T>(); }
 template <class T>
 void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, int load, int store,
                      double store_scale, hipStream_t st) {
-  int rpb = max_elems<T>() / dx.n;
+  // rows per workgroup: the plan's choice (default 1), within LDS and the grid
+  int rpb = dx.per_block > 0 ? dx.per_block : 1;
+  if (rpb > max_elems<T>() / dx.n) rpb = max_elems<T>() / dx.n;
   if (rpb < 1) rpb = 1;
-  if (rpb > SP_FFT_ROWS_MAX) rpb = SP_FFT_ROWS_MAX;
   if (rpb > ny) rpb = ny;
   const int E = rpb * dx.n;
   const int nt = round_threads(E);
@@ -294,9 +289,10 @@ void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, i
 
 template <class T>
 void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, double store_scale, hipStream_t st) {
-  int C = max_elems<T>() / dy.n;
+  // columns per workgroup: the plan's choice (default 8), within LDS and the grid
+  int C = dy.per_block > 0 ? dy.per_block : 8;
+  if (C > max_elems<T>() / dy.n) C = max_elems<T>() / dy.n;
   if (C < 1) C = 1;
-  if (C > SP_FFT_COLS_MAX) C = SP_FFT_COLS_MAX;
   if (C > nx) C = nx;
   const int E = C * dy.n;
   const int nt = round_threads(E);

@@ -216,12 +216,10 @@ template <> struct Acc<double> {
   static constexpr bool fixed = false;
 };
 
-// One workgroup's chunk: NT threads, K particles, deposit windows of at most
-// WMAX cells per plane (planes at fixed strides of WMAX, so a plane offset is
-// an instruction immediate).
-#ifndef SP_FP32_CHUNK
-#define SP_FP32_CHUNK 2048
-#endif
+// One workgroup's chunk: NT threads, K particles (default; the particle pass
+// is also built for 512 and 1024, chosen per context by its tuning), deposit
+// windows of at most WMAX cells per plane (planes at fixed strides of WMAX, so
+// a plane offset is an instruction immediate).
 #ifndef SP_WINDOW_COPIES
 #define SP_WINDOW_COPIES 4
 #endif
@@ -233,7 +231,6 @@ template <> struct Acc<double> {
 #endif
 template <class T> struct ChunkShape {
   static constexpr int NT = SP_PARTICLE_NT;
-  static constexpr int K = sizeof(T) == 4 ? SP_FP32_CHUNK : 1024;
   static constexpr int WMAX = sizeof(T) == 4 ? 2000 : 1000;  // 2 workgroups fit one CU's LDS
   static constexpr int NCOPY = SP_WINDOW_COPIES;  // most private window copies
 };
@@ -442,10 +439,11 @@ __device__ __forceinline__ int wave_or(int v) {
 // particles near the chunk's middle particle.  Particles whose stencil leaves
 // the window (periodic wrap, unsorted input) use global atomics directly, so
 // the result never depends on the particle order; only the speed does.
-template <class T, bool PUSH, bool DIRECT>
+template <class T, bool PUSH, bool DIRECT, int K>
 __global__ void __launch_bounds__(SP_PARTICLE_NT) __attribute__((amdgpu_waves_per_eu(SP_PARTICLE_WAVES_PER_EU)))
 k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
-  constexpr int NT = ChunkShape<T>::NT, K = ChunkShape<T>::K, WMAX = ChunkShape<T>::WMAX, PPT = K / NT;
+  constexpr int NT = ChunkShape<T>::NT, WMAX = ChunkShape<T>::WMAX, PPT = K / NT;
+  static_assert((K & (K - 1)) == 0 && K >= NT, "the chunk is a power of two of at least NT particles");
   constexpr int NCOPY = ChunkShape<T>::NCOPY;
   constexpr int KS = DIRECT ? 1 : K;  // staging arrays (unused when DIRECT)
   using A = typename Acc<T>::type;
@@ -773,9 +771,9 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs
 // whatever the scatter's atomics give.  The key of a particle is a pure
 // function of its cell, so the keys are bit-exact; only the order within a
 // tile is not reproducible.
-__device__ __forceinline__ int tile_of(int32_t c, int ntx, int nty) {
-  const int tx = min(max((cell_ix(c) - 1) >> TILE_W_LOG, 0), ntx - 1);
-  const int ty = min(max((cell_iy(c) - 1) >> TILE_H_LOG, 0), nty - 1);
+__device__ __forceinline__ int tile_of(int32_t c, int ntx, int nty, int twl, int thl) {
+  const int tx = min(max((cell_ix(c) - 1) >> twl, 0), ntx - 1);
+  const int ty = min(max((cell_iy(c) - 1) >> thl, 0), nty - 1);
   return ty * ntx + ((ty & 1) ? ntx - 1 - tx : tx);
 }
 
@@ -855,7 +853,7 @@ __device__ __forceinline__ int gcd_int(int a, int b) {
 // of a wave of the particle pass rarely hit the same deposit addresses.
 template <class T, bool HIST>
 __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty,
-                                                       int* ctr) {
+                                                       int twl, int thl, int* ctr) {
   constexpr int CH = SORT_NT * SORT_PPT;
   constexpr int OUT = HIST ? 1 : CH;
   __shared__ int s_keys[SORT_HT], s_cnt[SORT_HT], s_n[HIST ? 1 : SORT_HT];
@@ -892,7 +890,7 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
     const bool act = i < a.n;
     slot[j] = 0;
     rank[j] = 0;
-    table_add(keys, cnt, act ? tile_of(c[j], ntx, nty) : 0, act, slot[j], rank[j]);
+    table_add(keys, cnt, act ? tile_of(c[j], ntx, nty, twl, thl) : 0, act, slot[j], rank[j]);
   }
   __syncthreads();
   for (int sl = threadIdx.x; sl < SORT_HT; sl += SORT_NT) {
@@ -989,27 +987,42 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const int* __restrict__ coun
 }
 
 template <class T>
-void launch_tile_sort(const SpeciesDev<T>& a, const SpeciesDev<T>& b, int nx, int ny, int* count, int* cursor,
-                      hipStream_t st) {
+void launch_tile_sort(const SpeciesDev<T>& a, const SpeciesDev<T>& b, int nx, int ny, int twl, int thl,
+                      int* count, int* cursor, hipStream_t st) {
   if (a.n == 0) return;
-  const int ntx = (nx + TILE_W - 1) / TILE_W, nty = (ny + TILE_H - 1) / TILE_H;
+  const int tw = 1 << twl, th = 1 << thl;
+  const int ntx = (nx + tw - 1) / tw, nty = (ny + th - 1) / th;
   fill<int>(count, 0, (int64_t)ntx * nty, st);
   const unsigned nb = blocks_for(a.n, SORT_NT * SORT_PPT);
-  hipLaunchKernelGGL((k_tile_pass<T, true>), dim3(nb), dim3(SORT_NT), 0, st, a, b, ntx, nty, count);
+  hipLaunchKernelGGL((k_tile_pass<T, true>), dim3(nb), dim3(SORT_NT), 0, st, a, b, ntx, nty, twl, thl, count);
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, count, cursor, ntx * nty);
-  hipLaunchKernelGGL((k_tile_pass<T, false>), dim3(nb), dim3(SORT_NT), 0, st, a, b, ntx, nty, cursor);
+  hipLaunchKernelGGL((k_tile_pass<T, false>), dim3(nb), dim3(SORT_NT), 0, st, a, b, ntx, nty, twl, thl, cursor);
+}
+
+// The particle pass for the context's chunk size (FP32: 512, 1024 or 2048
+// particles per workgroup; FP64: 1024).
+template <class T, bool PUSH, bool DIRECT>
+static void launch_particles(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
+  const dim3 nt(ChunkShape<T>::NT);
+  if constexpr (sizeof(T) == 4) {
+    if (g.chunk == 512)
+      hipLaunchKernelGGL((k_particles<T, PUSH, DIRECT, 512>), dim3(blocks_for(s.n, 512)), nt, 0, st, s, g, gs, flags);
+    else if (g.chunk == 1024)
+      hipLaunchKernelGGL((k_particles<T, PUSH, DIRECT, 1024>), dim3(blocks_for(s.n, 1024)), nt, 0, st, s, g, gs,
+                         flags);
+    else
+      hipLaunchKernelGGL((k_particles<T, PUSH, DIRECT, 2048>), dim3(blocks_for(s.n, 2048)), nt, 0, st, s, g, gs,
+                         flags);
+  } else {
+    hipLaunchKernelGGL((k_particles<T, PUSH, DIRECT, 1024>), dim3(blocks_for(s.n, 1024)), nt, 0, st, s, g, gs, flags);
+  }
 }
 
 template <class T>
 void launch_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
   if (s.n == 0) return;
-  const int K = ChunkShape<T>::K;
-  if (g.direct)
-    hipLaunchKernelGGL((k_particles<T, false, true>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s,
-                       g, gs, flags);
-  else
-    hipLaunchKernelGGL((k_particles<T, false, false>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s,
-                       g, gs, flags);
+  if (g.direct) launch_particles<T, false, true>(s, g, gs, flags, st);
+  else launch_particles<T, false, false>(s, g, gs, flags, st);
 }
 template <class T>
 void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
@@ -1020,13 +1033,8 @@ template <class T>
 void launch_push_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags,
                               hipStream_t st) {
   if (s.n == 0) return;
-  const int K = ChunkShape<T>::K;
-  if (g.direct)
-    hipLaunchKernelGGL((k_particles<T, true, true>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s,
-                       g, gs, flags);
-  else
-    hipLaunchKernelGGL((k_particles<T, true, false>), dim3(blocks_for(s.n, K)), dim3(ChunkShape<T>::NT), 0, st, s,
-                       g, gs, flags);
+  if (g.direct) launch_particles<T, true, true>(s, g, gs, flags, st);
+  else launch_particles<T, true, false>(s, g, gs, flags, st);
 }
 template <class T>
 void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, double gamma_inj, uint64_t seed,
@@ -1047,7 +1055,8 @@ void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, do
                                             hipStream_t);                                                     \
   template void launch_init_species<T>(const SpeciesDev<T>&, const Geo<T>&, double, double, uint64_t, int,      \
                                        hipStream_t);                                                          \
-  template void launch_tile_sort<T>(const SpeciesDev<T>&, const SpeciesDev<T>&, int, int, int*, int*, hipStream_t);
+  template void launch_tile_sort<T>(const SpeciesDev<T>&, const SpeciesDev<T>&, int, int, int, int, int*, int*,   \
+                                    hipStream_t);
 INST(float)
 INST(double)
 #undef INST

@@ -30,6 +30,7 @@ template <class T> struct Geo {
   int lds_ok;              // dt * c < 1: a particle moves less than one cell per step
   int direct;              // particles are kept tile-sorted and interleaved: the particle
                            // pass reads them in place (no LDS staging or lane permutation)
+  int chunk;               // particles per workgroup of the particle pass (host-side choice)
   T dt, c, csqinv;         // csqinv = 1 / (c * c) in Float (src/prtls.rs:265)
   T half_dt;               // 0.5 * dt in Float (src/prtls.rs:103)
 };
@@ -68,11 +69,11 @@ template <class T> void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, con
 template <class T> void launch_push_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs,
                                                  int* flags, hipStream_t st);
 // Cell-tile sort keeping each particle chunk spatially compact (see
-// k_particles): tiles of TILE_W x TILE_H cells, tile rows in serpentine order.
-constexpr int TILE_W_LOG = 3, TILE_H_LOG = 3;
-constexpr int TILE_W = 1 << TILE_W_LOG, TILE_H = 1 << TILE_H_LOG;
+// k_particles): tiles of 2^twl x 2^thl cells, tile rows in serpentine order.
+// `count` and `cursor` hold one int per tile.
+constexpr int TILE_LOG_MIN = 1, TILE_LOG_MAX = 5;
 template <class T> void launch_tile_sort(const SpeciesDev<T>& src, const SpeciesDev<T>& dst, int nx, int ny,
-                                         int* count, int* cursor, hipStream_t st);
+                                         int twl, int thl, int* count, int* cursor, hipStream_t st);
 template <class T> void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, double gamma_inj,
                                             uint64_t seed, int sid, hipStream_t st);
 template <class T> void launch_fold_ghosts(T* const* grids, int ngrids, int nx, int ny, hipStream_t st);
@@ -106,6 +107,7 @@ template <class T> void launch_hermitian(void* const* spec, int count, int nx, i
 // FFT plan for one dimension: radices and an FP64-generated twiddle table.
 struct FftDim {
   int n;
+  int per_block;        // rows (x) / columns (y) per workgroup; 0 = the default
   int nrad;
   uint64_t radix_code;  // radix of pass p in bits [4p, 4p+4)
   void* tw;             // n complex Float: exp(-2 pi i k / n)
@@ -128,35 +130,50 @@ template <class T> void launch_fft_cols(const FftBatch& b, const FftDim& dy, int
 // Largest transform length one block holds in LDS.
 template <class T> constexpr int fft_max_len() { return (int)(65536 / (2 * sizeof(T))); }
 
-// Learned current denoiser (denoise.hip).  Parameters, one FP32 vector:
-// w1 [16][4*9], b1 [16], w2 [16][16*9], b2 [16], w3 [4][16*9], b3 [4], then
-// the 4 channel scales (not trained).  w[co][ci*9 + ky*3 + kx].
-constexpr int DN_NTRAIN = 16 * 36 + 16 + 16 * 144 + 16 + 4 * 144 + 4;  // 3492
-constexpr int DN_NPARAMS = DN_NTRAIN + 4;
+// Learned current denoiser (denoise.hip): DN_LAYERS 3x3 convolutions,
+// DN_IN -> DN_CH -> ... -> DN_CH -> DN_IN channels, GELU between them.
+// Parameters, one FP32 vector: per layer l, w_l [cout][cin*9] then b_l
+// [cout]; then the DN_IN channel scales (not trained).  w[co][ci*9 + ky*3 + kx].
+constexpr int DN_IN = 4, DN_CH = 32, DN_LAYERS = 5;
+__host__ __device__ constexpr int dn_cin(int l) { return l == 0 ? DN_IN : DN_CH; }
+__host__ __device__ constexpr int dn_cout(int l) { return l == DN_LAYERS - 1 ? DN_IN : DN_CH; }
+constexpr int dn_ntrain() {
+  int n = 0;
+  for (int l = 0; l < DN_LAYERS; ++l) n += dn_cout(l) * dn_cin(l) * 9 + dn_cout(l);
+  return n;
+}
+constexpr int DN_NTRAIN = dn_ntrain();  // 30084
+constexpr int DN_NPARAMS = DN_NTRAIN + DN_IN;
 struct DnParams {
-  float *w1, *b1, *w2, *b2, *w3, *b3, *scale;
+  float* w[DN_LAYERS];
+  float* b[DN_LAYERS];
+  float* scale;
 };
 __host__ __device__ inline DnParams dn_params(float* p) {
   DnParams d;
-  d.w1 = p;
-  d.b1 = d.w1 + 16 * 36;
-  d.w2 = d.b1 + 16;
-  d.b2 = d.w2 + 16 * 144;
-  d.w3 = d.b2 + 16;
-  d.b3 = d.w3 + 4 * 144;
-  d.scale = d.b3 + 4;
+  for (int l = 0; l < DN_LAYERS; ++l) {
+    d.w[l] = p;
+    p += dn_cout(l) * dn_cin(l) * 9;
+    d.b[l] = p;
+    p += dn_cout(l);
+  }
+  d.scale = p;
   return d;
 }
-// Activations of B images of H x W: x, t (targets), r (4 channels); a1, a2,
-// da1, da2 (16); dr (4); wt2, wt3 the flipped backward weights.
+// Activations of B images of H x W: x, t (targets), r, dr (DN_IN channels);
+// z[l] the pre-activations of layers 0..DN_LAYERS-2 and dza, dzb their
+// gradients (DN_CH); wt the flipped weights of one layer (backward).  The
+// simulation's fused path uses x and r only.
 struct DnTensors {
-  float *x, *t, *a1, *a2, *r, *dr, *da2, *da1, *wt2, *wt3;
+  float *x, *t, *r, *dr;
+  float* z[DN_LAYERS - 1];
+  float *dza, *dzb, *wt;
 };
 void dn_forward(const DnParams& P, const DnTensors& T, int B, int H, int W, bool periodic_y, hipStream_t st);
 void dn_backward(const DnParams& P, DnParams G, const DnTensors& T, const float* lw, int B, int H, int W,
                  double* loss, hipStream_t st);
 void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, int t, hipStream_t st);
-// The three layers of one image [4][H][W] in one launch, y periodic (the
+// All layers of one image [4][H][W] in one launch, y periodic (the
 // simulation's inference path); same results as dn_forward(periodic_y).
 void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipStream_t st);
 void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, int nx, int ny, int dens, double* sums,

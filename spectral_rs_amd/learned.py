@@ -1,21 +1,22 @@
 """Learned current denoiser: training data, training and the
 time-to-solution experiment (host side of csrc/denoise.hip).
 
-    python -m spectral_rs_amd.learned [--train-size 512x128] [--eval-size 2048x512] [--steps 1500] ...
+    python -m spectral_rs_amd.learned [--train-size 1024x256] [--eval-size 4096x1024] [--steps 1500] ...
 
-1. Paired data from the method itself: a 64-ppc shock run (the plain method),
-   sampled every `every` steps.  At a sample, every 16th particle (by tag) of
+1. Paired data from the method itself: 64-ppc shock runs (the plain method)
+   with gamma0 drawn uniformly from [2, 20], each sampled every `every` steps.  At a sample, every 16th particle (by tag) of
    each species forms a 4-ppc copy of the same particle state; the copy moves
    and deposits (sp_move_and_deposit, ghost fold, the same binomial filter),
    giving the input, and the 64-ppc run's own filtered currents of that step
    are the target.  Both are divided by their ppc (the solve divides J and rho
    by dens, src/flds/mod.rs:80-84, :127) and by a per-channel scale.
-2. Training on 64 x 64 patches with the BF16 MFMA kernels (Adam).
+2. Training on 64 x 64 patches (about 20k) with the BF16 MFMA kernels (Adam).
 3. Held-out pairs: mean squared error of the binomial-filtered 4-ppc currents
    and of the network's output against the 64-ppc target.
 4. Time to solution at the evaluation size: plain 64 ppc (the truth), plain
    4 ppc and learned 4 ppc, each run for `steps` steps; wall time of the step
-   loop and the error of the y-averaged density profile against the truth.
+   loop, the error of the y-averaged density profile (the shock's density
+   jump) and of the x power spectra of Ex and Bz against the truth.
 """
 import argparse
 import json
@@ -40,9 +41,10 @@ def shock_kw(size_x, size_y, ppc):
     return dict(SHOCK, size_x=size_x, size_y=size_y, delta=delta_for(size_x), dens=ppc)
 
 
-def shock_sim(size_x, size_y, ppc, seed=0, sort_interval=20):
-    sim = capi.GpuSim(prec=4, sort_interval=sort_interval, left_edge=capi.SP_LEFT_REFLECT,
-                      **shock_kw(size_x, size_y, ppc))
+def shock_sim(size_x, size_y, ppc, seed=0, sort_interval=20, gamma=10.0):
+    kw = shock_kw(size_x, size_y, ppc)
+    kw["gamma_inj"] = gamma
+    sim = capi.GpuSim(prec=4, sort_interval=sort_interval, left_edge=capi.SP_LEFT_REFLECT, **kw)
     for q, m, vth in SPECIES:
         sim.init_species(q, m, vth, seed=seed)
     return sim
@@ -53,17 +55,18 @@ def currents(sim, ppc):
     return np.stack([sim.grid(n)[1:-1, 1:-1] for n in CH]).astype(np.float32) / np.float32(ppc)
 
 
-def pairs(size_x, size_y, ppc_hi=64, sub=16, t0=200, every=60, n=12, seed=0, log=sys.stderr):
+def pairs(size_x, size_y, ppc_hi=64, sub=16, t0=200, every=60, n=12, seed=0, gamma=10.0, log=sys.stderr):
     """Input / target current planes [n][4][ny][nx] from one 64-ppc run."""
-    hi = shock_sim(size_x, size_y, ppc_hi, seed)
+    hi = shock_sim(size_x, size_y, ppc_hi, seed, gamma=gamma)
     hi.prestep()
     hi.step(t0)
     ppc_lo = ppc_hi // sub
     X, T = [], []
     for k in range(n):
         hi.sync()
-        lo = capi.GpuSim(prec=4, sort_interval=0, left_edge=capi.SP_LEFT_REFLECT,
-                         **shock_kw(size_x, size_y, ppc_lo))
+        kw = shock_kw(size_x, size_y, ppc_lo)
+        kw["gamma_inj"] = gamma
+        lo = capi.GpuSim(prec=4, sort_interval=0, left_edge=capi.SP_LEFT_REFLECT, **kw)
         for sid, (q, m, _) in enumerate(SPECIES):
             st = hi.species(sid)
             sel = slice(k % sub, None, sub)  # every sub-th particle of the state
@@ -98,25 +101,28 @@ def patches(X, T, rng, per_sample=64, size=PATCH):
     return np.ascontiguousarray(np.stack(xs)), np.ascontiguousarray(np.stack(ts))
 
 
+def layer_shapes():
+    """(cout, cin) of each convolution: 4 -> 32 -> 32 -> 32 -> 32 -> 4."""
+    return [(capi.DN_CH if l < capi.DN_LAYERS - 1 else 4, 4 if l == 0 else capi.DN_CH)
+            for l in range(capi.DN_LAYERS)]
+
+
 def init_params(seed, scales):
     """He-style initial weights, zero biases, small last layer; channel scales."""
     rng = np.random.default_rng(seed)
     parts = []
-    for shape in [(16, 4 * 9), (16,), (16, 16 * 9), (16,), (4, 16 * 9), (4,)]:
-        if len(shape) == 2:
-            w = rng.standard_normal(shape) * np.sqrt(2.0 / shape[1])
-            if shape[0] == 4:
-                w *= 0.1
-            parts.append(w.ravel())
-        else:
-            parts.append(np.zeros(shape))
+    for l, (co, ci) in enumerate(layer_shapes()):
+        w = rng.standard_normal((co, ci * 9)) * np.sqrt(2.0 / (ci * 9))
+        if l == capi.DN_LAYERS - 1:
+            w *= 0.1
+        parts += [w.ravel(), np.zeros(co)]
     parts.append(np.asarray(scales, np.float64))
     p = np.concatenate(parts).astype(np.float32)
     assert p.size == capi.DN_NPARAMS
     return p
 
 
-def train(X, T, iters=2000, lr=2e-3, batch=16, seed=0, log=sys.stderr):
+def train(X, T, iters=2000, lr=2e-3, batch=16, seed=0, per_sample=64, log=sys.stderr):
     """Trains on patches of the pairs; returns (params, loss history, scales).
     Each channel is scaled by the RMS of the input (values of order 1 in
     BF16), and its loss is weighted by 1 / (its scaled 4-ppc noise power), so
@@ -127,7 +133,7 @@ def train(X, T, iters=2000, lr=2e-3, batch=16, seed=0, log=sys.stderr):
     Xn = X / scales[None, :, None, None].astype(np.float32)
     Tn = T / scales[None, :, None, None].astype(np.float32)
     rng = np.random.default_rng(seed)
-    xs, ts = patches(Xn, Tn, rng)
+    xs, ts = patches(Xn, Tn, rng, per_sample=per_sample)
     dn = capi.Denoiser(batch, PATCH, PATCH)
     dn.set_params(init_params(seed, scales))
     dn.set_loss_weights((1.0 / (4.0 * noise)).astype(np.float32))
@@ -162,6 +168,12 @@ def profile(sim, ppc):
     return sim.grid("dens")[1:-1, 1:-1].astype(np.float64).mean(axis=0) / ppc
 
 
+def xspectrum(sim, name):
+    """Power of the field along x, averaged over rows: |FFT_x|^2 for kx = 0..Nx/2."""
+    f = sim.grid(name)[1:-1, 1:-1].astype(np.float64)
+    return np.mean(np.abs(np.fft.rfft(f, axis=1)) ** 2, axis=0)
+
+
 def run_case(size_x, size_y, ppc, steps, params=None, seed=0, max_rel=1.0):
     sim = shock_sim(size_x, size_y, ppc, seed)
     if params is not None:
@@ -176,9 +188,10 @@ def run_case(size_x, size_y, ppc, steps, params=None, seed=0, max_rel=1.0):
     sim.sync()
     el = time.perf_counter() - t0
     prof = profile(sim, ppc)
+    spec = {n: xspectrum(sim, n) for n in ("ex", "bz")}
     stats = sim.denoiser_stats() if params is not None else None
     sim.close()
-    return el, prof, stats
+    return el, prof, stats, spec
 
 
 def profile_error(prof, truth, delta):
@@ -186,19 +199,28 @@ def profile_error(prof, truth, delta):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def spectrum_error(spec, truth):
+    """Relative L2 error of a power spectrum over kx = 1 .. Nx/4 (the modes the
+    binomial filter leaves)."""
+    k = slice(1, (truth.size - 1) // 2 + 1)
+    return float(np.linalg.norm(spec[k] - truth[k]) / np.linalg.norm(truth[k]))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train-size", default="512x128")
-    ap.add_argument("--eval-size", default="2048x512")
+    ap.add_argument("--train-size", default="1024x256")
+    ap.add_argument("--eval-size", default="4096x1024")
     ap.add_argument("--steps", type=int, default=1500)
-    ap.add_argument("--samples", type=int, default=16, help="pairs per training run")
-    ap.add_argument("--seeds", default="0,1", help="seeds of the training runs")
-    ap.add_argument("--heldout-seed", type=int, default=5)
+    ap.add_argument("--train-runs", type=int, default=6, help="training runs, gamma0 uniform in [2, 20]")
+    ap.add_argument("--train-seed", type=int, default=0, help="seed of the first training run and of the gamma0 draw")
+    ap.add_argument("--samples", type=int, default=12, help="pairs per training run")
+    ap.add_argument("--patches", type=int, default=280, help="64x64 patches per pair")
+    ap.add_argument("--heldout-seed", type=int, default=100)
     ap.add_argument("--t0", type=int, default=100)
-    ap.add_argument("--every", type=int, default=50)
-    ap.add_argument("--iters", type=int, default=2000)
-    ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--eval-seeds", default="0,1,2", help="runs per case; profiles are averaged over them")
+    ap.add_argument("--every", type=int, default=75)
+    ap.add_argument("--iters", type=int, default=6000)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--eval-seeds", default="0,1", help="runs per case; profiles are averaged over them")
     ap.add_argument("--out", default=None, help="JSON report")
     ap.add_argument("--params-out", default=None, help=".npy of the trained parameters")
     ap.add_argument("--params-in", default=None, help="skip training: evaluate these parameters (.npy)")
@@ -209,35 +231,45 @@ def main():
         p = np.load(a.params_in)
         rep = {"params": a.params_in}
     else:
-        seeds = [int(v) for v in a.seeds.split(",")]
-        XT = [pairs(tx, ty, n=a.samples, t0=a.t0, every=a.every, seed=sd) for sd in seeds]
+        grng = np.random.default_rng(a.train_seed)
+        gammas = [float(g) for g in grng.uniform(2.0, 20.0, a.train_runs)]
+        seeds = [a.train_seed + k for k in range(a.train_runs)]
+        XT = [pairs(tx, ty, n=a.samples, t0=a.t0, every=a.every, seed=sd, gamma=g) for sd, g in zip(seeds, gammas)]
         X, T = np.concatenate([x for x, _ in XT]), np.concatenate([t for _, t in XT])
-        p, hist, scales = train(X, T, iters=a.iters, lr=a.lr)
-        # held out: a run with another seed, sampled over the same times
-        Xh, Th = pairs(tx, ty, n=max(4, a.samples // 4), t0=a.t0, every=a.every * 4, seed=a.heldout_seed)
+        del XT
+        p, hist, scales = train(X, T, iters=a.iters, lr=a.lr, per_sample=a.patches)
+        npatch = X.shape[0] * a.patches
+        del X, T
+        # held out: another seed and gamma0, sampled over the same times
+        gh = float(grng.uniform(2.0, 20.0))
+        Xh, Th = pairs(tx, ty, n=max(4, a.samples // 2), t0=a.t0, every=a.every * 2, seed=a.heldout_seed, gamma=gh)
         ev = evaluate(p, Xh, Th)
         print("held-out:", json.dumps(ev), file=sys.stderr, flush=True)
-        rep = {"train_size": a.train_size, "seeds": seeds, "samples_per_run": a.samples, "t0": a.t0,
-               "every": a.every, "iters": a.iters, "loss_first": float(hist[:20].mean()),
-               "loss_last": float(hist[-50:].mean()), "heldout_seed": a.heldout_seed, "heldout": ev}
+        rep = {"train_size": a.train_size, "seeds": seeds, "gammas": gammas, "samples_per_run": a.samples,
+               "patches": npatch, "t0": a.t0, "every": a.every, "iters": a.iters, "lr": a.lr,
+               "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean()),
+               "heldout_seed": a.heldout_seed, "heldout_gamma": gh, "heldout": ev}
     d = delta_for(ex)
     eseeds = [int(v) for v in a.eval_seeds.split(",")]
     cases = {"plain_64ppc": (64, None), "plain_4ppc": (4, None), "learned_4ppc": (4, p)}
     res = {}
     for name, (ppc, prm) in cases.items():
         runs = [run_case(ex, ey, ppc, a.steps, params=prm, seed=sd) for sd in eseeds]
-        res[name] = (float(np.mean([r[0] for r in runs])), np.mean([r[1] for r in runs], axis=0),
-                     [r[2] for r in runs])
-        print(f"time_to_solution: {name} {res[name][0]:.3f} s per run", file=sys.stderr, flush=True)
-    truth = res["plain_64ppc"][1]
-    rep["time_to_solution"] = {
-        "eval_size": a.eval_size, "steps": a.steps, "seeds": eseeds,
-        "plain_64ppc_s": res["plain_64ppc"][0], "plain_4ppc_s": res["plain_4ppc"][0],
-        "learned_4ppc_s": res["learned_4ppc"][0],
-        "speedup_learned_vs_plain64": res["plain_64ppc"][0] / res["learned_4ppc"][0],
-        "profile_rel_err_plain_4ppc": profile_error(res["plain_4ppc"][1], truth, d),
-        "profile_rel_err_learned_4ppc": profile_error(res["learned_4ppc"][1], truth, d),
-        "denoiser_applied_fallbacks": res["learned_4ppc"][2]}
+        res[name] = dict(t=float(np.mean([r[0] for r in runs])), prof=np.mean([r[1] for r in runs], axis=0),
+                         stats=[r[2] for r in runs],
+                         spec={n: np.mean([r[3][n] for r in runs], axis=0) for n in ("ex", "bz")})
+        print(f"time_to_solution: {name} {res[name]['t']:.3f} s per run", file=sys.stderr, flush=True)
+    truth = res["plain_64ppc"]
+    tts = {"eval_size": a.eval_size, "steps": a.steps, "seeds": eseeds,
+           "plain_64ppc_s": truth["t"], "plain_4ppc_s": res["plain_4ppc"]["t"],
+           "learned_4ppc_s": res["learned_4ppc"]["t"],
+           "speedup_learned_vs_plain64": truth["t"] / res["learned_4ppc"]["t"],
+           "denoiser_applied_fallbacks": res["learned_4ppc"]["stats"]}
+    for case in ("plain_4ppc", "learned_4ppc"):
+        tts[f"profile_rel_err_{case}"] = profile_error(res[case]["prof"], truth["prof"], d)
+        for n in ("ex", "bz"):
+            tts[f"spectrum_rel_err_{n}_{case}"] = spectrum_error(res[case]["spec"][n], truth["spec"][n])
+    rep["time_to_solution"] = tts
     print(json.dumps(rep), flush=True)
     if a.out:
         with open(a.out, "w") as f:

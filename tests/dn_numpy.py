@@ -2,21 +2,39 @@
 for the HIP kernels of spectral_rs_amd/csrc/denoise.hip; the reference code
 has no such network).
 
-Network: x (4 planes) -> conv3x3(4->16)+b, ReLU -> conv3x3(16->16)+b, ReLU ->
-conv3x3(16->4)+b = r, output x + r; "same" convolutions, zero outside the
-image.  Parameter vector layout: w1 [16][4*9], b1 [16], w2 [16][16*9], b2 [16],
-w3 [4][16*9], b3 [4], scales [4]; w[co][ci*9 + ky*3 + kx].
+Network: x (4 planes) -> z0 = conv3x3(4->32) + b0 -> a0 = GELU(z0) ->
+z_l = conv3x3(32->32)(a_{l-1}) + b_l, a_l = GELU(z_l) (l = 1..3) ->
+r = conv3x3(32->4)(a3) + b4, output x + r; "same" convolutions, zero
+outside the image; GELU(z) = z (1 + erf(z / sqrt 2)) / 2.  Parameter vector
+layout: per layer w_l [cout][cin*9] then b_l [cout]; then scales [4];
+w[co][ci*9 + ky*3 + kx].
 
 With emulate=True every convolution operand (input planes, weights, and in
 the backward pass the incoming gradient) is rounded to BF16 (round to nearest
-even) as the MFMA kernels do; sums are FP64.  With emulate=False it is plain
-FP64, which the finite-difference test pins.
+even) as the MFMA kernels do, and pre-activations pass through FP32 where the
+kernels store them; sums are FP64.  With emulate=False it is plain FP64,
+which the finite-difference test pins.
 """
 import numpy as np
+from scipy.special import erf
 
-NTRAIN, NPARAMS = 3492, 3496
-SHAPES = [("w1", (16, 4, 3, 3)), ("b1", (16,)), ("w2", (16, 16, 3, 3)), ("b2", (16,)), ("w3", (4, 16, 3, 3)),
-          ("b3", (4,)), ("scale", (4,))]
+IN, CH, LAYERS = 4, 32, 5
+
+
+def cin(l):
+    return IN if l == 0 else CH
+
+
+def cout(l):
+    return IN if l == LAYERS - 1 else CH
+
+
+SHAPES = []
+for _l in range(LAYERS):
+    SHAPES += [(f"w{_l}", (cout(_l), cin(_l), 3, 3)), (f"b{_l}", (cout(_l),))]
+SHAPES.append(("scale", (IN,)))
+NTRAIN = int(sum(np.prod(s) for _, s in SHAPES[:-1]))
+NPARAMS = NTRAIN + IN
 
 
 def unpack(p):
@@ -36,6 +54,14 @@ def bf16(a):
     u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
     r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def gelu(z):
+    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+
+
+def gelu_grad(z):
+    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
 
 
 def conv(x, w, b=None, periodic_y=False):
@@ -74,38 +100,51 @@ def wgrad(x, d):
     return g
 
 
+def _ops(emulate):
+    if emulate:
+        return bf16, lambda z: gelu(np.asarray(z, np.float32).astype(np.float64))
+    return (lambda a: np.asarray(a, np.float64)), gelu
+
+
 def forward(p, x, emulate=True, periodic_y=False):
-    R = bf16 if emulate else (lambda a: np.asarray(a, np.float64))
+    """Returns x + r and (pre-activations z_0..z_3, r)."""
+    R, act = _ops(emulate)
     q = unpack(p)
-    a1 = np.maximum(conv(R(x), R(q["w1"]), q["b1"], periodic_y), 0)
-    a2 = np.maximum(conv(R(a1), R(q["w2"]), q["b2"], periodic_y), 0)
-    r = conv(R(a2), R(q["w3"]), q["b3"], periodic_y)
-    return np.asarray(x, np.float64) + r, (a1, a2, r)
+    zs = []
+    a = np.asarray(x, np.float64)
+    for l in range(LAYERS):
+        z = conv(R(a), R(q[f"w{l}"]), q[f"b{l}"], periodic_y)
+        if l < LAYERS - 1:
+            zs.append(z)
+            a = act(z)
+        else:
+            r = z
+    return np.asarray(x, np.float64) + r, (zs, r)
 
 
 def loss_grad(p, x, t, emulate=True):
     """Mean squared error of forward(x) against t and its gradient (NTRAIN)."""
-    R = bf16 if emulate else (lambda a: np.asarray(a, np.float64))
+    R, act = _ops(emulate)
     q = unpack(p)
-    y, (a1, a2, r) = forward(p, x, emulate)
+    y, (zs, r) = forward(p, x, emulate)
     n = y.size
     d = y - t
     loss = float((d * d).sum() / n)
-    dr = 2.0 * d / n
     g = {}
-    g["w3"] = wgrad(R(a2), R(dr))
-    g["b3"] = dr.sum(axis=(0, 2, 3))
-    da2 = conv_t(R(dr), R(q["w3"])) * (a2 > 0)
-    g["w2"] = wgrad(R(a1), R(da2))
-    g["b2"] = da2.sum(axis=(0, 2, 3))
-    da1 = conv_t(R(da2), R(q["w2"])) * (a1 > 0)
-    g["w1"] = wgrad(R(x), R(da1))
-    g["b1"] = da1.sum(axis=(0, 2, 3))
+    L = LAYERS
+    dz = 2.0 * d / n  # gradient of the last layer's output r
+    for l in range(L - 1, -1, -1):
+        a_in = np.asarray(x, np.float64) if l == 0 else act(zs[l - 1])
+        g[f"w{l}"] = wgrad(R(a_in), R(dz))
+        g[f"b{l}"] = dz.sum(axis=(0, 2, 3))
+        if l > 0:
+            dz = conv_t(R(dz), R(q[f"w{l}"])) * gelu_grad(zs[l - 1])
     return loss, pack_grad(g)
 
 
 def init_params(seed, scale=1.0):
-    """He-style random initial parameters with zero biases; channel scales set to `scale`."""
+    """He-style random initial parameters with zero biases, a small last
+    layer; channel scales set to `scale`."""
     rng = np.random.default_rng(seed)
     q = {}
     for name, shp in SHAPES:
@@ -114,6 +153,6 @@ def init_params(seed, scale=1.0):
             q[name] = rng.standard_normal(shp) * np.sqrt(2.0 / fan_in)
         else:
             q[name] = np.zeros(shp)
-    q["w3"] *= 0.1
+    q[f"w{LAYERS - 1}"] *= 0.1
     q["scale"][:] = scale
     return np.concatenate([q[name].ravel() for name, _ in SHAPES]).astype(np.float32)

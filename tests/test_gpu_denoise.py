@@ -34,7 +34,7 @@ def test_forward_matches_numpy(shape):
     dn = capi.Denoiser(B, H, W)
     dn.set_params(p)
     y = dn.forward(x)
-    yr, (_, _, r) = D.forward(p, x, emulate=True)
+    yr, (_, r) = D.forward(p, x, emulate=True)
     # BF16 operands, FP32 sums: the correction r agrees to FP32 accumulation and
     # the occasional one-ulp BF16 rounding difference of an FP32 activation
     assert rel(y - x, r) < 3e-3
@@ -43,7 +43,7 @@ def test_forward_matches_numpy(shape):
 
 
 def test_periodic_and_fused_forward():
-    # the simulation's path: rows periodic, all three layers in one launch; it
+    # the simulation's path: rows periodic, all five layers in one launch; it
     # keeps the same BF16 rounding points and sum order as the layer-by-layer
     # kernels, so the two agree to the last bit
     B, H, W = 2, 21, 70
@@ -55,7 +55,7 @@ def test_periodic_and_fused_forward():
     dn.set_params(p)
     y1 = dn.forward(x, mode=1)
     y2 = dn.forward(x, mode=2)
-    _, (_, _, r) = D.forward(p, x, emulate=True, periodic_y=True)
+    _, (_, r) = D.forward(p, x, emulate=True, periodic_y=True)
     assert rel(y1 - x, r) < 3e-3
     assert np.array_equal(y1, y2)
     dn.close()
@@ -135,7 +135,7 @@ def test_in_loop_identity_network_and_fallback():
     # a network whose correction is huge trips the |r| <= max_rel |x| check:
     # every solve falls back to the binomial filter, i.e. the plain run
     pb = p0.copy()
-    pb[D.NTRAIN - 4:D.NTRAIN] = 1e3  # b3
+    pb[D.NTRAIN - 4:D.NTRAIN] = 1e3  # bias of the last layer
     fb = shock(pb, max_rel=0.5)
     assert fb.denoiser_stats() == (0, 10)
     for name, f in floor.items():
