@@ -88,8 +88,9 @@ def ensemble_runs(count=64, seed=0):
 
 def bench_ensemble(args):
     """64 independent runs of uneven size stepped together: one hipGraph per
-    2 steps of every member, 4 concurrent branches, tunings from the
-    autotuner.  Also times the same ensemble eagerly and with default tunings."""
+    `unit` steps of every member in `branches` concurrent branches, tunings
+    from the autotuner.  Also times the same ensemble eagerly and with default
+    tunings."""
     from spectral_rs_amd import capi
 
     steps = args.steps if args.steps_given else 1000
@@ -115,7 +116,7 @@ def bench_ensemble(args):
     print(f"bench: autotuned {len(shapes)} distinct configurations in {t_tune:.1f} s", file=sys.stderr, flush=True)
     for s in sims:
         s.prestep()
-    ens = capi.Ensemble(sims, branches=4, unit=2)
+    ens = capi.Ensemble(sims, branches=args.branches, unit=args.unit)
     ens.step(max(args.warmup, 2))
     ens.sync()
 
@@ -134,7 +135,7 @@ def bench_ensemble(args):
     ens.close()
     for s in sims:
         s.set_tuning(chunk=2048, tile_w_log=3, tile_h_log=3, fft_rows=1, fft_cols=8, fft_radix_x=8, fft_radix_y=8)
-    ens = capi.Ensemble(sims, branches=4, unit=2)
+    ens = capi.Ensemble(sims, branches=args.branches, unit=args.unit)
     ens.step(2)
     el_default = timed(cmp_steps)
     st = ens.stats()
@@ -161,7 +162,8 @@ def bench_ensemble(args):
         "data": "synthetic (GPU-generated shock initial states, seed 0)",
         "config": {"workload": "ensemble of 64 independent FP32 shock runs, grids {128..1024}x{32..256}, "
                                "ppc {4..32}, gamma0 in [2,20], T log-uniform in [1e-5,1e-3], 4 binomial passes; "
-                               "one hipGraph per 2 steps of all runs, 4 branches, autotuned launch shapes",
+                               f"one hipGraph per {args.unit} steps of all runs, {args.branches} branches, "
+                               "autotuned launch shapes",
                    "runs": len(runs), "particles": n, "cells": cells},
         "modes_ms_per_step": {"graph_tuned": ms, "eager_tuned": el_eager * 1e3 / cmp_steps,
                               "graph_default_tuning": el_default * 1e3 / cmp_steps},
@@ -186,6 +188,8 @@ def main():
     ap.add_argument("--sort-interval", type=int, default=20, help="steps between tile sorts of the particles")
     ap.add_argument("--trace", type=int, default=0, help="print phase times every N steps (diagnostic)")
     ap.add_argument("--autotune", action="store_true", help="shock workload: autotune the launch shapes first")
+    ap.add_argument("--branches", type=int, default=4, help="ensemble workload: concurrent branches")
+    ap.add_argument("--unit", type=int, default=2, help="ensemble workload: steps per captured graph")
     ap.add_argument("--workload", choices=["shock", "ensemble"], default="shock",
                     help="shock: the 4096x1024 run (the metric); ensemble: 64 uneven runs stepped together")
     args = ap.parse_args()

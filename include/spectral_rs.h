@@ -71,8 +71,18 @@ typedef struct sp_params {
   int32_t checked;        /* 1 = the asserts of the default build are checked */
   int32_t left_edge;      /* enum sp_left_edge */
   int32_t sort_interval;  /* steps between cell sorts of the particles, 0 = never */
-  int32_t reserved[3];
+  int32_t study;          /* SP_F32 precision-study variants (enum sp_study bits), 0 = none */
+  int32_t reserved[2];
 } sp_params;
+
+/* Precision-study variants of the SP_F32 path (no reference counterpart),
+ * for comparing it with an all-FP32 code: SP_STUDY_ABS_POS32 keeps each
+ * position to the resolution of an absolute FP32 coordinate (after every
+ * position update the offset becomes fl(fl(ix + dx) - ix), the rounding an
+ * x stored as one float would get); SP_STUDY_TWIDDLE32 generates the FFT
+ * twiddles in FP32 instead of FP64; SP_STUDY_SUM32 accumulates the
+ * diagnostics' energy and momentum sums in FP32. */
+enum sp_study { SP_STUDY_ABS_POS32 = 1, SP_STUDY_TWIDDLE32 = 2, SP_STUDY_SUM32 = 4 };
 
 /* FP64 diagnostics; none has a reference counterpart.  Energies share one
  * unit: with J and rho divided by dens in the solve (src/flds/mod.rs:80-84,
@@ -193,8 +203,8 @@ typedef struct sp_tuning {
   int32_t fft_radix_y;
   int32_t reserved;
 } sp_tuning;
-/* Defaults: chunk 2048 (SP_F64: 1024), 8x8 tiles, 1 row and 8 columns per
- * workgroup, radix 8. */
+/* Defaults: chunk 2048 and 8x4 tiles (SP_F64: 1024 and 8x8), 1 row and 8
+ * columns per workgroup, radix 8. */
 int sp_set_tuning(sp_ctx* ctx, const sp_tuning* t);
 int sp_get_tuning(const sp_ctx* ctx, sp_tuning* t);
 /* Times the candidates of every knob on a scratch copy of the context (same
@@ -267,6 +277,11 @@ int sp_dn_train(sp_dn* dn, const float* xs, const float* ts, int nsamples, int i
  * |r| > max_rel |x| or r is not finite, or when the previous solve's relative
  * pre-correction Gauss residual exceeds max_gauss.  params NULL disables. */
 int sp_set_denoiser(sp_ctx* ctx, const float* params, int n, float max_rel, float max_gauss);
+/* Channels the network's output is written to when it is applied: bit c
+ * for Jx, Jy, Jz, rho (default 0xF).  With rho left out, the Gauss-law
+ * projection of the solve (src/flds/mod.rs:185-201) keeps the longitudinal
+ * field tied to the deposited charge. */
+int sp_set_denoiser_channels(sp_ctx* ctx, int mask);
 /* Solves that applied the network / fell back since the denoiser was set. */
 int sp_get_denoiser_stats(sp_ctx* ctx, int* applied, int* fallbacks);
 

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libspectral_rs_hip.so")
 SP_OK, SP_EINVAL, SP_EHIP, SP_EINVARIANT, SP_ENOMEM = 0, -1, -2, -3, -4
 SP_F32, SP_F64 = 0, 1
 SP_LEFT_ASSERT, SP_LEFT_REFLECT = 0, 1
+SP_STUDY_ABS_POS32, SP_STUDY_TWIDDLE32, SP_STUDY_SUM32 = 1, 2, 4
 GRIDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5, "jx": 6, "jy": 7, "jz": 8,
          "rho": 9, "dsty": 9, "dens": 10, "bxw": 11, "byw": 12, "bzw": 13}
 
@@ -29,7 +30,7 @@ SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_spe
            "sp_ensemble_destroy", "sp_ensemble_last_error", "sp_ensemble_set_graph", "sp_ensemble_step",
            "sp_ensemble_sync", "sp_ensemble_stats", "sp_dn_create", "sp_dn_destroy", "sp_dn_last_error", "sp_dn_set_params",
            "sp_dn_get_params", "sp_dn_set_loss_weights", "sp_dn_forward", "sp_dn_gradient", "sp_dn_train",
-           "sp_set_denoiser",
+           "sp_set_denoiser", "sp_set_denoiser_channels",
            "sp_get_denoiser_stats"]
 DN_NTRAIN, DN_NPARAMS = 30084, 30088
 DN_LAYERS, DN_CH = 5, 32  # the denoiser: 5 convolutions, 32 hidden channels
@@ -40,7 +41,7 @@ class SpParams(ct.Structure):
                 ("dens", ct.c_int32), ("n_pass", ct.c_int32), ("dt", ct.c_double), ("c", ct.c_double),
                 ("gamma_inj", ct.c_double), ("precision", ct.c_int32), ("periodic_x", ct.c_int32),
                 ("checked", ct.c_int32), ("left_edge", ct.c_int32), ("sort_interval", ct.c_int32),
-                ("reserved", ct.c_int32 * 3)]
+                ("study", ct.c_int32), ("reserved", ct.c_int32 * 2)]
 
 
 class SpDiag(ct.Structure):
@@ -107,6 +108,7 @@ def load(path=LIB_PATH):
     L.sp_dn_train.argtypes = [vp, vp, vp, i32, i32, ct.c_float, ct.c_uint64, vp]
     L.sp_set_denoiser.argtypes = [vp, vp, i32, ct.c_float, ct.c_float]
     L.sp_get_denoiser_stats.argtypes = [vp, ct.POINTER(i32), ct.POINTER(i32)]
+    L.sp_set_denoiser_channels.argtypes = [vp, i32]
     L.sp_set_tuning.argtypes = [vp, ct.POINTER(SpTuning)]
     L.sp_get_tuning.argtypes = [vp, ct.POINTER(SpTuning)]
     L.sp_autotune.argtypes = [vp, i32, i32, ct.POINTER(SpTuning)]
@@ -133,7 +135,7 @@ class GpuSim:
     """One simulation context on the current GPU (Sim + Flds + Vec<Prtl>)."""
 
     def __init__(self, size_x, size_y, delta, dens, n_pass=4, dt=0.1, c=3.0, gamma_inj=5.0,
-                 prec=4, periodic_x=False, checked=True, left_edge=SP_LEFT_ASSERT, sort_interval=0):
+                 prec=4, periodic_x=False, checked=True, left_edge=SP_LEFT_ASSERT, sort_interval=0, study=0):
         L = load()
         self.nx, self.ny = size_x, size_y
         self.prec = prec
@@ -141,7 +143,7 @@ class GpuSim:
         self.ct = np.complex128 if prec == 8 else np.complex64
         p = SpParams(size_x, size_y, delta, dens, n_pass, dt, c, gamma_inj,
                      SP_F64 if prec == 8 else SP_F32, int(periodic_x), int(checked), int(left_edge),
-                     int(sort_interval))
+                     int(sort_interval), int(study))
         h = ct.c_void_p()
         rc = L.sp_create(ct.byref(p), ct.byref(h))
         if rc != SP_OK:
@@ -255,6 +257,10 @@ class GpuSim:
             return
         p = np.ascontiguousarray(params, np.float32)
         self._chk(load().sp_set_denoiser(self.h, _p(p), p.size, max_rel, max_gauss))
+
+    def set_denoiser_channels(self, mask):
+        """Bit c: write the network's output to Jx, Jy, Jz, rho (c = 0..3)."""
+        self._chk(load().sp_set_denoiser_channels(self.h, int(mask)))
 
     def denoiser_stats(self):
         a, f = ct.c_int32(), ct.c_int32()

@@ -55,7 +55,7 @@ struct sp_ctx {
   hipStream_t own_st = nullptr;  // created by sp_create; st differs from it inside an ensemble
   sp_ensemble* ens = nullptr;    // the ensemble this context belongs to, if any
   sp_tuning tune{};              // launch shapes (sp_set_tuning / sp_autotune)
-  int64_t tune_gen = 0;          // bumped on every tuning change (graph signatures)
+  int64_t tune_gen = 0;          // bumped on every change of launch arguments (graph signatures)
   int device = 0;
   void* grid[11] = {};
   void* work[4] = {};
@@ -72,6 +72,7 @@ struct sp_ctx {
   double* dn_sums = nullptr;
   int* dn_stats = nullptr;
   float dn_tol = 0, dn_gtol = 0;
+  int dn_mask = 0xF;  // channels the correction is written to (bit c: Jx Jy Jz rho)
   // step-loop graph (sp_set_graph): the captured period and the buffers it uses
   bool graph_on = false;
   hipGraphExec_t gexec = nullptr;
@@ -96,7 +97,7 @@ sp_tuning default_tuning(bool f64) {
   sp_tuning t{};
   t.chunk = f64 ? 1024 : 2048;
   t.tile_w_log = 3;
-  t.tile_h_log = 3;
+  t.tile_h_log = f64 ? 3 : 2;  // FP32: 8x4 tiles (the autotuner's pick for the 4096x1024 shock)
   t.fft_rows = 1;
   t.fft_cols = 8;
   t.fft_radix_x = 8;
@@ -131,6 +132,7 @@ template <class T> Geo<T> geo(const sp_ctx* c) {
   g.lds_ok = c->p.dt * c->p.c < 1.0;
   g.direct = c->p.sort_interval > 0;
   g.chunk = c->tune.chunk;
+  g.abs_pos32 = (c->p.study & SP_STUDY_ABS_POS32) != 0;
   g.dt = (T)c->p.dt;
   g.c = (T)c->p.c;
   g.csqinv = T(1.) / (g.c * g.c);
@@ -255,7 +257,7 @@ template <class T> void fields_update(sp_ctx* c) {
     if (c->dn_on) {  // learned correction of the filtered currents, or the fallback
       float* g4[4] = {(float*)c->grid[SP_JX], (float*)c->grid[SP_JY], (float*)c->grid[SP_JZ], (float*)c->grid[SP_RHO]};
       dn_apply_sim(dn_params(c->dn_buf), dn_tensors(c), g4, nx, ny, c->p.dens, c->dn_sums, c->gauss_d, c->dn_tol,
-                   c->dn_gtol, c->dn_stats, c->st);
+                   c->dn_gtol, c->dn_mask, c->dn_stats, c->st);
       launch_update_ghosts<float>(g4, 4, nx, ny, c->st);
     }
   }
@@ -296,13 +298,14 @@ template <class T> void diag_sums(sp_ctx* c) {
   const int nd = DIAG_FIXED + 4 * (int)c->species.size();
   fill<double>(c->diag_d, 0.0, nd, c->st);
   const int nx = c->p.size_x, ny = c->p.size_y;
+  const bool s32 = (c->p.study & SP_STUDY_SUM32) != 0;
   launch_sumsq3<T>((const T*)c->grid[SP_EX], (const T*)c->grid[SP_EY], (const T*)c->grid[SP_EZ], nx, ny,
-                   c->diag_d + 0, c->st);
+                   c->diag_d + 0, c->st, s32);
   launch_sumsq3<T>((const T*)c->grid[SP_BX], (const T*)c->grid[SP_BY], (const T*)c->grid[SP_BZ], nx, ny,
-                   c->diag_d + 1, c->st);
+                   c->diag_d + 1, c->st, s32);
   const double csqinv = 1.0 / ((double)c->p.c * c->p.c);
   for (size_t s = 0; s < c->species.size(); ++s)
-    launch_kinetic<T>(sdev<T>(c, c->species[s]), csqinv, c->diag_d + DIAG_FIXED + 4 * s, c->st);
+    launch_kinetic<T>(sdev<T>(c, c->species[s]), csqinv, c->diag_d + DIAG_FIXED + 4 * s, c->st, s32);
 }
 
 // Planes of the in-loop denoiser after the parameters in dn_buf: its input x
@@ -786,6 +789,9 @@ int sp_create(const sp_params* params, sp_ctx** out) {
   if (p.precision != SP_F32 && p.precision != SP_F64) return fail(nullptr, SP_EINVAL, "unknown precision");
   if (p.left_edge != SP_LEFT_ASSERT && p.left_edge != SP_LEFT_REFLECT)
     return fail(nullptr, SP_EINVAL, "unknown left_edge rule");
+  if (p.study & ~(SP_STUDY_ABS_POS32 | SP_STUDY_TWIDDLE32 | SP_STUDY_SUM32))
+    return fail(nullptr, SP_EINVAL, "unknown study flag");
+  if (p.study && p.precision != SP_F32) return fail(nullptr, SP_EINVAL, "study variants are of the FP32 path");
   // this build's limits: packed cells, LDS-resident 1D transforms
   if (p.size_x + 2 > 65535 || p.size_y + 2 > 32767)
     return fail(nullptr, SP_EINVAL, "grid too large for the packed (iy << 16 | ix) cell index");
@@ -833,8 +839,9 @@ int sp_create(const sp_params* params, sp_ctx** out) {
     fft_make_twiddles<double>(c->fx, c->st);
     fft_make_twiddles<double>(c->fy, c->st);
   } else {
-    fft_make_twiddles<float>(c->fx, c->st);
-    fft_make_twiddles<float>(c->fy, c->st);
+    const bool tw32 = (p.study & SP_STUDY_TWIDDLE32) != 0;
+    fft_make_twiddles<float>(c->fx, c->st, tw32);
+    fft_make_twiddles<float>(c->fy, c->st, tw32);
   }
   {
     // room for the smallest sort tile a tuning may pick
@@ -1147,6 +1154,8 @@ int sp_step(sp_ctx* c, int n) {
 
 int sp_set_denoiser(sp_ctx* c, const float* params, int n, float max_rel, float max_gauss) {
   ENTRY_PRELUDE(c);
+  c->tune_gen++;  // captured steps hold the denoiser's settings as kernel arguments
+  drop_graph(c);
   if (!params) {
     c->dn_on = false;
     return SP_OK;
@@ -1166,6 +1175,15 @@ int sp_set_denoiser(sp_ctx* c, const float* params, int n, float max_rel, float 
   c->dn_tol = max_rel;
   c->dn_gtol = max_gauss;
   c->dn_on = true;
+  return SP_OK;
+}
+
+int sp_set_denoiser_channels(sp_ctx* c, int mask) {
+  if (!c) return SP_EINVAL;
+  if (mask < 0 || mask > 0xF) return fail(c, SP_EINVAL, "channel mask must be in 0..15");
+  c->dn_mask = mask;
+  c->tune_gen++;  // captured steps hold the mask as a kernel argument
+  drop_graph(c);
   return SP_OK;
 }
 

@@ -9,7 +9,7 @@
 // z_l = conv(32->32)(a_{l-1}) + b, a_l = GELU(z_l) for l = 1..3 ->
 // r = conv(32->4)(a3) + b; output x + r (residual), times s_c dens.  "same"
 // convolutions; outside the image the input is zero, except that in the
-// simulation y is periodic.  GELU(z) = z Phi(z) = z (1 + erf(z / sqrt 2)) / 2.
+// simulation y is periodic.  GELU in its tanh form (below).
 //
 // Every convolution is an implicit GEMM on v_mfma_f32_16x16x16bf16_1k: a
 // workgroup owns 8 x 32 output pixels, stages its input tile (+1 halo) and the
@@ -36,10 +36,20 @@ __device__ __forceinline__ short bf16_bits(float f) {
   return (short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
-// d GELU / dz = Phi(z) + z phi(z)
+// GELU in its tanh form, z (1 + tanh u) / 2 = z sigmoid(2u) with
+// u = sqrt(2/pi) (z + 0.044715 z^3), evaluated with the hardware exp2 and
+// reciprocal (about 1 ulp each): a handful of instructions, where erf costs
+// tens; it runs on every activation of every layer.
+constexpr float GELU_C = 0.7978845608028654f, GELU_A = 0.044715f, LOG2E = 1.4426950408889634f;
+__device__ __forceinline__ float gelu_sig(float z) {  // sigmoid(2u)
+  const float u = GELU_C * z * (1.0f + GELU_A * z * z);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-2.0f * LOG2E * u));
+}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
+// d GELU / dz = s + z s (1 - s) 2 u', u' = sqrt(2/pi) (1 + 3 * 0.044715 z^2)
 __device__ __forceinline__ float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+  const float s = gelu_sig(z);
+  return s + z * s * (1.0f - s) * 2.0f * GELU_C * (1.0f + 3.0f * GELU_A * z * z);
 }
 
 constexpr int DN_TY = 8, DN_TX = 32, DN_NT = 256;  // output pixels of a workgroup: 8 rows x 32 columns
@@ -69,6 +79,27 @@ __device__ __forceinline__ void stage_tile(short* s_in, const float* __restrict_
   }
 }
 
+// Forward K order, tap-major: kk = tap * CIN + ci (tap = ky * 3 + kx), padded
+// to a multiple of 16 with zero weights; the fused inference kernel uses the
+// same order and k-step grouping, so both paths sum identically.
+template <int CIN> __device__ __forceinline__ void stage_offsets_tm(int* s_off) {
+  using S = ConvShape<CIN>;
+  for (int kk = threadIdx.x; kk < S::KP; kk += DN_NT) {
+    int o = 0;  // padded taps read any valid element: their weight is zero
+    if (kk < S::K) {
+      const int t = kk / CIN, ci = kk - t * CIN;
+      o = ci * S::LY * S::LX + (t / 3) * S::LX + (t % 3);
+    }
+    s_off[kk] = o;
+  }
+}
+// weight of K index kk (tap-major) in a [cout][CIN*9] (channel-major) row
+template <int CIN> __device__ __forceinline__ int tm_col(int kk) {
+  const int t = kk / CIN, ci = kk - t * CIN;
+  return ci * 9 + t;
+}
+
+// Weight-gradient K order (channel-major, the parameter layout): kk = ci * 9 + tap.
 template <int CIN> __device__ __forceinline__ void stage_offsets(int* s_off) {
   using S = ConvShape<CIN>;
   for (int kk = threadIdx.x; kk < S::KP; kk += DN_NT) {
@@ -98,9 +129,9 @@ __global__ void __launch_bounds__(DN_NT) k_conv3x3(const float* __restrict__ in,
   stage_tile<CIN, PERIODIC_Y, GIN>(s_in, in + (size_t)b * CIN * H * W, H, W, y0, x0);
   for (int i = threadIdx.x; i < NTL * 16 * S::KP; i += DN_NT) {
     const int n = i / S::KP, kk = i - n * S::KP;
-    s_w[i] = (n < COUT && kk < S::K) ? bf16_bits(w[n * S::K + kk]) : (short)0;
+    s_w[i] = (n < COUT && kk < S::K) ? bf16_bits(w[n * S::K + tm_col<CIN>(kk)]) : (short)0;
   }
-  stage_offsets<CIN>(s_off);
+  stage_offsets_tm<CIN>(s_off);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = (lane >> 4) * 4;
@@ -284,14 +315,15 @@ __global__ void __launch_bounds__(256) k_dn_sumsq(const float* __restrict__ r, i
   block_add(acc, sums + 1);
 }
 
-// Writes dens s_c (x + r) into the interior of the grids when the correction
+// Writes dens s_c (x + r) into the interior of grid c (for the channels set
+// in chmask) when the correction
 // passes the checks: |r| <= tol |x| (finite), and the previous solve's
 // pre-correction Gauss residual (gauss[0] / gauss[2], relative) is below
 // gauss_tol.  Otherwise the binomial-filtered grids stay (the fallback).
 // stats[0] counts applications, stats[1] fallbacks.
 __global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float dens, const float* __restrict__ x,
                            const float* __restrict__ r, int nx, int ny, const double* __restrict__ sums,
-                           const double* __restrict__ gauss, float tol, float gauss_tol, int* stats) {
+                           const double* __restrict__ gauss, float tol, float gauss_tol, int chmask, int* stats) {
   const double ratio2 = sums[0] > 0 ? sums[1] / sums[0] : 0.0;
   const double g2 = gauss[2] > 0 ? gauss[0] / gauss[2] : 0.0;
   const bool ok = ratio2 <= (double)tol * tol && g2 <= (double)gauss_tol * gauss_tol;
@@ -300,6 +332,7 @@ __global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float den
   const int64_t n = (int64_t)nx * ny;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n; i += (int64_t)gridDim.x * blockDim.x) {
     const int c = (int)(i / n);
+    if (!((chmask >> c) & 1)) continue;
     const int64_t p = i - c * n;
     const int yy = (int)(p / nx), xx = (int)(p - (int64_t)yy * nx);
     gr.g[c][(int64_t)(yy + 1) * (nx + 2) + xx + 1] = dens * scale[c] * (x[i] + r[i]);
@@ -307,139 +340,150 @@ __global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float den
 }
 
 // ---- fused inference: the five layers of one output tile in LDS ----
-// A workgroup owns FY x FX output pixels.  Layer l (0..4) is evaluated on the
-// tile plus a ring of 4 - l pixels; the activations GELU(z) stay in LDS as
-// BF16 (the precision the next layer's MFMA reads anyway), in two ping-pong
-// buffers, and each layer's weights are staged in turn.  Activations of
-// pixels outside the image are zero, as in the layer-by-layer path (columns
-// outside; rows wrap periodically), so both paths compute the same numbers.
+// A workgroup (FNT threads) owns FY x FX output pixels.  Layer l (0..4) is
+// evaluated on the tile plus a ring of 4 - l pixels; the activations GELU(z)
+// stay in LDS as BF16 (the precision the next layer's MFMA reads anyway),
+// pixel-major with the channels of a pixel contiguous (stride FCS, padded so
+// that the 16 lanes of a group read distinct banks), in two ping-pong
+// buffers.  The K order is tap-major (kk = tap * CIN + ci), so a lane's A
+// operand -- 4 consecutive channels of one tap of one pixel -- is one 8-byte
+// LDS read, and each wave keeps the layer's B operands (the weights) in
+// registers for all its pixel tiles.  Activations of pixels outside the
+// image are zero, as in the layer-by-layer path (columns outside; rows wrap
+// periodically); with the same K order and k-step grouping both paths
+// compute the same numbers.
 constexpr int FY = 8, FX = 32, FRING = DN_LAYERS;  // input region: tile + 5 rings
+constexpr int FNT = 512;                             // 8 waves
+constexpr int FCS = DN_CH + 4;                       // LDS pixel stride of 32-channel activations (72 B)
 
-// s_off[kk]: offset of tap kk in an input region of ih x iw pixels per plane
-// (0 for padded taps).
-template <int CIN> __device__ __forceinline__ void fused_offsets(int* s_off, int ih, int iw) {
-  constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16;
-  for (int kk = threadIdx.x; kk < KP; kk += DN_NT) {
-    int o = 0;
-    if (kk < K) {
-      const int ci = kk / 9, tp = kk - ci * 9;
-      o = ci * ih * iw + (tp / 3) * iw + (tp % 3);
-    }
-    s_off[kk] = o;
-  }
-}
-
-// The weights of one layer, [16 NTL][KP] as BF16, zero-padded.
+// The weights of one layer in tap-major K order, [16 NTL][KP] as BF16, zero-padded.
 template <int CIN, int COUT> __device__ __forceinline__ void fused_weights(short* s_w, const float* __restrict__ w) {
   constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16, NTL = (COUT + 15) / 16;
-  for (int i = threadIdx.x; i < NTL * 16 * KP; i += DN_NT) {
+  for (int i = threadIdx.x; i < NTL * 16 * KP; i += FNT) {
     const int n = i / KP, kk = i - n * KP;
-    s_w[i] = (n < COUT && kk < K) ? bf16_bits(w[n * K + kk]) : (short)0;
+    s_w[i] = (n < COUT && kk < K) ? bf16_bits(w[n * K + tm_col<CIN>(kk)]) : (short)0;
   }
 }
 
 // One conv of the fused chain: output region oh x ow, input region
-// (oh+2) x (ow+2) in s_src (planes of CIN).  Output pixel (qy, qx) of the
-// region is image pixel (y0 + qy - ring, x0 + qx - ring).  Not LAST: GELU,
-// zero outside the image's columns, BF16 into s_dst; LAST: r to HBM.
-template <int CIN, int COUT, bool LAST>
-__device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w, const int* s_off,
-                                            const float* __restrict__ bias, short* s_dst, float* __restrict__ out,
-                                            int oh, int ow, int ring, int y0, int x0, int H, int W) {
-  constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16, NTL = (COUT + 15) / 16;
+// (oh+2) x (ow+2) in s_src (pixel-major, SSTR shorts per pixel).  Output
+// pixel (qy, qx) of the region is image pixel (y0 + qy - ring, x0 + qx - ring).
+// Not LAST: GELU, zero outside the image's columns, BF16 into s_dst (pixel
+// stride FCS); LAST: r to HBM ([n][H][W]).
+template <int CIN, int COUT, int SSTR, bool LAST>
+__device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w, const float* __restrict__ bias,
+                                            short* s_dst, float* __restrict__ out, int oh, int ow, int ring, int y0,
+                                            int x0, int H, int W) {
+  constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16, NS = KP / 16, NTL = (COUT + 15) / 16;
   const int iw = ow + 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m = lane & 15, kq = (lane >> 4) * 4;
+  const int m = lane & 15, g = lane >> 4, kq = g * 4;
+  // B operands of this lane for every k-step, kept in registers
+  bf16x4 bw[NS][NTL];
+#pragma unroll
+  for (int st = 0; st < NS; ++st)
+#pragma unroll
+    for (int j = 0; j < NTL; ++j) bw[st][j] = *(const bf16x4*)(s_w + (j * 16 + m) * KP + st * 16 + kq);
   const int npx = oh * ow, ntile = (npx + 15) / 16;
-  for (int t = wave; t < ntile; t += DN_NT / 64) {
+  for (int t = wave; t < ntile; t += FNT / 64) {
     const int q = min(t * 16 + m, npx - 1);  // this lane's A-row pixel (clamped; its row is discarded)
     const int qy = q / ow, qx = q - qy * ow;
-    const int pbase = qy * iw + qx;
+    const int base = qy * iw + qx;
     f32x4 acc[NTL];
 #pragma unroll
     for (int j = 0; j < NTL; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int kc = 0; kc < KP; kc += 16) {
-      bf16x4 a, bv[NTL];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int kk = kc + kq + i;
-        a[i] = s_src[pbase + s_off[kk]];
-#pragma unroll
-        for (int j = 0; j < NTL; ++j) bv[j][i] = s_w[(j * 16 + m) * KP + kk];
+    for (int st = 0; st < NS; ++st) {
+      int tap, c0;
+      if constexpr (CIN == 4) {  // k-step st: taps 4 st .. 4 st + 3, one per lane group
+        tap = st * 4 + g;
+        c0 = 0;
+        if (tap > 8) tap = 0;  // padding: zero weights
+      } else {  // k-step st: 16 channels of tap st / (CIN / 16)
+        tap = st / (CIN / 16);
+        c0 = (st % (CIN / 16)) * 16 + kq;
       }
+      const int off = (tap / 3) * iw + (tap % 3);
+      const bf16x4 a = *(const bf16x4*)(s_src + (base + off) * SSTR + c0);
 #pragma unroll
-      for (int j = 0; j < NTL; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[j], acc[j], 0, 0, 0);
+      for (int j = 0; j < NTL; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[st][j], acc[j], 0, 0, 0);
+    }
+    // this lane's 4 output pixels p0 .. p0 + 3 (ow > 4: at most one row wrap)
+    const int p0 = t * 16 + kq;
+    int py[4], px[4];
+    py[0] = p0 / ow;
+    px[0] = p0 - py[0] * ow;
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+      const bool wrap = px[i - 1] + 1 == ow;
+      px[i] = wrap ? 0 : px[i - 1] + 1;
+      py[i] = py[i - 1] + (wrap ? 1 : 0);
     }
 #pragma unroll
     for (int j = 0; j < NTL; ++j) {
-      const int n = j * 16 + (lane & 15);
+      const int n = j * 16 + m;
       if (n >= COUT) continue;
       const float bn = bias[n];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int p = t * 16 + kq + i;
+        const int p = p0 + i;
         if (p >= npx) continue;
-        const int py = p / ow, px = p - py * ow;
-        const int x = x0 + px - ring;
+        const int x = x0 + px[i] - ring;
         float v = acc[j][i] + bn;
         if (LAST) {
-          const int y = y0 + py - ring;
+          const int y = y0 + py[i] - ring;
           if (y < H && x >= 0 && x < W) out[((size_t)n * H + y) * W + x] = v;
         } else {
           v = (x < 0 || x >= W) ? 0.0f : gelu(v);  // outside the image (columns); rows wrap
-          s_dst[n * npx + p] = bf16_bits(v);
+          s_dst[p * FCS + n] = bf16_bits(v);
         }
       }
     }
   }
 }
 
-__global__ void __launch_bounds__(DN_NT) k_dn_fused(const float* __restrict__ x, const float* __restrict__ params,
-                                                    float* __restrict__ r, int H, int W) {
+__global__ void __launch_bounds__(FNT) k_dn_fused(const float* __restrict__ x, const float* __restrict__ params,
+                                                  float* __restrict__ r, int H, int W) {
   constexpr int H0 = FY + 2 * FRING, W0 = FX + 2 * FRING;
   // layer l's output region: (FY + 2 (4 - l)) x (FX + 2 (4 - l)); ping-pong
   // buffers: A holds the outputs of layers 0 and 2, B those of 1 and 3
   constexpr int NA = (FY + 8) * (FX + 8), NB = (FY + 6) * (FX + 6);
-  __shared__ short s_x[DN_IN * H0 * W0];
-  __shared__ short s_a[DN_CH * NA];
-  __shared__ short s_b[DN_CH * NB];
-  __shared__ short s_w[DN_CH * DN_CH * 9];
-  __shared__ int s_off[DN_CH * 9];
+  __shared__ __attribute__((aligned(16))) short s_x[H0 * W0 * DN_IN];
+  __shared__ __attribute__((aligned(16))) short s_a[NA * FCS];
+  __shared__ __attribute__((aligned(16))) short s_b[NB * FCS];
+  __shared__ __attribute__((aligned(16))) short s_w[DN_CH * DN_CH * 9];
   const DnParams P = dn_params(const_cast<float*>(params));
   const int y0 = blockIdx.y * FY, x0 = blockIdx.x * FX;
-  for (int i = threadIdx.x; i < DN_IN * H0 * W0; i += DN_NT) {
-    const int c = i / (H0 * W0), rc = i - c * (H0 * W0);
-    const int ry = rc / W0, rx = rc - ry * W0;
+  for (int i = threadIdx.x; i < DN_IN * H0 * W0; i += FNT) {
+    const int pix = i / DN_IN, c = i - pix * DN_IN;  // pixel-major, channel fastest
+    const int ry = pix / W0, rx = pix - ry * W0;
     int y = y0 + ry - FRING;
     const int xx = x0 + rx - FRING;
     y = ((y % H) + H) % H;
     s_x[i] = bf16_bits(xx >= 0 && xx < W ? x[((size_t)c * H + y) * W + xx] : 0.0f);
   }
   fused_weights<DN_IN, DN_CH>(s_w, P.w[0]);
-  fused_offsets<DN_IN>(s_off, H0, W0);
   __syncthreads();
-  fused_layer<DN_IN, DN_CH, false>(s_x, s_w, s_off, P.b[0], s_a, nullptr, FY + 8, FX + 8, 4, y0, x0, H, W);
+  fused_layer<DN_IN, DN_CH, DN_IN, false>(s_x, s_w, P.b[0], s_a, nullptr, FY + 8, FX + 8, 4, y0, x0, H, W);
 #pragma unroll
   for (int l = 1; l < DN_LAYERS - 1; ++l) {
     const int oh = FY + 2 * (4 - l), ow = FX + 2 * (4 - l);
     const short* src = (l & 1) ? s_a : s_b;
     short* dst = (l & 1) ? s_b : s_a;
-    __syncthreads();  // the previous layer is done with s_w
+    __syncthreads();  // the previous layer is done with s_w and has written src
     fused_weights<DN_CH, DN_CH>(s_w, P.w[l]);
-    fused_offsets<DN_CH>(s_off, oh + 2, ow + 2);
     __syncthreads();
-    fused_layer<DN_CH, DN_CH, false>(src, s_w, s_off, P.b[l], dst, nullptr, oh, ow, 4 - l, y0, x0, H, W);
+    fused_layer<DN_CH, DN_CH, FCS, false>(src, s_w, P.b[l], dst, nullptr, oh, ow, 4 - l, y0, x0, H, W);
   }
   __syncthreads();
   fused_weights<DN_CH, DN_IN>(s_w, P.w[DN_LAYERS - 1]);
-  fused_offsets<DN_CH>(s_off, FY + 2, FX + 2);
   __syncthreads();
-  fused_layer<DN_CH, DN_IN, true>(s_b, s_w, s_off, P.b[DN_LAYERS - 1], nullptr, r, FY, FX, 0, y0, x0, H, W);
+  fused_layer<DN_CH, DN_IN, FCS, true>(s_b, s_w, P.b[DN_LAYERS - 1], nullptr, r, FY, FX, 0, y0, x0, H, W);
 }
 
 void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipStream_t st) {
   dim3 grid((W + FX - 1) / FX, (H + FY - 1) / FY);
-  hipLaunchKernelGGL(k_dn_fused, grid, dim3(DN_NT), 0, st, T.x, P.w[0], T.r, H, W);
+  hipLaunchKernelGGL(k_dn_fused, grid, dim3(FNT), 0, st, T.x, P.w[0], T.r, H, W);
 }
 
 // ---- launchers ----
@@ -508,7 +552,7 @@ void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, in
 }
 
 void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, int nx, int ny, int dens, double* sums,
-                  const double* gauss, float tol, float gauss_tol, int* stats, hipStream_t st) {
+                  const double* gauss, float tol, float gauss_tol, int chmask, int* stats, hipStream_t st) {
   Grids4 gr;
   for (int c = 0; c < 4; ++c) gr.g[c] = grids[c];
   const int64_t n4 = (int64_t)4 * nx * ny;
@@ -519,7 +563,7 @@ void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, in
   dn_forward_fused(P, T, ny, nx, st);
   hipLaunchKernelGGL(k_dn_sumsq, dim3(nblk(n4, 1024)), dim3(256), 0, st, T.r, n4, sums);
   hipLaunchKernelGGL(k_dn_apply, dim3(nblk(n4)), dim3(256), 0, st, gr, P.scale, (float)dens, T.x, T.r, nx, ny, sums,
-                     gauss, tol, gauss_tol, stats);
+                     gauss, tol, gauss_tol, chmask, stats);
 }
 
 }  // namespace sp

@@ -249,16 +249,22 @@ __global__ void __launch_bounds__(512) k_fft_cols(FftBatch b, FftDim d, int nx, 
   }
 }
 
-template <class T> __global__ void k_twiddles(V2<T>* tw, int n) {
+template <class T> __global__ void k_twiddles(V2<T>* tw, int n, int fp32_gen) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
+  if (fp32_gen) {  // precision study: the twiddle computed in FP32
+    float s, c;
+    sincospif(-2.0f * (float)k / (float)n, &s, &c);
+    tw[k] = V2<T>{(T)c, (T)s};
+    return;
+  }
   double s, c;
   sincospi(-2.0 * (double)k / (double)n, &s, &c);  // FP64 generation
   tw[k] = V2<T>{(T)c, (T)s};
 }
 
-template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st) {
-  hipLaunchKernelGGL(k_twiddles<T>, dim3((d.n + 255) / 256), dim3(256), 0, st, (V2<T>*)d.tw, d.n);
+template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st, bool fp32_gen) {
+  hipLaunchKernelGGL(k_twiddles<T>, dim3((d.n + 255) / 256), dim3(256), 0, st, (V2<T>*)d.tw, d.n, (int)fp32_gen);
 }
 
 static int round_threads(int e) {
@@ -306,7 +312,7 @@ void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, d
 }
 
 #define INST(T)                                                                                 \
-  template void fft_make_twiddles<T>(FftDim&, hipStream_t);                                     \
+  template void fft_make_twiddles<T>(FftDim&, hipStream_t, bool);                               \
   template void launch_fft_rows<T>(const FftBatch&, const FftDim&, int, int, int, int, double, hipStream_t); \
   template void launch_fft_cols<T>(const FftBatch&, const FftDim&, int, int, double, hipStream_t);
 INST(float)

@@ -77,10 +77,16 @@ __device__ __forceinline__ void move_half(int& ix, int& iy, T& dx, T& dy, T& px,
   // a crossing moves by one cell: d - 1 above 0.5, d - (-1) = d + 1 below
   // -0.5, d - 0 = d otherwise (all exact, as the reference's branches)
   dx += c1 * px;
+  dy += c1 * py;
+  if (sizeof(T) == 4 && g.abs_pos32) {
+    // precision study: the position held as one absolute FP32 coordinate
+    // (x = ix + dx rounded to float), the offset read back exactly
+    dx = (T(ix) + dx) - T(ix);
+    dy = (T(iy) + dy) - T(iy);
+  }
   const int sx = (dx >= T(0.5)) - (dx < T(-0.5));
   dx -= T(sx);
   ix += sx;
-  dy += c1 * py;
   const int sy = (dy >= T(0.5)) - (dy < T(-0.5));
   dy -= T(sy);
   iy += sy;

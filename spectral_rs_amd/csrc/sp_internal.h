@@ -31,6 +31,7 @@ template <class T> struct Geo {
   int direct;              // particles are kept tile-sorted and interleaved: the particle
                            // pass reads them in place (no LDS staging or lane permutation)
   int chunk;               // particles per workgroup of the particle pass (host-side choice)
+  int abs_pos32;           // precision study: keep positions to absolute-FP32 resolution
   T dt, c, csqinv;         // csqinv = 1 / (c * c) in Float (src/prtls.rs:265)
   T half_dt;               // 0.5 * dt in Float (src/prtls.rs:103)
 };
@@ -98,9 +99,11 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
                                                hipStream_t st);
 // FP64 diagnostics: out[0] += sum over the interior of a^2 + b^2 + c^2;
 // kinetic: out[0] += sum (gamma - 1), out[1..3] += sum p.
+// sum32: accumulate in FP32 (precision study), the total still stored as FP64.
 template <class T> void launch_sumsq3(const T* a, const T* b, const T* c, int nx, int ny, double* out,
-                                      hipStream_t st);
-template <class T> void launch_kinetic(const SpeciesDev<T>& s, double csqinv, double* out, hipStream_t st);
+                                      hipStream_t st, bool sum32 = false);
+template <class T> void launch_kinetic(const SpeciesDev<T>& s, double csqinv, double* out, hipStream_t st,
+                                       bool sum32 = false);
 // Hermitian projection of `count` spectra of real fields (see fields.hip).
 template <class T> void launch_hermitian(void* const* spec, int count, int nx, int ny, hipStream_t st);
 
@@ -112,7 +115,8 @@ struct FftDim {
   uint64_t radix_code;  // radix of pass p in bits [4p, 4p+4)
   void* tw;             // n complex Float: exp(-2 pi i k / n)
 };
-template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st);
+// fp32_gen: generate in FP32 (precision study) instead of FP64.
+template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st, bool fp32_gen = false);
 // Row (x) and column (y) passes over up to 8 spectra at once.
 enum FftLoad { LOAD_COMPLEX = 0, LOAD_REAL_PADDED = 1 };
 enum FftStore { STORE_COMPLEX = 0, STORE_REAL_PADDED = 1 };
@@ -176,7 +180,10 @@ void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, in
 // All layers of one image [4][H][W] in one launch, y periodic (the
 // simulation's inference path); same results as dn_forward(periodic_y).
 void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipStream_t st);
+// Packs the 4 filtered grids, runs the fused network, and when the checks
+// pass writes the corrected planes back for the channels set in chmask (bit c
+// = grid c of Jx Jy Jz rho); stats[0] counts applications, stats[1] fallbacks.
 void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, int nx, int ny, int dens, double* sums,
-                  const double* gauss, float tol, float gauss_tol, int* stats, hipStream_t st);
+                  const double* gauss, float tol, float gauss_tol, int chmask, int* stats, hipStream_t st);
 
 }  // namespace sp

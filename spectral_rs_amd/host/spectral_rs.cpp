@@ -10,8 +10,15 @@
 //   [params]   mass_ratio = 1.0, vth_ion = 1e-3, vth_lec = 1e-3, seed = 0,
 //              precision = "f32" | "f64" (cargo feature dprec),
 //              periodic = false (feature periodic), checked = true
-//              (unchecked = !checked), left_edge = "assert" | "reflect"
+//              (unchecked = !checked), left_edge = "assert" | "reflect",
+//              sort_interval = 20 (steps between tile sorts of the particles;
+//              0 = never), autotune = false (time the launch shapes first)
 //   [output]   verbose = false (print the reference's per-step timer lines)
+//
+// With several config files (spectral_rs a/config.toml b/config.toml ...)
+// the runs are independent members of one ensemble stepped together on the
+// GPU (sp_ensemble_*); each writes its output under its config's directory
+// (a/output/dat_NNNNN/...).  They must share t_final.
 #include <sys/stat.h>
 
 #include <cerrno>
@@ -88,8 +95,10 @@ struct Config {  // src/lib.rs:34-66 plus the optional extensions
   int64_t track_interval = 0, output_interval = 1, stride = 1, istep = 1;
   double mass_ratio = 1.0, vth_ion = 1e-3, vth_lec = 1e-3;
   uint64_t seed = 0;
-  bool f64 = false, periodic = false, checked = true;
+  bool f64 = false, periodic = false, checked = true, autotune = false;
   int left_edge = SP_LEFT_ASSERT;
+  int64_t sort_interval = 20;
+  std::string out_dir = "output";  // where save_output writes
 };
 
 bool get_num(const Toml& t, const std::string& k, double& v, bool required, std::string& err) {
@@ -155,7 +164,9 @@ bool load_config(const std::string& path, Config& c, std::string& err) {
             get_bool(t, "output.verbose", c.verbose, false, err) &&
             get_num(t, "params.mass_ratio", c.mass_ratio, false, err) &&
             get_num(t, "params.vth_ion", c.vth_ion, false, err) && get_num(t, "params.vth_lec", c.vth_lec, false, err) &&
-            get_bool(t, "params.periodic", c.periodic, false, err) && get_bool(t, "params.checked", c.checked, false, err);
+            get_bool(t, "params.periodic", c.periodic, false, err) && get_bool(t, "params.checked", c.checked, false, err) &&
+            get_bool(t, "params.autotune", c.autotune, false, err) &&
+            get_int(t, "params.sort_interval", c.sort_interval, false, err);
   if (!ok) return false;
   int64_t seed = 0;
   if (!get_int(t, "params.seed", seed, false, err)) return false;
@@ -238,8 +249,8 @@ template <class F> std::vector<F> strided(const std::vector<F>& v, size_t stride
 template <class F> int save_output(sp_ctx* ctx, const Config& cfg, uint32_t t, std::string& err) {
   if (t % cfg.output_interval != 0) return 0;
   char pre[64];
-  std::snprintf(pre, sizeof pre, "output/dat_%05u", (unsigned)(t / cfg.output_interval));
-  const std::string prefix = pre;
+  std::snprintf(pre, sizeof pre, "/dat_%05u", (unsigned)(t / cfg.output_interval));
+  const std::string prefix = cfg.out_dir + pre;
   if (!mkdirs(prefix + "/flds")) {
     err = "Unable to create output directory";
     return 1;
@@ -304,8 +315,9 @@ template <class F> int save_output(sp_ctx* ctx, const Config& cfg, uint32_t t, s
     }                                                                      \
   } while (0)
 
-// run (src/lib.rs:119-202)
-int run(const Config& cfg) {
+// Flds::new + Sim::new + the two Prtl::new of run() (src/lib.rs:119-142):
+// context, species, pre-loop move & deposit; optionally the autotuner.
+int setup(const Config& cfg, sp_ctx** out) {
   sp_params p{};
   p.size_x = (int32_t)cfg.size_x;
   p.size_y = (int32_t)cfg.size_y;
@@ -319,6 +331,7 @@ int run(const Config& cfg) {
   p.periodic_x = cfg.periodic;
   p.checked = cfg.checked;
   p.left_edge = cfg.left_edge;
+  p.sort_interval = (int32_t)cfg.sort_interval;
   sp_ctx* ctx = nullptr;
   if (sp_create(&p, &ctx) != SP_OK) {
     std::printf("Application error: %s\n", sp_last_error(nullptr));
@@ -328,18 +341,35 @@ int run(const Config& cfg) {
   int sid = 0;
   CHECK(sp_species_init(ctx, 1.0, cfg.mass_ratio, cfg.vth_ion, cfg.seed, &sid));  // ions (lib.rs:124)
   CHECK(sp_species_init(ctx, -1.0, 1.0, cfg.vth_lec, cfg.seed, &sid));           // electrons (lib.rs:126)
-  CHECK(sp_prestep(ctx));                                                        // lib.rs:130-142
+  if (cfg.autotune) {
+    sp_tuning tu;
+    CHECK(sp_autotune(ctx, 3, 1, &tu));
+    std::printf("autotuned: chunk %d, sort tile %dx%d, FFT rows %d (radix %d), columns %d (radix %d)\n", tu.chunk,
+                1 << tu.tile_w_log, 1 << tu.tile_h_log, tu.fft_rows, tu.fft_radix_x, tu.fft_cols, tu.fft_radix_y);
+  }
+  CHECK(sp_prestep(ctx));  // lib.rs:130-142
+  *out = ctx;
+  return 0;
+}
+
+int save(sp_ctx* ctx, const Config& cfg, uint32_t t) {
+  if (!cfg.write_output) return 0;
+  std::string err;
+  const int rc = cfg.f64 ? save_output<double>(ctx, cfg, t, err) : save_output<float>(ctx, cfg, t, err);
+  if (rc) std::printf("Application error: %s\n", err.c_str());
+  return rc;
+}
+
+// run (src/lib.rs:119-202)
+int run(const Config& cfg) {
+  sp_ctx* ctx = nullptr;
+  if (setup(cfg, &ctx)) return 1;
   CHECK(sp_set_timing(ctx, 1));
   double ms[4];
   for (uint32_t t = 0; t <= cfg.t_final; ++t) {
-    if (cfg.write_output) {
-      std::string err;
-      int rc = cfg.f64 ? save_output<double>(ctx, cfg, t, err) : save_output<float>(ctx, cfg, t, err);
-      if (rc) {
-        std::printf("Application error: %s\n", err.c_str());
-        sp_destroy(ctx);
-        return 1;
-      }
+    if (save(ctx, cfg, t)) {
+      sp_destroy(ctx);
+      return 1;
     }
     CHECK(sp_step(ctx, 1));
     if (cfg.verbose) {
@@ -357,14 +387,83 @@ int run(const Config& cfg) {
   return 0;
 }
 
+// Several runs as one ensemble: the same loop for every member, stepped
+// together between the output steps of any member.
+int run_ensemble(std::vector<Config>& cfgs) {
+  std::vector<sp_ctx*> ctx;
+  auto cleanup = [&](sp_ensemble* e) {
+    if (e) sp_ensemble_destroy(e);
+    for (sp_ctx* c : ctx) sp_destroy(c);
+  };
+  for (auto& c : cfgs) {
+    sp_ctx* x = nullptr;
+    if (setup(c, &x)) {
+      cleanup(nullptr);
+      return 1;
+    }
+    ctx.push_back(x);
+  }
+  sp_ensemble* e = nullptr;
+  if (sp_ensemble_create(ctx.data(), (int)ctx.size(), 4, 2, &e) != SP_OK) {
+    std::printf("Application error: %s\n", sp_ensemble_last_error(nullptr));
+    cleanup(nullptr);
+    return 1;
+  }
+  const uint32_t tf = cfgs[0].t_final;
+  uint32_t t = 0;
+  while (t <= tf) {
+    uint32_t next = tf + 1;
+    for (size_t k = 0; k < cfgs.size(); ++k) {
+      const Config& c = cfgs[k];
+      if (save(ctx[k], c, t)) {
+        cleanup(e);
+        return 1;
+      }
+      if (c.write_output) {
+        const uint32_t nt = (t / c.output_interval + 1) * c.output_interval;
+        if (nt < next) next = nt;
+      }
+    }
+    if (sp_ensemble_step(e, (int)(next - t)) != SP_OK) {
+      std::printf("Application error: %s\n", sp_ensemble_last_error(e));
+      cleanup(e);
+      return 1;
+    }
+    t = next;
+  }
+  int bad = -1;
+  if (sp_ensemble_sync(e, &bad) != SP_OK) {
+    std::printf("Application error: %s\n", sp_ensemble_last_error(e));
+    cleanup(e);
+    return 1;
+  }
+  std::printf("done: %zu runs x %u steps\n", cfgs.size(), tf + 1);
+  cleanup(e);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {  // src/main.rs:3-13
-  Config cfg;
-  std::string err;
-  if (!load_config(argc > 1 ? argv[1] : "config.toml", cfg, err)) {
-    std::printf("Problem parsing arguments: %s\n", err.c_str());
-    return 1;
+  std::vector<Config> cfgs;
+  const int nfiles = argc > 1 ? argc - 1 : 1;
+  for (int i = 0; i < nfiles; ++i) {
+    const std::string path = argc > 1 ? argv[1 + i] : "config.toml";
+    Config cfg;
+    std::string err;
+    if (!load_config(path, cfg, err)) {
+      std::printf("Problem parsing arguments: %s\n", err.c_str());
+      return 1;
+    }
+    if (nfiles > 1) {  // each member writes beside its config file
+      const size_t slash = path.find_last_of('/');
+      cfg.out_dir = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/output";
+      if (!cfgs.empty() && cfg.t_final != cfgs[0].t_final) {
+        std::printf("Problem parsing arguments: ensemble members must share t_final\n");
+        return 1;
+      }
+    }
+    cfgs.push_back(cfg);
   }
-  return run(cfg);
+  return nfiles > 1 ? run_ensemble(cfgs) : run(cfgs[0]);
 }

@@ -5,7 +5,7 @@ has no such network).
 Network: x (4 planes) -> z0 = conv3x3(4->32) + b0 -> a0 = GELU(z0) ->
 z_l = conv3x3(32->32)(a_{l-1}) + b_l, a_l = GELU(z_l) (l = 1..3) ->
 r = conv3x3(32->4)(a3) + b4, output x + r; "same" convolutions, zero
-outside the image; GELU(z) = z (1 + erf(z / sqrt 2)) / 2.  Parameter vector
+outside the image; GELU in its tanh form (gelu below).  Parameter vector
 layout: per layer w_l [cout][cin*9] then b_l [cout]; then scales [4];
 w[co][ci*9 + ky*3 + kx].
 
@@ -16,7 +16,6 @@ kernels store them; sums are FP64.  With emulate=False it is plain FP64,
 which the finite-difference test pins.
 """
 import numpy as np
-from scipy.special import erf
 
 IN, CH, LAYERS = 4, 32, 5
 
@@ -56,12 +55,17 @@ def bf16(a):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+GELU_C, GELU_A = np.sqrt(2.0 / np.pi), 0.044715
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    """GELU, tanh form: z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))) / 2."""
+    return 0.5 * z * (1.0 + np.tanh(GELU_C * (z + GELU_A * z ** 3)))
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+    t = np.tanh(GELU_C * (z + GELU_A * z ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * GELU_C * (1.0 + 3.0 * GELU_A * z * z)
 
 
 def conv(x, w, b=None, periodic_y=False):

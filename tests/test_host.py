@@ -73,3 +73,50 @@ def test_host_run_writes_reference_output_layout(tmp_path):
         assert np.all(gam >= 1.0)
         for f in ["Jx", "Jy", "Jz", "Ex", "Ey", "Ez", "Bx", "By", "Bz", "Dens"]:
             assert np.load(d / "flds" / f"{f}.npy").shape == (48 * 24,)
+
+
+def write_cfg(d, **kw):
+    p = dict(t_final=3, size_x=48, size_y=24, delta=4, gamma_inj=10.0, write_output="true", istep=1)
+    p.update(kw)
+    d.mkdir(parents=True, exist_ok=True)
+    (d / "config.toml").write_text(REF_CONFIG.format(**p))
+    return str(d / "config.toml")
+
+
+def test_ensemble_members_must_share_t_final(tmp_path):
+    a = write_cfg(tmp_path / "a", t_final=3)
+    b = write_cfg(tmp_path / "b", t_final=4)
+    if not os.path.exists(B.HOST):
+        B.build_host()
+    r = subprocess.run([B.HOST, a, b], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "ensemble members must share t_final" in r.stdout
+
+
+def test_sort_interval_must_not_be_negative(tmp_path):
+    a = write_cfg(tmp_path)
+    txt = open(a).read().replace("n_pass = 4", "n_pass = 4\nsort_interval = -1")
+    open(a, "w").write(txt)
+    if not os.path.exists(B.HOST):
+        B.build_host()
+    r = subprocess.run([B.HOST, a], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "`params.sort_interval` must be a non-negative integer" in r.stdout
+
+
+@pytest.mark.gpu
+def test_host_ensemble_writes_each_members_output(tmp_path):
+    # two runs of different sizes (one autotuned) as one ensemble; each writes
+    # the reference's layout beside its config file
+    a = write_cfg(tmp_path / "a")
+    b = write_cfg(tmp_path / "b", size_x=96, size_y=32, delta=8)
+    txt = open(b).read().replace("n_pass = 4", "n_pass = 4\nautotune = true")
+    open(b, "w").write(txt)
+    if not os.path.exists(B.HOST):
+        B.build_host()
+    r = subprocess.run([B.HOST, a, b], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    assert "autotuned" in r.stdout
+    for d, n in [(tmp_path / "a", 2 * 40 * 24), (tmp_path / "b", 2 * 80 * 32)]:
+        for t in range(4):
+            x = np.load(d / "output" / f"dat_{t:05d}" / "x.npy")
+            assert x.size == n
+        assert np.load(d / "output" / "dat_00003" / "flds" / "Bz.npy").size in (48 * 24, 96 * 32)
