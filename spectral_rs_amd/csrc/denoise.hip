@@ -21,6 +21,8 @@
 // pre-activations z in FP32; a layer's input a = GELU(z) is formed when the
 // next layer stages it (rounded to BF16 there, as the MFMA reads it).  The
 // weight gradient is the same MFMA with pixels as the K dimension.
+#include <algorithm>
+
 #include "sp_internal.h"
 
 static_assert(SP_DN_NTRAIN == sp::DN_NTRAIN && SP_DN_NPARAMS == sp::DN_NPARAMS, "parameter count of the C ABI");
@@ -30,11 +32,8 @@ namespace sp {
 typedef __attribute__((ext_vector_type(4))) short bf16x4;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 
-// FP32 -> BF16 bits, round to nearest even (finite inputs).
-__device__ __forceinline__ short bf16_bits(float f) {
-  const unsigned u = __builtin_bit_cast(unsigned, f);
-  return (short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
+// FP32 -> BF16 bits, round to nearest even (gfx950's v_cvt_pk_bf16_f32).
+__device__ __forceinline__ short bf16_bits(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
 // GELU in its tanh form, z (1 + tanh u) / 2 = z sigmoid(2u) with
 // u = sqrt(2/pi) (z + 0.044715 z^3), evaluated with the hardware exp2 and
@@ -288,17 +287,18 @@ __device__ __forceinline__ void block_add(double v, double* out) {
 }
 
 // x[c][y][x] = grid_c(interior) / (dens s_c); sums |x|^2 into sums[0].
-__global__ void __launch_bounds__(256) k_dn_pack(Grids4 gr, const float* __restrict__ scale, float inv_dens, float* __restrict__ x, int nx,
-                          int ny, double* sums) {
-  const int64_t n = (int64_t)nx * ny;
+// Grid: x over the columns, y over (channel, row) pairs with a stride.
+__global__ void __launch_bounds__(256) k_dn_pack(Grids4 gr, const float* __restrict__ scale, float inv_dens,
+                                                 float* __restrict__ x, int nx, int ny, double* sums) {
+  const int xx = blockIdx.x * blockDim.x + threadIdx.x;
   double acc = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i / n);
-    const int64_t p = i - c * n;
-    const int yy = (int)(p / nx), xx = (int)(p - (int64_t)yy * nx);
-    const float v = gr.g[c][(int64_t)(yy + 1) * (nx + 2) + xx + 1] * inv_dens / scale[c];
-    x[i] = v;
-    acc += (double)v * v;
+  if (xx < nx) {
+    for (int cr = blockIdx.y; cr < 4 * ny; cr += gridDim.y) {
+      const int c = cr / ny, yy = cr - c * ny;
+      const float v = gr.g[c][(int64_t)(yy + 1) * (nx + 2) + xx + 1] * inv_dens / scale[c];
+      x[(int64_t)cr * nx + xx] = v;
+      acc += (double)v * v;
+    }
   }
   block_add(acc, sums);
 }
@@ -327,14 +327,14 @@ __global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float den
   const double ratio2 = sums[0] > 0 ? sums[1] / sums[0] : 0.0;
   const double g2 = gauss[2] > 0 ? gauss[0] / gauss[2] : 0.0;
   const bool ok = ratio2 <= (double)tol * tol && g2 <= (double)gauss_tol * gauss_tol;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stats + (ok ? 0 : 1), 1);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) atomicAdd(stats + (ok ? 0 : 1), 1);
   if (!ok) return;
-  const int64_t n = (int64_t)nx * ny;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i / n);
+  const int xx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (xx >= nx) return;
+  for (int cr = blockIdx.y; cr < 4 * ny; cr += gridDim.y) {  // (channel, row) pairs, as k_dn_pack
+    const int c = cr / ny, yy = cr - c * ny;
     if (!((chmask >> c) & 1)) continue;
-    const int64_t p = i - c * n;
-    const int yy = (int)(p / nx), xx = (int)(p - (int64_t)yy * nx);
+    const int64_t i = (int64_t)cr * nx + xx;
     gr.g[c][(int64_t)(yy + 1) * (nx + 2) + xx + 1] = dens * scale[c] * (x[i] + r[i]);
   }
 }
@@ -347,35 +347,69 @@ __global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float den
 // that the 16 lanes of a group read distinct banks), in two ping-pong
 // buffers.  The K order is tap-major (kk = tap * CIN + ci), so a lane's A
 // operand -- 4 consecutive channels of one tap of one pixel -- is one 8-byte
-// LDS read, and each wave keeps the layer's B operands (the weights) in
-// registers for all its pixel tiles.  Activations of pixels outside the
-// image are zero, as in the layer-by-layer path (columns outside; rows wrap
-// periodically); with the same K order and k-step grouping both paths
-// compute the same numbers.
-constexpr int FY = 8, FX = 32, FRING = DN_LAYERS;  // input region: tile + 5 rings
-constexpr int FNT = 512;                             // 8 waves
+// LDS read at a compile-time offset from the lane's pixel, and each wave
+// keeps the layer's B operands (the weights) in registers for all its pixel
+// tiles.  The weights come pre-converted (BF16, tap-major, zero-padded:
+// dn_pack_weights) and are copied to LDS with 16-byte loads.  Activations of
+// pixels outside the image are zero, as in the layer-by-layer path (columns
+// outside; rows wrap periodically); with the same K order and k-step grouping
+// both paths compute the same numbers.
+constexpr int FY = 16, FX = 32, FRING = DN_LAYERS;  // input region: tile + 5 rings (156 KB of LDS in all)
+constexpr int FNT = 768;                             // 12 waves: 3 per SIMD (<= 170 VGPRs each)
 constexpr int FCS = DN_CH + 4;                       // LDS pixel stride of 32-channel activations (72 B)
 
-// The weights of one layer in tap-major K order, [16 NTL][KP] as BF16, zero-padded.
-template <int CIN, int COUT> __device__ __forceinline__ void fused_weights(short* s_w, const float* __restrict__ w) {
-  constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16, NTL = (COUT + 15) / 16;
-  for (int i = threadIdx.x; i < NTL * 16 * KP; i += FNT) {
-    const int n = i / KP, kk = i - n * KP;
-    s_w[i] = (n < COUT && kk < K) ? bf16_bits(w[n * K + tm_col<CIN>(kk)]) : (short)0;
+// Packed layer l: [16 NTL][KP] BF16 in tap-major K order.
+__host__ __device__ constexpr int pack_kp(int l) { return (dn_cin(l) * 9 + 15) / 16 * 16; }
+__host__ __device__ constexpr int pack_rows(int l) { return (dn_cout(l) + 15) / 16 * 16; }
+__host__ __device__ constexpr int pack_off(int l) {
+  int o = 0;
+  for (int k = 0; k < l; ++k) o += pack_rows(k) * pack_kp(k);
+  return o;
+}
+static_assert(pack_off(DN_LAYERS) == DN_PACK_SHORTS, "packed weight size");
+
+__global__ void k_dn_pack_weights(DnParams P, short* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= DN_PACK_SHORTS) return;
+  int l = 0;
+  while (l + 1 < DN_LAYERS && i >= pack_off(l + 1)) ++l;
+  const int j = i - pack_off(l), KP = pack_kp(l), cin = dn_cin(l), K = cin * 9;
+  const int n = j / KP, kk = j - n * KP;
+  short v = 0;
+  if (n < dn_cout(l) && kk < K) {
+    const int t = kk / cin, ci = kk - t * cin;
+    v = bf16_bits(P.w[l][n * K + ci * 9 + t]);
+  }
+  out[i] = v;
+}
+
+// In LDS a weight row (one output channel) is padded by 4 BF16 to KP + 4:
+// lanes m = 0..15 of a group read rows m at a word stride of (KP + 4) / 2,
+// which spreads their 8-byte reads over distinct banks (KP alone, a multiple
+// of 16, would put every second or fourth row in the same bank).
+__host__ __device__ constexpr int lds_kp(int l) { return pack_kp(l) + 4; }
+
+template <int L> __device__ __forceinline__ void fused_weights(short* s_w, const short* __restrict__ packed) {
+  constexpr int KP = pack_kp(L), Q = KP / 4;  // 8-byte chunks per row
+  constexpr int N8 = pack_rows(L) * Q;
+  const uint2* src = (const uint2*)(packed + pack_off(L));
+  for (int i = threadIdx.x; i < N8; i += FNT) {
+    const int row = i / Q, q = i - row * Q;
+    *(uint2*)(s_w + row * lds_kp(L) + 4 * q) = src[i];
   }
 }
 
-// One conv of the fused chain: output region oh x ow, input region
-// (oh+2) x (ow+2) in s_src (pixel-major, SSTR shorts per pixel).  Output
-// pixel (qy, qx) of the region is image pixel (y0 + qy - ring, x0 + qx - ring).
+// Layer L of the fused chain: output region OH x OW, input region
+// (OH+2) x (OW+2) in s_src (pixel-major, SSTR shorts per pixel).  Output
+// pixel (qy, qx) of the region is image pixel (y0 + qy - RING, x0 + qx - RING).
 // Not LAST: GELU, zero outside the image's columns, BF16 into s_dst (pixel
 // stride FCS); LAST: r to HBM ([n][H][W]).
-template <int CIN, int COUT, int SSTR, bool LAST>
+template <int L, int SSTR, int OH, int OW, int RING, bool LAST>
 __device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w, const float* __restrict__ bias,
-                                            short* s_dst, float* __restrict__ out, int oh, int ow, int ring, int y0,
-                                            int x0, int H, int W) {
-  constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16, NS = KP / 16, NTL = (COUT + 15) / 16;
-  const int iw = ow + 2;
+                                            short* s_dst, float* __restrict__ out, int y0, int x0, int H, int W) {
+  constexpr int CIN = dn_cin(L), COUT = dn_cout(L);
+  constexpr int KP = pack_kp(L), NS = KP / 16, NTL = pack_rows(L) / 16;
+  constexpr int IW = OW + 2, NPX = OH * OW, NTILE = (NPX + 15) / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4, kq = g * 4;
   // B operands of this lane for every k-step, kept in registers
@@ -383,39 +417,49 @@ __device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w
 #pragma unroll
   for (int st = 0; st < NS; ++st)
 #pragma unroll
-    for (int j = 0; j < NTL; ++j) bw[st][j] = *(const bf16x4*)(s_w + (j * 16 + m) * KP + st * 16 + kq);
-  const int npx = oh * ow, ntile = (npx + 15) / 16;
-  for (int t = wave; t < ntile; t += FNT / 64) {
-    const int q = min(t * 16 + m, npx - 1);  // this lane's A-row pixel (clamped; its row is discarded)
-    const int qy = q / ow, qx = q - qy * ow;
-    const int base = qy * iw + qx;
+    for (int j = 0; j < NTL; ++j) bw[st][j] = *(const bf16x4*)(s_w + (j * 16 + m) * lds_kp(L) + st * 16 + kq);
+  // CIN = 4: k-step st covers taps 4 st .. 4 st + 3, one per lane group
+  // (taps past 8 are padding with zero weights); CIN = 32: 16 channels of tap
+  // st / 2.  The lane-group part of the offset is added once here.
+  const int lane_off = CIN == 4 ? 0 : kq;
+  for (int t = wave; t < NTILE; t += FNT / 64) {
+    const int q = min(t * 16 + m, NPX - 1);  // this lane's A-row pixel (clamped; its row is discarded)
+    const int qy = q / OW, qx = q - qy * OW;
+    const short* a0 = s_src + (qy * IW + qx) * SSTR + lane_off;
+    // all A operands of the tile first, so the LDS reads are in flight
+    // together instead of one wait per k-step
+    bf16x4 av[NS];
+#pragma unroll
+    for (int st = 0; st < NS; ++st) {
+      if constexpr (CIN == 4) {
+        int tap = st * 4 + g;
+        tap = tap > 8 ? 0 : tap;
+        av[st] = *(const bf16x4*)(a0 + ((tap / 3) * IW + (tap % 3)) * SSTR);
+      } else {
+        constexpr int PER = CIN / 16;
+        const int tap = st / PER, c0 = (st % PER) * 16;
+        av[st] = *(const bf16x4*)(a0 + ((tap / 3) * IW + (tap % 3)) * SSTR + c0);
+      }
+    }
+    // keep the scheduler from sinking each read next to its MFMA (which
+    // would put a full LDS latency before every k-step)
+    __builtin_amdgcn_sched_barrier(0);
     f32x4 acc[NTL];
 #pragma unroll
     for (int j = 0; j < NTL; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int st = 0; st < NS; ++st) {
-      int tap, c0;
-      if constexpr (CIN == 4) {  // k-step st: taps 4 st .. 4 st + 3, one per lane group
-        tap = st * 4 + g;
-        c0 = 0;
-        if (tap > 8) tap = 0;  // padding: zero weights
-      } else {  // k-step st: 16 channels of tap st / (CIN / 16)
-        tap = st / (CIN / 16);
-        c0 = (st % (CIN / 16)) * 16 + kq;
-      }
-      const int off = (tap / 3) * iw + (tap % 3);
-      const bf16x4 a = *(const bf16x4*)(s_src + (base + off) * SSTR + c0);
+    for (int st = 0; st < NS; ++st)
 #pragma unroll
-      for (int j = 0; j < NTL; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[st][j], acc[j], 0, 0, 0);
-    }
-    // this lane's 4 output pixels p0 .. p0 + 3 (ow > 4: at most one row wrap)
+      for (int j = 0; j < NTL; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[st], bw[st][j], acc[j], 0, 0, 0);
+    // this lane's 4 output pixels p0 .. p0 + 3 (OW > 4: at most one row wrap)
     const int p0 = t * 16 + kq;
     int py[4], px[4];
-    py[0] = p0 / ow;
-    px[0] = p0 - py[0] * ow;
+    py[0] = p0 / OW;
+    px[0] = p0 - py[0] * OW;
 #pragma unroll
     for (int i = 1; i < 4; ++i) {
-      const bool wrap = px[i - 1] + 1 == ow;
+      const bool wrap = px[i - 1] + 1 == OW;
       px[i] = wrap ? 0 : px[i - 1] + 1;
       py[i] = py[i - 1] + (wrap ? 1 : 0);
     }
@@ -427,11 +471,11 @@ __device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int p = p0 + i;
-        if (p >= npx) continue;
-        const int x = x0 + px[i] - ring;
+        if (p >= NPX) continue;
+        const int x = x0 + px[i] - RING;
         float v = acc[j][i] + bn;
         if (LAST) {
-          const int y = y0 + py[i] - ring;
+          const int y = y0 + py[i] - RING;
           if (y < H && x >= 0 && x < W) out[((size_t)n * H + y) * W + x] = v;
         } else {
           v = (x < 0 || x >= W) ? 0.0f : gelu(v);  // outside the image (columns); rows wrap
@@ -442,8 +486,9 @@ __device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w
   }
 }
 
-__global__ void __launch_bounds__(FNT) k_dn_fused(const float* __restrict__ x, const float* __restrict__ params,
-                                                  float* __restrict__ r, int H, int W) {
+__global__ void __launch_bounds__(FNT) k_dn_fused(const float* __restrict__ x, DnParams P,
+                                                  const short* __restrict__ packed, float* __restrict__ r, int H,
+                                                  int W) {
   constexpr int H0 = FY + 2 * FRING, W0 = FX + 2 * FRING;
   // layer l's output region: (FY + 2 (4 - l)) x (FX + 2 (4 - l)); ping-pong
   // buffers: A holds the outputs of layers 0 and 2, B those of 1 and 3
@@ -451,39 +496,42 @@ __global__ void __launch_bounds__(FNT) k_dn_fused(const float* __restrict__ x, c
   __shared__ __attribute__((aligned(16))) short s_x[H0 * W0 * DN_IN];
   __shared__ __attribute__((aligned(16))) short s_a[NA * FCS];
   __shared__ __attribute__((aligned(16))) short s_b[NB * FCS];
-  __shared__ __attribute__((aligned(16))) short s_w[DN_CH * DN_CH * 9];
-  const DnParams P = dn_params(const_cast<float*>(params));
+  __shared__ __attribute__((aligned(16))) short s_w[DN_CH * (DN_CH * 9 + 4)];
   const int y0 = blockIdx.y * FY, x0 = blockIdx.x * FX;
   for (int i = threadIdx.x; i < DN_IN * H0 * W0; i += FNT) {
     const int pix = i / DN_IN, c = i - pix * DN_IN;  // pixel-major, channel fastest
     const int ry = pix / W0, rx = pix - ry * W0;
     int y = y0 + ry - FRING;
+    while (y < 0) y += H;  // rows wrap (once, unless H < FY + FRING)
+    while (y >= H) y -= H;
     const int xx = x0 + rx - FRING;
-    y = ((y % H) + H) % H;
     s_x[i] = bf16_bits(xx >= 0 && xx < W ? x[((size_t)c * H + y) * W + xx] : 0.0f);
   }
-  fused_weights<DN_IN, DN_CH>(s_w, P.w[0]);
+  fused_weights<0>(s_w, packed);
   __syncthreads();
-  fused_layer<DN_IN, DN_CH, DN_IN, false>(s_x, s_w, P.b[0], s_a, nullptr, FY + 8, FX + 8, 4, y0, x0, H, W);
-#pragma unroll
-  for (int l = 1; l < DN_LAYERS - 1; ++l) {
-    const int oh = FY + 2 * (4 - l), ow = FX + 2 * (4 - l);
-    const short* src = (l & 1) ? s_a : s_b;
-    short* dst = (l & 1) ? s_b : s_a;
-    __syncthreads();  // the previous layer is done with s_w and has written src
-    fused_weights<DN_CH, DN_CH>(s_w, P.w[l]);
-    __syncthreads();
-    fused_layer<DN_CH, DN_CH, FCS, false>(src, s_w, P.b[l], dst, nullptr, oh, ow, 4 - l, y0, x0, H, W);
-  }
+  fused_layer<0, DN_IN, FY + 8, FX + 8, 4, false>(s_x, s_w, P.b[0], s_a, nullptr, y0, x0, H, W);
+  __syncthreads();  // the layer is done with s_w and has written s_a
+  fused_weights<1>(s_w, packed);
   __syncthreads();
-  fused_weights<DN_CH, DN_IN>(s_w, P.w[DN_LAYERS - 1]);
+  fused_layer<1, FCS, FY + 6, FX + 6, 3, false>(s_a, s_w, P.b[1], s_b, nullptr, y0, x0, H, W);
   __syncthreads();
-  fused_layer<DN_CH, DN_IN, FCS, true>(s_b, s_w, P.b[DN_LAYERS - 1], nullptr, r, FY, FX, 0, y0, x0, H, W);
+  fused_weights<2>(s_w, packed);
+  __syncthreads();
+  fused_layer<2, FCS, FY + 4, FX + 4, 2, false>(s_b, s_w, P.b[2], s_a, nullptr, y0, x0, H, W);
+  __syncthreads();
+  fused_weights<3>(s_w, packed);
+  __syncthreads();
+  fused_layer<3, FCS, FY + 2, FX + 2, 1, false>(s_a, s_w, P.b[3], s_b, nullptr, y0, x0, H, W);
+  __syncthreads();
+  fused_weights<4>(s_w, packed);
+  __syncthreads();
+  fused_layer<4, FCS, FY, FX, 0, true>(s_b, s_w, P.b[4], nullptr, r, y0, x0, H, W);
 }
 
 void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipStream_t st) {
+  hipLaunchKernelGGL(k_dn_pack_weights, dim3((DN_PACK_SHORTS + 255) / 256), dim3(256), 0, st, P, T.packed);
   dim3 grid((W + FX - 1) / FX, (H + FY - 1) / FY);
-  hipLaunchKernelGGL(k_dn_fused, grid, dim3(FNT), 0, st, T.x, P.w[0], T.r, H, W);
+  hipLaunchKernelGGL(k_dn_fused, grid, dim3(FNT), 0, st, T.x, P, (const short*)T.packed, T.r, H, W);
 }
 
 // ---- launchers ----
@@ -558,12 +606,13 @@ void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, in
   const int64_t n4 = (int64_t)4 * nx * ny;
   fill<double>(sums, 0.0, 2, st);
   // reductions: 1024 workgroups, one FP64 atomic each
-  hipLaunchKernelGGL(k_dn_pack, dim3(nblk(n4, 1024)), dim3(256), 0, st, gr, P.scale, 1.0f / (float)dens, T.x, nx,
-                     ny, sums);
+  // (channel, row) pairs strided so that ~1024 workgroups each add one FP64 partial
+  const dim3 pgrid((nx + 255) / 256, (unsigned)std::max(1, std::min(4 * ny, 1024 / ((nx + 255) / 256))));
+  hipLaunchKernelGGL(k_dn_pack, pgrid, dim3(256), 0, st, gr, P.scale, 1.0f / (float)dens, T.x, nx, ny, sums);
   dn_forward_fused(P, T, ny, nx, st);
   hipLaunchKernelGGL(k_dn_sumsq, dim3(nblk(n4, 1024)), dim3(256), 0, st, T.r, n4, sums);
-  hipLaunchKernelGGL(k_dn_apply, dim3(nblk(n4)), dim3(256), 0, st, gr, P.scale, (float)dens, T.x, T.r, nx, ny, sums,
-                     gauss, tol, gauss_tol, chmask, stats);
+  hipLaunchKernelGGL(k_dn_apply, dim3((nx + 255) / 256, (unsigned)std::min(4 * ny, 4096)), dim3(256), 0, st, gr,
+                     P.scale, (float)dens, T.x, T.r, nx, ny, sums, gauss, tol, gauss_tol, chmask, stats);
 }
 
 }  // namespace sp
@@ -651,6 +700,9 @@ int sp_dn_create(int batch, int h, int w, sp_dn** out) {
   rc |= dn_alloc(d, &d->T.dza, sp::DN_CH * px);
   rc |= dn_alloc(d, &d->T.dzb, sp::DN_CH * px);
   rc |= dn_alloc(d, &d->T.wt, sp::DN_CH * sp::DN_CH * 9);
+  float* packed = nullptr;
+  rc |= dn_alloc(d, &packed, sp::DN_PACK_SHORTS / 2);
+  d->T.packed = (short*)packed;
   float* lossf = nullptr;
   rc |= dn_alloc(d, &lossf, 2);
   rc |= dn_alloc(d, &d->lw, 4);

@@ -167,12 +167,16 @@ __host__ __device__ inline DnParams dn_params(float* p) {
 // Activations of B images of H x W: x, t (targets), r, dr (DN_IN channels);
 // z[l] the pre-activations of layers 0..DN_LAYERS-2 and dza, dzb their
 // gradients (DN_CH); wt the flipped weights of one layer (backward).  The
-// simulation's fused path uses x and r only.
+// simulation's fused path uses x, r and packed only.
 struct DnTensors {
   float *x, *t, *r, *dr;
   float* z[DN_LAYERS - 1];
   float *dza, *dzb, *wt;
+  short* packed;  // the fused path's BF16 weights (DN_PACK_SHORTS, 16-byte aligned)
 };
+// BF16 weights of all layers, each [16 ceil(cout/16)][cin*9 rounded up to 16]
+// in tap-major K order, for the fused inference kernel.
+constexpr int DN_PACK_SHORTS = 32 * 48 + 3 * 32 * 288 + 16 * 288;
 void dn_forward(const DnParams& P, const DnTensors& T, int B, int H, int W, bool periodic_y, hipStream_t st);
 void dn_backward(const DnParams& P, DnParams G, const DnTensors& T, const float* lw, int B, int H, int W,
                  double* loss, hipStream_t st);

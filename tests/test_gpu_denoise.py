@@ -145,22 +145,25 @@ def test_in_loop_identity_network_and_fallback():
 
 
 def test_trained_network_gains_over_the_binomial_filter():
-    # Fresh 4-ppc / 64-ppc pairs from a run unlike the training runs (512 x 128
-    # instead of 1024 x 256, gamma0 10, seed 5), sampled over the run as in
-    # training (steps 100 to 1000): the committed network brings the filtered
-    # 4-ppc Jx, Jz (almost pure noise) and rho closer to the 64-ppc ones than
-    # the binomial filter alone, by the factors measured on held-out runs
-    # (about 3x, 15x, 3x).  Jy is the exception: on held-out runs the network
-    # is up to ~1.5x worse than the filter there, so it is only bounded, and
-    # the simulation writes back J only by default in the time-to-solution
-    # study.  Relative errors are summed over the whole run, as the loss is.
+    # Fresh 4-ppc / 64-ppc pairs from a run the network was not trained on: the
+    # training size (1024 x 256) with gamma0 10 and seed 5 (training runs: seeds
+    # 0-5, gamma0 drawn from [2, 20]), sampled over the run as in training
+    # (steps 100 to 1000).  The committed network brings the filtered 4-ppc
+    # currents closer to the 64-ppc ones than the binomial filter alone on every
+    # channel; measured: Jx 0.50x, Jy 0.80x, Jz 0.08x (almost pure noise), rho
+    # 0.55x of the filter's error.  Jy is the weak channel: on another held-out
+    # run (gamma0 12.9) the network was 1.5x worse than the filter there, which
+    # is why the time-to-solution study writes back J only.  Relative errors
+    # are summed over the whole run, as the loss is.
     import os
+    import sys
     from spectral_rs_amd import learned
     p = np.load(os.path.join(os.path.dirname(capi.__file__), "denoiser_shock.npy"))
-    X, T = learned.pairs(512, 128, n=10, t0=100, every=100, seed=5)
+    X, T = learned.pairs(1024, 256, n=8, t0=100, every=125, seed=5)
     ev = learned.evaluate(p, X, T)
+    print("held-out 1024x256 gamma0 10:", ev, file=sys.stderr)
     b, l = ev["binomial_rel_mse"], ev["learned_rel_mse"]
-    assert l[0] < 0.7 * b[0], ev        # jx
-    assert l[1] < 2.0 * b[1], ev        # jy (bounded, not improved)
-    assert l[2] < 0.2 * b[2], ev        # jz
+    assert l[0] < 0.65 * b[0], ev       # jx
+    assert l[1] < 1.0 * b[1], ev        # jy
+    assert l[2] < 0.15 * b[2], ev       # jz
     assert l[3] < 0.7 * b[3], ev        # rho
