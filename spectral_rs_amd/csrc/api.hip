@@ -68,7 +68,7 @@ struct sp_ctx {
   double* gauss_d = nullptr;   // Gauss-law sums of the last solve + block partials
   // learned denoiser (sp_set_denoiser): parameters, input/correction planes, sums, counters
   bool dn_on = false;
-  float* dn_buf = nullptr;  // parameters + packed weights + x (4 planes) + r (4 planes)
+  float* dn_buf = nullptr;  // parameters + packed weights + r (4 planes)
   double* dn_sums = nullptr;
   int* dn_stats = nullptr;
   float dn_tol = 0, dn_gtol = 0;
@@ -309,14 +309,12 @@ template <class T> void diag_sums(sp_ctx* c) {
 }
 
 // Buffers of the in-loop denoiser after the parameters in dn_buf: the packed
-// BF16 weights, its input x and correction r (4 channels each; the fused
-// kernel keeps the rest in LDS).
+// BF16 weights and the correction r (4 channels; the fused kernel reads its
+// input from the grids and keeps the rest in LDS).
 DnTensors dn_tensors(const sp_ctx* c) {
-  const size_t n = c->nspec_elems();
   DnTensors t{};
   t.packed = (short*)(c->dn_buf + DN_NPARAMS);  // 16-byte aligned: DN_NPARAMS % 4 == 0
-  t.x = c->dn_buf + DN_NPARAMS + DN_PACK_SHORTS / 2;
-  t.r = t.x + DN_IN * n;
+  t.r = c->dn_buf + DN_NPARAMS + DN_PACK_SHORTS / 2;
   return t;
 }
 static_assert(DN_NPARAMS % 4 == 0 && DN_PACK_SHORTS % 8 == 0, "alignment of the packed weights");
@@ -1167,7 +1165,7 @@ int sp_set_denoiser(sp_ctx* c, const float* params, int n, float max_rel, float 
   if (n != DN_NPARAMS) return fail(c, SP_EINVAL, "expected SP_DN_NPARAMS parameters");
   if (!(max_rel > 0) || !(max_gauss > 0)) return fail(c, SP_EINVAL, "tolerances must be positive");
   if (!c->dn_buf) {
-    const size_t planes = 2 * DN_IN * c->nspec_elems();
+    const size_t planes = DN_IN * c->nspec_elems();
     HIPCHK(c, hipMalloc(&c->dn_buf, (DN_NPARAMS + DN_PACK_SHORTS / 2 + planes) * sizeof(float)));
     HIPCHK(c, hipMalloc(&c->dn_sums, 2 * sizeof(double)));
     HIPCHK(c, hipMalloc(&c->dn_stats, 2 * sizeof(int)));

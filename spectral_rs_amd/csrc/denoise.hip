@@ -31,6 +31,27 @@ namespace sp {
 
 typedef __attribute__((ext_vector_type(4))) short bf16x4;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+
+// Forward convolutions run on v_mfma_f32_16x16x32_bf16 (gfx950): K steps of
+// 32, lane l holding k = 8 (l / 16) + i, i < 8, of row / column l % 16 of A /
+// B; D as for the 16x16x16 form (rows 4 (l / 16) + i, column l % 16).
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ __bf16 as_bf(short v) { return __builtin_bit_cast(__bf16, v); }
+// two 8-byte LDS reads (8-byte aligned) as one 8-element operand
+__device__ __forceinline__ bf16x8 ld8(const short* p0, const short* p1) {
+  const bf16x4 lo = *(const bf16x4*)p0, hi = *(const bf16x4*)p1;
+  bf16x8 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    r[i] = as_bf(lo[i]);
+    r[4 + i] = as_bf(hi[i]);
+  }
+  return r;
+}
 
 // FP32 -> BF16 bits, round to nearest even (gfx950's v_cvt_pk_bf16_f32).
 __device__ __forceinline__ short bf16_bits(float f) { return __builtin_bit_cast(short, (__bf16)f); }
@@ -40,9 +61,11 @@ __device__ __forceinline__ short bf16_bits(float f) { return __builtin_bit_cast(
 // reciprocal (about 1 ulp each): a handful of instructions, where erf costs
 // tens; it runs on every activation of every layer.
 constexpr float GELU_C = 0.7978845608028654f, GELU_A = 0.044715f, LOG2E = 1.4426950408889634f;
+// -2 u log2(e) = z (K1 + K3 z^2): one multiply, one fma, one multiply
+constexpr float GELU_K1 = -2.0f * LOG2E * GELU_C, GELU_K3 = GELU_K1 * GELU_A;
 __device__ __forceinline__ float gelu_sig(float z) {  // sigmoid(2u)
-  const float u = GELU_C * z * (1.0f + GELU_A * z * z);
-  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-2.0f * LOG2E * u));
+  const float w = z * __builtin_fmaf(GELU_K3, z * z, GELU_K1);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(w));
 }
 __device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
 // d GELU / dz = s + z s (1 - s) 2 u', u' = sqrt(2/pi) (1 + 3 * 0.044715 z^2)
@@ -54,7 +77,8 @@ __device__ __forceinline__ float gelu_grad(float z) {
 constexpr int DN_TY = 8, DN_TX = 32, DN_NT = 256;  // output pixels of a workgroup: 8 rows x 32 columns
 
 template <int CIN> struct ConvShape {
-  static constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16, LY = DN_TY + 2, LX = DN_TX + 2;
+  // KP: K padded for the 16-wide weight-gradient steps; KF: for the 32-wide forward steps
+  static constexpr int K = CIN * 9, KP = (K + 15) / 16 * 16, KF = (K + 31) / 32 * 32, LY = DN_TY + 2, LX = DN_TX + 2;
 };
 
 // Stages rows y0-1 .. y0+DN_TY and columns x0-1 .. x0+DN_TX of CIN planes of
@@ -83,7 +107,7 @@ __device__ __forceinline__ void stage_tile(short* s_in, const float* __restrict_
 // same order and k-step grouping, so both paths sum identically.
 template <int CIN> __device__ __forceinline__ void stage_offsets_tm(int* s_off) {
   using S = ConvShape<CIN>;
-  for (int kk = threadIdx.x; kk < S::KP; kk += DN_NT) {
+  for (int kk = threadIdx.x; kk < S::KF; kk += DN_NT) {
     int o = 0;  // padded taps read any valid element: their weight is zero
     if (kk < S::K) {
       const int t = kk / CIN, ci = kk - t * CIN;
@@ -122,18 +146,18 @@ __global__ void __launch_bounds__(DN_NT) k_conv3x3(const float* __restrict__ in,
   using S = ConvShape<CIN>;
   constexpr int NTL = (COUT + 15) / 16;  // 16-channel output tiles
   __shared__ short s_in[CIN * S::LY * S::LX];
-  __shared__ short s_w[NTL * 16 * S::KP];
-  __shared__ int s_off[S::KP];
+  __shared__ short s_w[NTL * 16 * S::KF];
+  __shared__ int s_off[S::KF];
   const int b = blockIdx.z, y0 = blockIdx.y * DN_TY, x0 = blockIdx.x * DN_TX;
   stage_tile<CIN, PERIODIC_Y, GIN>(s_in, in + (size_t)b * CIN * H * W, H, W, y0, x0);
-  for (int i = threadIdx.x; i < NTL * 16 * S::KP; i += DN_NT) {
-    const int n = i / S::KP, kk = i - n * S::KP;
+  for (int i = threadIdx.x; i < NTL * 16 * S::KF; i += DN_NT) {
+    const int n = i / S::KF, kk = i - n * S::KF;
     s_w[i] = (n < COUT && kk < S::K) ? bf16_bits(w[n * S::K + tm_col<CIN>(kk)]) : (short)0;
   }
   stage_offsets_tm<CIN>(s_off);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m = lane & 15, kq = (lane >> 4) * 4;
+  const int m = lane & 15, kq = (lane >> 4) * 4, k8 = (lane >> 4) * 8;
   for (int t = wave; t < (DN_TY * DN_TX) / 16; t += DN_NT / 64) {
     const int py = t / (DN_TX / 16), px = (t % (DN_TX / 16)) * 16;
     const int pbase = py * S::LX + px + m;  // window corner of pixel (py, px + m) in the staged tile
@@ -141,17 +165,17 @@ __global__ void __launch_bounds__(DN_NT) k_conv3x3(const float* __restrict__ in,
 #pragma unroll
     for (int j = 0; j < NTL; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kc = 0; kc < S::KP; kc += 16) {
-      bf16x4 a, bv[NTL];
+    for (int kc = 0; kc < S::KF; kc += 32) {
+      bf16x8 a, bv[NTL];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int kk = kc + kq + i;
-        a[i] = s_in[pbase + s_off[kk]];
+      for (int i = 0; i < 8; ++i) {
+        const int kk = kc + k8 + i;
+        a[i] = as_bf(s_in[pbase + s_off[kk]]);
 #pragma unroll
-        for (int j = 0; j < NTL; ++j) bv[j][i] = s_w[(j * 16 + m) * S::KP + kk];
+        for (int j = 0; j < NTL; ++j) bv[j][i] = as_bf(s_w[(j * 16 + m) * S::KF + kk]);
       }
 #pragma unroll
-      for (int j = 0; j < NTL; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[j], acc[j], 0, 0, 0);
+      for (int j = 0; j < NTL; ++j) acc[j] = mfma32(a, bv[j], acc[j]);
     }
 #pragma unroll
     for (int j = 0; j < NTL; ++j) {
@@ -276,54 +300,16 @@ struct Grids4 {
   float* g[4];
 };
 
-// Sum of v over the workgroup, added to *out by one atomic (256 threads).
-__device__ __forceinline__ void block_add(double v, double* out) {
-  __shared__ double red[4];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(out, ((red[0] + red[1]) + red[2]) + red[3]);
-}
-
-// x[c][y][x] = grid_c(interior) / (dens s_c); sums |x|^2 into sums[0].
-// Grid: x over the columns, y over (channel, row) pairs with a stride.
-__global__ void __launch_bounds__(256) k_dn_pack(Grids4 gr, const float* __restrict__ scale, float inv_dens,
-                                                 float* __restrict__ x, int nx, int ny, double* sums) {
-  const int xx = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc = 0.0;
-  if (xx < nx) {
-    for (int cr = blockIdx.y; cr < 4 * ny; cr += gridDim.y) {
-      const int c = cr / ny, yy = cr - c * ny;
-      const float v = gr.g[c][(int64_t)(yy + 1) * (nx + 2) + xx + 1] * inv_dens / scale[c];
-      x[(int64_t)cr * nx + xx] = v;
-      acc += (double)v * v;
-    }
-  }
-  block_add(acc, sums);
-}
-
-__global__ void __launch_bounds__(256) k_dn_sumsq(const float* __restrict__ r, int64_t n, double* sums) {
-  double acc = 0.0;
-  bool finite = true;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const float v = r[i];
-    finite = finite && isfinite(v);
-    acc += (double)v * v;
-  }
-  if (!finite) acc = __builtin_inf();
-  block_add(acc, sums + 1);
-}
-
-// Writes dens s_c (x + r) into the interior of grid c (for the channels set
-// in chmask) when the correction
-// passes the checks: |r| <= tol |x| (finite), and the previous solve's
-// pre-correction Gauss residual (gauss[0] / gauss[2], relative) is below
-// gauss_tol.  Otherwise the binomial-filtered grids stay (the fallback).
-// stats[0] counts applications, stats[1] fallbacks.
-__global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float dens, const float* __restrict__ x,
-                           const float* __restrict__ r, int nx, int ny, const double* __restrict__ sums,
-                           const double* __restrict__ gauss, float tol, float gauss_tol, int chmask, int* stats) {
+// Adds dens s_c r to the interior of grid c (for the channels set in chmask)
+// when the correction passes the checks: |r| <= tol |x| and r finite (sums,
+// reduced by the fused kernel), and the previous solve's pre-correction
+// Gauss residual (gauss[0] / gauss[2], relative) is below gauss_tol.
+// Otherwise the binomial-filtered grids stay (the fallback).  stats[0]
+// counts applications, stats[1] fallbacks.  Grid: x over the columns, y over
+// (channel, row) pairs with a stride.
+__global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float dens, const float* __restrict__ r,
+                           int nx, int ny, const double* __restrict__ sums, const double* __restrict__ gauss, float tol,
+                           float gauss_tol, int chmask, int* stats) {
   const double ratio2 = sums[0] > 0 ? sums[1] / sums[0] : 0.0;
   const double g2 = gauss[2] > 0 ? gauss[0] / gauss[2] : 0.0;
   const bool ok = ratio2 <= (double)tol * tol && g2 <= (double)gauss_tol * gauss_tol;
@@ -331,11 +317,11 @@ __global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float den
   if (!ok) return;
   const int xx = blockIdx.x * blockDim.x + threadIdx.x;
   if (xx >= nx) return;
-  for (int cr = blockIdx.y; cr < 4 * ny; cr += gridDim.y) {  // (channel, row) pairs, as k_dn_pack
+  for (int cr = blockIdx.y; cr < 4 * ny; cr += gridDim.y) {
     const int c = cr / ny, yy = cr - c * ny;
     if (!((chmask >> c) & 1)) continue;
-    const int64_t i = (int64_t)cr * nx + xx;
-    gr.g[c][(int64_t)(yy + 1) * (nx + 2) + xx + 1] = dens * scale[c] * (x[i] + r[i]);
+    float* g = gr.g[c] + (int64_t)(yy + 1) * (nx + 2) + xx + 1;
+    *g = *g + dens * scale[c] * r[(int64_t)cr * nx + xx];
   }
 }
 
@@ -346,8 +332,9 @@ __global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float den
 // pixel-major with the channels of a pixel contiguous (stride FCS, padded so
 // that the 16 lanes of a group read distinct banks), in two ping-pong
 // buffers.  The K order is tap-major (kk = tap * CIN + ci), so a lane's A
-// operand -- 4 consecutive channels of one tap of one pixel -- is one 8-byte
-// LDS read at a compile-time offset from the lane's pixel, and each wave
+// operand -- 8 consecutive channels of one tap of one pixel (two taps of 4
+// in the first layer) -- is two 8-byte LDS reads at compile-time offsets
+// from the lane's pixel, and each wave
 // keeps the layer's B operands (the weights) in registers for all its pixel
 // tiles.  The weights come pre-converted (BF16, tap-major, zero-padded:
 // dn_pack_weights) and are copied to LDS with 16-byte loads.  Activations of
@@ -359,7 +346,7 @@ constexpr int FNT = 768;                             // 12 waves: 3 per SIMD (<=
 constexpr int FCS = DN_CH + 4;                       // LDS pixel stride of 32-channel activations (72 B)
 
 // Packed layer l: [16 NTL][KP] BF16 in tap-major K order.
-__host__ __device__ constexpr int pack_kp(int l) { return (dn_cin(l) * 9 + 15) / 16 * 16; }
+__host__ __device__ constexpr int pack_kp(int l) { return (dn_cin(l) * 9 + 31) / 32 * 32; }
 __host__ __device__ constexpr int pack_rows(int l) { return (dn_cout(l) + 15) / 16 * 16; }
 __host__ __device__ constexpr int pack_off(int l) {
   int o = 0;
@@ -389,13 +376,28 @@ __global__ void k_dn_pack_weights(DnParams P, short* __restrict__ out) {
 // of 16, would put every second or fourth row in the same bank).
 __host__ __device__ constexpr int lds_kp(int l) { return pack_kp(l) + 4; }
 
-template <int L> __device__ __forceinline__ void fused_weights(short* s_w, const short* __restrict__ packed) {
-  constexpr int KP = pack_kp(L), Q = KP / 4;  // 8-byte chunks per row
-  constexpr int N8 = pack_rows(L) * Q;
+// A layer's packed weights move global -> registers (weights_fetch, issued
+// early so that the latency hides behind the previous layer's MFMAs) ->
+// LDS rows (weights_store).  At most WREG 8-byte chunks per thread.
+constexpr int WREG = (DN_CH * (DN_CH * 9 / 4) + FNT - 1) / FNT;
+template <int L> __device__ __forceinline__ void weights_fetch(uint2 (&reg)[WREG], const short* __restrict__ packed) {
+  constexpr int Q = pack_kp(L) / 4, N8 = pack_rows(L) * Q;
   const uint2* src = (const uint2*)(packed + pack_off(L));
-  for (int i = threadIdx.x; i < N8; i += FNT) {
-    const int row = i / Q, q = i - row * Q;
-    *(uint2*)(s_w + row * lds_kp(L) + 4 * q) = src[i];
+#pragma unroll
+  for (int k = 0; k < WREG; ++k) {
+    const int i = threadIdx.x + k * FNT;
+    if (i < N8) reg[k] = src[i];
+  }
+}
+template <int L> __device__ __forceinline__ void weights_store(const uint2 (&reg)[WREG], short* s_w) {
+  constexpr int Q = pack_kp(L) / 4, N8 = pack_rows(L) * Q;
+#pragma unroll
+  for (int k = 0; k < WREG; ++k) {
+    const int i = threadIdx.x + k * FNT;
+    if (i < N8) {
+      const int row = i / Q, q = i - row * Q;
+      *(uint2*)(s_w + row * lds_kp(L) + 4 * q) = reg[k];
+    }
   }
 }
 
@@ -403,42 +405,55 @@ template <int L> __device__ __forceinline__ void fused_weights(short* s_w, const
 // (OH+2) x (OW+2) in s_src (pixel-major, SSTR shorts per pixel).  Output
 // pixel (qy, qx) of the region is image pixel (y0 + qy - RING, x0 + qx - RING).
 // Not LAST: GELU, zero outside the image's columns, BF16 into s_dst (pixel
-// stride FCS); LAST: r to HBM ([n][H][W]).
+// stride FCS); LAST: r to HBM ([n][H][W]) and sum r^2 into *rsq.
+// The layer first takes its B operands from s_w into registers; after a
+// barrier s_w is free, so it fetches the next layer's weights before its
+// tiles and stores them into s_w after them (the caller's barrier follows).
 template <int L, int SSTR, int OH, int OW, int RING, bool LAST>
-__device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w, const float* __restrict__ bias,
-                                            short* s_dst, float* __restrict__ out, int y0, int x0, int H, int W) {
+__device__ __forceinline__ void fused_layer(const short* s_src, short* s_w, const float* __restrict__ bias,
+                                            short* s_dst, float* __restrict__ out, int y0, int x0, int H, int W,
+                                            const short* __restrict__ packed, double* rsq = nullptr) {
   constexpr int CIN = dn_cin(L), COUT = dn_cout(L);
-  constexpr int KP = pack_kp(L), NS = KP / 16, NTL = pack_rows(L) / 16;
+  constexpr int KP = pack_kp(L), NS = KP / 32, NTL = pack_rows(L) / 16;
   constexpr int IW = OW + 2, NPX = OH * OW, NTILE = (NPX + 15) / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m = lane & 15, g = lane >> 4, kq = g * 4;
+  const int m = lane & 15, g = lane >> 4, kq = g * 4, k8 = g * 8;
   // B operands of this lane for every k-step, kept in registers
-  bf16x4 bw[NS][NTL];
+  bf16x8 bw[NS][NTL];
 #pragma unroll
   for (int st = 0; st < NS; ++st)
 #pragma unroll
-    for (int j = 0; j < NTL; ++j) bw[st][j] = *(const bf16x4*)(s_w + (j * 16 + m) * lds_kp(L) + st * 16 + kq);
-  // CIN = 4: k-step st covers taps 4 st .. 4 st + 3, one per lane group
-  // (taps past 8 are padding with zero weights); CIN = 32: 16 channels of tap
-  // st / 2.  The lane-group part of the offset is added once here.
-  const int lane_off = CIN == 4 ? 0 : kq;
+    for (int j = 0; j < NTL; ++j) {
+      const short* row = s_w + (j * 16 + m) * lds_kp(L) + st * 32 + k8;
+      bw[st][j] = ld8(row, row + 4);
+    }
+  uint2 wnext[WREG];
+  if constexpr (L + 1 < DN_LAYERS) {
+    __syncthreads();  // every wave holds its B operands: s_w is free
+    weights_fetch<L + 1>(wnext, packed);
+    __builtin_amdgcn_sched_barrier(0);  // keep the fetch ahead of the tiles
+  }
+  // CIN = 4: k-step st covers taps 8 st .. 8 st + 7, two per lane group
+  // (taps past 8 are padding with zero weights); CIN = 32: the 32 channels of
+  // tap st, 8 per lane group (added to the lane's base once here).
+  const int lane_off = CIN == 4 ? 0 : k8;
   for (int t = wave; t < NTILE; t += FNT / 64) {
     const int q = min(t * 16 + m, NPX - 1);  // this lane's A-row pixel (clamped; its row is discarded)
     const int qy = q / OW, qx = q - qy * OW;
     const short* a0 = s_src + (qy * IW + qx) * SSTR + lane_off;
     // all A operands of the tile first, so the LDS reads are in flight
     // together instead of one wait per k-step
-    bf16x4 av[NS];
+    bf16x8 av[NS];
 #pragma unroll
     for (int st = 0; st < NS; ++st) {
       if constexpr (CIN == 4) {
-        int tap = st * 4 + g;
-        tap = tap > 8 ? 0 : tap;
-        av[st] = *(const bf16x4*)(a0 + ((tap / 3) * IW + (tap % 3)) * SSTR);
+        int t0 = st * 8 + 2 * g, t1 = t0 + 1;
+        t0 = t0 > 8 ? 0 : t0;
+        t1 = t1 > 8 ? 0 : t1;
+        av[st] = ld8(a0 + ((t0 / 3) * IW + (t0 % 3)) * SSTR, a0 + ((t1 / 3) * IW + (t1 % 3)) * SSTR);
       } else {
-        constexpr int PER = CIN / 16;
-        const int tap = st / PER, c0 = (st % PER) * 16;
-        av[st] = *(const bf16x4*)(a0 + ((tap / 3) * IW + (tap % 3)) * SSTR + c0);
+        const short* pa = a0 + ((st / 3) * IW + (st % 3)) * SSTR;
+        av[st] = ld8(pa, pa + 4);
       }
     }
     // keep the scheduler from sinking each read next to its MFMA (which
@@ -451,7 +466,7 @@ __device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w
     for (int st = 0; st < NS; ++st)
 #pragma unroll
       for (int j = 0; j < NTL; ++j)
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[st], bw[st][j], acc[j], 0, 0, 0);
+        acc[j] = mfma32(av[st], bw[st][j], acc[j]);
     // this lane's 4 output pixels p0 .. p0 + 3 (OW > 4: at most one row wrap)
     const int p0 = t * 16 + kq;
     int py[4], px[4];
@@ -476,19 +491,58 @@ __device__ __forceinline__ void fused_layer(const short* s_src, const short* s_w
         float v = acc[j][i] + bn;
         if (LAST) {
           const int y = y0 + py[i] - RING;
-          if (y < H && x >= 0 && x < W) out[((size_t)n * H + y) * W + x] = v;
+          if (y < H && x >= 0 && x < W) {
+            out[((size_t)n * H + y) * W + x] = v;
+            *rsq += isfinite(v) ? (double)v * v : __builtin_inf();
+          }
         } else {
           v = (x < 0 || x >= W) ? 0.0f : gelu(v);  // outside the image (columns); rows wrap
-          s_dst[p * FCS + n] = bf16_bits(v);
+          // channels n and n + 1 sit in lanes m and m ^ 1 (same pixel, same
+          // active state): the even lane converts both and writes one dword
+          const float w = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF,
+                                                                             0xF, false));  // quad_perm [1,0,3,2]
+          if ((m & 1) == 0) {
+            bf16x2 pk;
+            pk[0] = (__bf16)v;
+            pk[1] = (__bf16)w;
+            *(bf16x2*)(s_dst + p * FCS + n) = pk;
+          }
         }
       }
     }
   }
+  if constexpr (L + 1 < DN_LAYERS) weights_store<L + 1>(wnext, s_w);
 }
 
-__global__ void __launch_bounds__(FNT) k_dn_fused(const float* __restrict__ x, DnParams P,
-                                                  const short* __restrict__ packed, float* __restrict__ r, int H,
-                                                  int W) {
+// Input of the fused kernel: planes x [4][H][W] (GRIDS false), or the
+// simulation's padded grids, read as grid / dens / s_c (GRIDS true); then the
+// kernel also adds sum x^2 over its own pixels to sums[0] and sum r^2 to
+// sums[1] (infinity when r is not finite), for the application check.
+struct FusedIn {
+  const float* x;
+  Grids4 gr;
+  const float* scale;
+  float inv_dens;
+  double* sums;
+};
+
+// FP64 sum over the workgroup (FNT threads), added to *out by one atomic.
+__device__ __forceinline__ void fused_block_add(double v, double* red, double* out) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  __syncthreads();  // red may still be read by an earlier call
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < FNT / 64; ++w) t += red[w];
+    atomicAdd(out, t);
+  }
+}
+
+template <bool GRIDS>
+__global__ void __launch_bounds__(FNT) k_dn_fused(FusedIn in, DnParams P, const short* __restrict__ packed,
+                                                  float* __restrict__ r, int H, int W) {
   constexpr int H0 = FY + 2 * FRING, W0 = FX + 2 * FRING;
   // layer l's output region: (FY + 2 (4 - l)) x (FX + 2 (4 - l)); ping-pong
   // buffers: A holds the outputs of layers 0 and 2, B those of 1 and 3
@@ -497,41 +551,55 @@ __global__ void __launch_bounds__(FNT) k_dn_fused(const float* __restrict__ x, D
   __shared__ __attribute__((aligned(16))) short s_a[NA * FCS];
   __shared__ __attribute__((aligned(16))) short s_b[NB * FCS];
   __shared__ __attribute__((aligned(16))) short s_w[DN_CH * (DN_CH * 9 + 4)];
+  __shared__ double s_red[FNT / 64];
   const int y0 = blockIdx.y * FY, x0 = blockIdx.x * FX;
+  double xsq = 0.0;
   for (int i = threadIdx.x; i < DN_IN * H0 * W0; i += FNT) {
     const int pix = i / DN_IN, c = i - pix * DN_IN;  // pixel-major, channel fastest
     const int ry = pix / W0, rx = pix - ry * W0;
-    int y = y0 + ry - FRING;
+    const int yu = y0 + ry - FRING;
+    int y = yu;
     while (y < 0) y += H;  // rows wrap (once, unless H < FY + FRING)
     while (y >= H) y -= H;
     const int xx = x0 + rx - FRING;
-    s_x[i] = bf16_bits(xx >= 0 && xx < W ? x[((size_t)c * H + y) * W + xx] : 0.0f);
+    float v = 0.0f;
+    if (xx >= 0 && xx < W) {
+      if constexpr (GRIDS) {
+        v = in.gr.g[c][(size_t)(y + 1) * (W + 2) + xx + 1] * in.inv_dens / in.scale[c];
+        const bool own = ry >= FRING && ry < FRING + FY && rx >= FRING && rx < FRING + FX && yu < H;
+        if (own) xsq += (double)v * v;
+      } else {
+        v = in.x[((size_t)c * H + y) * W + xx];
+      }
+    }
+    s_x[i] = bf16_bits(v);
   }
-  fused_weights<0>(s_w, packed);
+  if constexpr (GRIDS) fused_block_add(xsq, s_red, in.sums);
+  {
+    uint2 w0[WREG];
+    weights_fetch<0>(w0, packed);
+    weights_store<0>(w0, s_w);
+  }
   __syncthreads();
-  fused_layer<0, DN_IN, FY + 8, FX + 8, 4, false>(s_x, s_w, P.b[0], s_a, nullptr, y0, x0, H, W);
-  __syncthreads();  // the layer is done with s_w and has written s_a
-  fused_weights<1>(s_w, packed);
+  fused_layer<0, DN_IN, FY + 8, FX + 8, 4, false>(s_x, s_w, P.b[0], s_a, nullptr, y0, x0, H, W, packed);
+  __syncthreads();  // layer 0 is done: s_a written, layer 1's weights in s_w
+  fused_layer<1, FCS, FY + 6, FX + 6, 3, false>(s_a, s_w, P.b[1], s_b, nullptr, y0, x0, H, W, packed);
   __syncthreads();
-  fused_layer<1, FCS, FY + 6, FX + 6, 3, false>(s_a, s_w, P.b[1], s_b, nullptr, y0, x0, H, W);
+  fused_layer<2, FCS, FY + 4, FX + 4, 2, false>(s_b, s_w, P.b[2], s_a, nullptr, y0, x0, H, W, packed);
   __syncthreads();
-  fused_weights<2>(s_w, packed);
+  fused_layer<3, FCS, FY + 2, FX + 2, 1, false>(s_a, s_w, P.b[3], s_b, nullptr, y0, x0, H, W, packed);
   __syncthreads();
-  fused_layer<2, FCS, FY + 4, FX + 4, 2, false>(s_b, s_w, P.b[2], s_a, nullptr, y0, x0, H, W);
-  __syncthreads();
-  fused_weights<3>(s_w, packed);
-  __syncthreads();
-  fused_layer<3, FCS, FY + 2, FX + 2, 1, false>(s_a, s_w, P.b[3], s_b, nullptr, y0, x0, H, W);
-  __syncthreads();
-  fused_weights<4>(s_w, packed);
-  __syncthreads();
-  fused_layer<4, FCS, FY, FX, 0, true>(s_b, s_w, P.b[4], nullptr, r, y0, x0, H, W);
+  double rsq = 0.0;
+  fused_layer<4, FCS, FY, FX, 0, true>(s_b, s_w, P.b[4], nullptr, r, y0, x0, H, W, packed, &rsq);
+  if constexpr (GRIDS) fused_block_add(rsq, s_red, in.sums + 1);
 }
 
 void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipStream_t st) {
   hipLaunchKernelGGL(k_dn_pack_weights, dim3((DN_PACK_SHORTS + 255) / 256), dim3(256), 0, st, P, T.packed);
+  FusedIn in{};
+  in.x = T.x;
   dim3 grid((W + FX - 1) / FX, (H + FY - 1) / FY);
-  hipLaunchKernelGGL(k_dn_fused, grid, dim3(FNT), 0, st, T.x, P, (const short*)T.packed, T.r, H, W);
+  hipLaunchKernelGGL((k_dn_fused<false>), grid, dim3(FNT), 0, st, in, P, (const short*)T.packed, T.r, H, W);
 }
 
 // ---- launchers ----
@@ -601,18 +669,17 @@ void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, in
 
 void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, int nx, int ny, int dens, double* sums,
                   const double* gauss, float tol, float gauss_tol, int chmask, int* stats, hipStream_t st) {
-  Grids4 gr;
-  for (int c = 0; c < 4; ++c) gr.g[c] = grids[c];
-  const int64_t n4 = (int64_t)4 * nx * ny;
+  FusedIn in{};
+  for (int c = 0; c < 4; ++c) in.gr.g[c] = grids[c];
+  in.scale = P.scale;
+  in.inv_dens = 1.0f / (float)dens;
+  in.sums = sums;
   fill<double>(sums, 0.0, 2, st);
-  // reductions: 1024 workgroups, one FP64 atomic each
-  // (channel, row) pairs strided so that ~1024 workgroups each add one FP64 partial
-  const dim3 pgrid((nx + 255) / 256, (unsigned)std::max(1, std::min(4 * ny, 1024 / ((nx + 255) / 256))));
-  hipLaunchKernelGGL(k_dn_pack, pgrid, dim3(256), 0, st, gr, P.scale, 1.0f / (float)dens, T.x, nx, ny, sums);
-  dn_forward_fused(P, T, ny, nx, st);
-  hipLaunchKernelGGL(k_dn_sumsq, dim3(nblk(n4, 1024)), dim3(256), 0, st, T.r, n4, sums);
-  hipLaunchKernelGGL(k_dn_apply, dim3((nx + 255) / 256, (unsigned)std::min(4 * ny, 4096)), dim3(256), 0, st, gr,
-                     P.scale, (float)dens, T.x, T.r, nx, ny, sums, gauss, tol, gauss_tol, chmask, stats);
+  hipLaunchKernelGGL(k_dn_pack_weights, dim3((DN_PACK_SHORTS + 255) / 256), dim3(256), 0, st, P, T.packed);
+  dim3 grid((nx + FX - 1) / FX, (ny + FY - 1) / FY);
+  hipLaunchKernelGGL((k_dn_fused<true>), grid, dim3(FNT), 0, st, in, P, (const short*)T.packed, T.r, ny, nx);
+  hipLaunchKernelGGL(k_dn_apply, dim3((nx + 255) / 256, (unsigned)std::min(4 * ny, 4096)), dim3(256), 0, st, in.gr,
+                     P.scale, (float)dens, T.r, nx, ny, sums, gauss, tol, gauss_tol, chmask, stats);
 }
 
 }  // namespace sp

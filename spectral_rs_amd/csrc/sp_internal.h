@@ -167,16 +167,16 @@ __host__ __device__ inline DnParams dn_params(float* p) {
 // Activations of B images of H x W: x, t (targets), r, dr (DN_IN channels);
 // z[l] the pre-activations of layers 0..DN_LAYERS-2 and dza, dzb their
 // gradients (DN_CH); wt the flipped weights of one layer (backward).  The
-// simulation's fused path uses x, r and packed only.
+// simulation's fused path uses r and packed only.
 struct DnTensors {
   float *x, *t, *r, *dr;
   float* z[DN_LAYERS - 1];
   float *dza, *dzb, *wt;
   short* packed;  // the fused path's BF16 weights (DN_PACK_SHORTS, 16-byte aligned)
 };
-// BF16 weights of all layers, each [16 ceil(cout/16)][cin*9 rounded up to 16]
+// BF16 weights of all layers, each [16 ceil(cout/16)][cin*9 rounded up to 32]
 // in tap-major K order, for the fused inference kernel.
-constexpr int DN_PACK_SHORTS = 32 * 48 + 3 * 32 * 288 + 16 * 288;
+constexpr int DN_PACK_SHORTS = 32 * 64 + 3 * 32 * 288 + 16 * 288;
 void dn_forward(const DnParams& P, const DnTensors& T, int B, int H, int W, bool periodic_y, hipStream_t st);
 void dn_backward(const DnParams& P, DnParams G, const DnTensors& T, const float* lw, int B, int H, int W,
                  double* loss, hipStream_t st);
@@ -184,9 +184,10 @@ void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, in
 // All layers of one image [4][H][W] in one launch, y periodic (the
 // simulation's inference path); same results as dn_forward(periodic_y).
 void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipStream_t st);
-// Packs the 4 filtered grids, runs the fused network, and when the checks
-// pass writes the corrected planes back for the channels set in chmask (bit c
-// = grid c of Jx Jy Jz rho); stats[0] counts applications, stats[1] fallbacks.
+// Runs the fused network on the 4 filtered grids (read as grid / dens / s_c)
+// and, when the checks pass, adds dens s_c r to the grids for the channels
+// set in chmask (bit c = grid c of Jx Jy Jz rho); T.packed and T.r are its
+// buffers; stats[0] counts applications, stats[1] fallbacks.
 void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, int nx, int ny, int dens, double* sums,
                   const double* gauss, float tol, float gauss_tol, int chmask, int* stats, hipStream_t st);
 
