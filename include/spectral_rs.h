@@ -241,7 +241,8 @@ int sp_ensemble_stats(const sp_ensemble* e, int* kept, int* captures);
 
 /* ---- Learned current denoiser (no reference counterpart) ----
  * Five 3x3 convolutional layers (4 -> 32 -> 32 -> 32 -> 32 -> 4 channels,
- * GELU after each but the last) on BF16 MFMA with FP32 accumulation.  Input
+ * GELU in its tanh form after each but the last) on BF16 MFMA with FP32
+ * accumulation.  Input
  * x: Jx Jy Jz rho after the binomial filter, each divided by dens and a
  * channel scale; output x + r (residual).  Parameters, one FP32 vector of
  * SP_DN_NPARAMS: for each layer l = 0..4, w_l [cout][cin*9] then b_l [cout];
