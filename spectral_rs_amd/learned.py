@@ -230,6 +230,8 @@ def main():
     ap.add_argument("--iters", type=int, default=12000)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval-seeds", default="0,1", help="runs per case; profiles are averaged over them")
+    ap.add_argument("--cases", default="plain_64ppc,plain_4ppc,learned",
+                    help="cases of the time-to-solution study to run (plain_64ppc, plain_4ppc, learned)")
     ap.add_argument("--dn-masks", default="7",
                     help="channel masks of the learned cases (bit c: Jx Jy Jz rho written back)")
     ap.add_argument("--out", default=None, help="JSON report")
@@ -267,9 +269,11 @@ def main():
     d = delta_for(ex)
     eseeds = [int(v) for v in a.eval_seeds.split(",")]
     masks = [int(v) for v in a.dn_masks.split(",")]
-    cases = {"plain_64ppc": (64, None, 0), "plain_4ppc": (4, None, 0)}
-    for mk in masks:
-        cases[f"learned_4ppc_mask{mk}"] = (4, p, mk)
+    want = set(a.cases.split(","))
+    cases = {k: v for k, v in {"plain_64ppc": (64, None, 0), "plain_4ppc": (4, None, 0)}.items() if k in want}
+    if "learned" in want:
+        for mk in masks:
+            cases[f"learned_4ppc_mask{mk}"] = (4, p, mk)
     res = {}
     for name, (ppc, prm, mk) in cases.items():
         runs = [run_case(ex, ey, ppc, a.steps, params=prm, seed=sd, mask=mk) for sd in eseeds]
@@ -277,12 +281,13 @@ def main():
                          stats=[r[2] for r in runs],
                          spec={n: np.mean([r[3][n] for r in runs], axis=0) for n in ("ex", "bz")})
         print(f"time_to_solution: {name} {res[name]['t']:.3f} s per run", file=sys.stderr, flush=True)
-    truth = res["plain_64ppc"]
+    truth = res.get("plain_64ppc")
     tts = {"eval_size": a.eval_size, "steps": a.steps, "seeds": eseeds, "cases": {}}
     for case in cases:
         r = res[case]
-        e = {"s_per_run": r["t"], "speedup_vs_plain_64ppc": truth["t"] / r["t"]}
-        if case != "plain_64ppc":
+        e = {"s_per_run": r["t"]}
+        if truth is not None and case != "plain_64ppc":
+            e["speedup_vs_plain_64ppc"] = truth["t"] / r["t"]
             e["profile_rel_err"] = profile_error(r["prof"], truth["prof"], d)
             for n in ("ex", "bz"):
                 e[f"spectrum_rel_err_{n}"] = spectrum_error(r["spec"][n], truth["spec"][n])
