@@ -14,6 +14,8 @@
 // the device and rounded to Float once.
 #pragma clang fp contract(off)
 
+#include <mutex>
+
 #include "sp_internal.h"
 
 #ifndef SP_FFT_XCD_REMAP
@@ -89,57 +91,121 @@ template <class T, int R, bool INV> __device__ __forceinline__ void dftR(V2<T>* 
 
 constexpr int EPT = 16;  // elements per thread per pass
 
+// LDS position of logical element i.  Row transforms (PAD) leave one pad slot
+// after every 8 elements, so the first pass's butterfly writes (stride R
+// elements, 64 bytes at radix 8) fall into distinct banks: it cuts the row
+// kernels' LDS wait cycles ~5x and their time ~10% (rocprofv3, 4096-point
+// rows).  Column blocks keep the plain layout, which measured faster for them.
+template <bool PAD> __device__ __forceinline__ int lp(int i) { return PAD ? i + (i >> 3) : i; }
+__host__ __device__ constexpr int lds_slots(int e) { return e + (e >> 3) + 1; }
+
 // Division by a runtime constant d: q = hi32(x * m) with m = ceil(2^32 / d)
 // is exact whenever x * d < 2^32 (the error x (m - 2^32/d) / 2^32 < x / 2^32
 // stays below the gap 1/d between x/d and the next integer); every use below
-// has x * d < 2^27.  Replaces integer divisions in the index arithmetic, which
-// cost tens of instructions each.
+// has x * d < 2^27.  A power of two d = 2^sh divides by a shift instead: the
+// kernels are built twice, with every divisor of the launch a power of two
+// (POW2, radix 2/4/8 plans: shifts only) or not (the multiply; on the
+// 4-cycle-per-lane integer multiplier, so the cheaper path matters).
 // (d = 1 would need m = 2^32, which does not fit: it is handled apart.)
-struct FastDiv {
+struct Div {
   uint32_t d, m;
-  __device__ __forceinline__ explicit FastDiv(uint32_t dd)
-      : d(dd), m(dd > 1 ? (uint32_t)((0xFFFFFFFFull + dd) / dd) : 0u) {}
-  __device__ __forceinline__ uint32_t div(uint32_t x) const { return d > 1 ? __umulhi(x, m) : x; }
-  __device__ __forceinline__ uint32_t mod(uint32_t x) const { return x - div(x) * d; }
+  int sh;  // d = 2^sh, or -1
 };
+__host__ __device__ inline Div make_div(uint32_t d) {
+  Div v;
+  v.d = d;
+  v.m = d > 1 ? (uint32_t)((0xFFFFFFFFull + d) / d) : 0u;
+  int sh = -1;
+  if (d != 0 && (d & (d - 1)) == 0) {
+    sh = 0;
+    while ((1u << sh) != d) ++sh;
+  }
+  v.sh = sh;
+  return v;
+}
+template <bool POW2> __device__ __forceinline__ uint32_t qdiv(const Div& v, uint32_t x) {
+  if constexpr (POW2) return x >> v.sh;
+  else return v.d > 1 ? __umulhi(x, v.m) : x;
+}
+// x * d, a shift under POW2
+template <bool POW2> __device__ __forceinline__ int qmul(const Div& v, int x) {
+  if constexpr (POW2) return x << v.sh;
+  else return x * (int)v.d;
+}
+
+// Per-pass constants of one transform length, built on the host.
+struct FftPlan {
+  const void* tw;  // n complex Float: exp(-2 pi i k / n)
+  int n, nrad;
+  int R[16];          // radix of pass p
+  Div nb[16], ns[16];  // n / R_p and the stride Ns_p = R_0 ... R_{p-1}
+  Div nsr[16];         // Ns_p R_p
+  Div twstep[16];      // n / (Ns_p R_p)
+};
+
+// Twiddle k: a 32-bit byte offset from the table's (uniform) base, so the
+// load takes the scalar-base form with no 64-bit address arithmetic.
+template <class T> __device__ __forceinline__ V2<T> tw_load(const V2<T>* __restrict__ tw, int k) {
+  const uint32_t off = (uint32_t)k * (uint32_t)sizeof(V2<T>);
+  return *(const V2<T>*)((const char*)tw + off);
+}
 
 // One Stockham pass of radix R over `seqs` sequences of length N held in LDS.
 // Sequence s, element n lives at lds[s * sstride + n * estride].  SEQ_FAST
 // picks which index varies fastest across threads (columns: the sequence,
-// rows: the butterfly) so that LDS accesses stay unit-stride.
-template <class T, int R, bool INV, bool SEQ_FAST>
-__device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, int seqs, int sstride, int estride,
-                                              const V2<T>* __restrict__ tw, int nthreads) {
+// rows: the butterfly) so that LDS accesses stay unit-stride.  The twiddle
+// loads of all of a thread's butterflies are issued first, so their latency
+// overlaps the LDS reads.
+template <class T, int R, bool INV, bool SEQ_FAST, bool POW2>
+__device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, const Div& fnb, const Div& fns,
+                                              const Div& fnsr, const Div& fseq, const Div& twstep, int seqs,
+                                              int sstride, int estride, const V2<T>* __restrict__ tw,
+                                              int nthreads) {
   constexpr int BPT = (EPT + R - 1) / R;
   const int nb = N / R;
   const int total = nb * seqs;
-  const int twstep = N / (Ns * R);
-  const FastDiv fseq(SEQ_FAST ? seqs : nb), fns(Ns);
+  const Div& fq = SEQ_FAST ? fseq : fnb;
   V2<T> v[BPT][R];
-  int dst[BPT];
+  V2<T> w[BPT][R];
+  int src[BPT], dst[BPT];
 #pragma unroll
   for (int m = 0; m < BPT; ++m) {
     const int b = threadIdx.x + m * nthreads;
     dst[m] = -1;
     if (b < total) {
-      const int q = (int)fseq.div((uint32_t)b), rm = b - q * (int)fseq.d;
+      const int q = (int)qdiv<POW2>(fq, (uint32_t)b), rm = b - qmul<POW2>(fq, q);
       const int s = SEQ_FAST ? rm : q;
       const int j = SEQ_FAST ? q : rm;
-      const V2<T>* src = lds + s * sstride;
+      const int jq = (int)qdiv<POW2>(fns, (uint32_t)j);
+      const int k = j - qmul<POW2>(fns, jq);
+      if (Ns > 1) {
+        const int kt = qmul<POW2>(twstep, k);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[m][r] = src[(j + r * nb) * estride];
-      const int jq = (int)fns.div((uint32_t)j);
-      const int k = j - jq * Ns;
+        for (int r = 1; r < R; ++r) w[m][r] = tw_load<T>(tw, r * kt);
+      }
+      src[m] = s * sstride + j * estride;
+      dst[m] = s * sstride + (qmul<POW2>(fnsr, jq) + k) * estride;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < BPT; ++m) {
+    if (dst[m] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[m][r] = lds[lp<!SEQ_FAST>(src[m] + r * nb * estride)];
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < BPT; ++m) {
+    if (dst[m] >= 0) {
       if (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          V2<T> w = tw[r * k * twstep];
-          if (INV) w.y = -w.y;
-          v[m][r] = cm(v[m][r], w);
+          V2<T> t = w[m][r];
+          if (INV) t.y = -t.y;
+          v[m][r] = cm(v[m][r], t);
         }
       }
       dftR<T, R, INV>(v[m]);
-      dst[m] = s * sstride + (jq * Ns * R + k) * estride;
     }
   }
   __syncthreads();
@@ -147,61 +213,67 @@ __device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, int seq
   for (int m = 0; m < BPT; ++m) {
     if (dst[m] >= 0) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[dst[m] + r * Ns * estride] = v[m][r];
+      for (int r = 0; r < R; ++r) lds[lp<!SEQ_FAST>(dst[m] + r * Ns * estride)] = v[m][r];
     }
   }
   __syncthreads();
 }
 
-template <class T, bool INV, bool SEQ_FAST>
-__device__ void fft_lds(V2<T>* lds, const FftDim& d, int seqs, int sstride, int estride, int nthreads) {
-  const V2<T>* tw = (const V2<T>*)d.tw;
+template <class T, bool INV, bool SEQ_FAST, bool POW2>
+__device__ void fft_lds(V2<T>* lds, const FftPlan& pl, const Div& fseq, int seqs, int sstride, int estride,
+                        int nthreads) {
+  const V2<T>* tw = (const V2<T>*)pl.tw;
   int Ns = 1;
-  for (int p = 0; p < d.nrad; ++p) {
-    const int R = (int)((d.radix_code >> (4 * p)) & 15u);
-    if (R == 8) stockham_pass<T, 8, INV, SEQ_FAST>(lds, d.n, Ns, seqs, sstride, estride, tw, nthreads);
-    else if (R == 4) stockham_pass<T, 4, INV, SEQ_FAST>(lds, d.n, Ns, seqs, sstride, estride, tw, nthreads);
-    else if (R == 2) stockham_pass<T, 2, INV, SEQ_FAST>(lds, d.n, Ns, seqs, sstride, estride, tw, nthreads);
-    else stockham_pass<T, 3, INV, SEQ_FAST>(lds, d.n, Ns, seqs, sstride, estride, tw, nthreads);
+  for (int p = 0; p < pl.nrad; ++p) {
+    const int R = pl.R[p];
+    const Div fnb = pl.nb[p], fns = pl.ns[p], fnsr = pl.nsr[p], ts = pl.twstep[p];
+#define SP_PASS(RR, P2) \
+  stockham_pass<T, RR, INV, SEQ_FAST, P2>(lds, pl.n, Ns, fnb, fns, fnsr, fseq, ts, seqs, sstride, estride, tw, nthreads)
+    if (R == 8) SP_PASS(8, POW2);
+    else if (R == 4) SP_PASS(4, POW2);
+    else if (R == 2) SP_PASS(2, POW2);
+    else if constexpr (!POW2) SP_PASS(3, false);
+#undef SP_PASS
     Ns *= R;
   }
 }
 
 // Row transforms: block (bx, field) handles rows [bx*rpb, bx*rpb + rpb).
-template <class T, bool INV>
-__global__ void __launch_bounds__(512) k_fft_rows(FftBatch b, FftDim d, int ny, int rpb, int load, int store, T scale_store) {
+// `dn`, `dP`: division by the row length and by the padded row length.
+template <class T, bool INV, bool POW2>
+__global__ void __launch_bounds__(512) k_fft_rows(FftBatch b, FftPlan pl, Div dn, Div dP, int ny, int rpb, int load,
+                                                  int store, T scale_store) {
   extern __shared__ unsigned char smem[];
   V2<T>* lds = (V2<T>*)smem;
   const int f = blockIdx.y;
-  const int nx = d.n, P = nx + 2;
+  const int nx = pl.n, P = nx + 2;
   const int r0 = blockIdx.x * rpb;
   const int rows = min(rpb, ny - r0);
   const int E = rows * nx;
   const int nt = blockDim.x;
   V2<T>* spec = (V2<T>*)b.spec[f];
   T* grid = (T*)b.grid[f];
-  const FastDiv fnx(nx), fP(P);
   if (load == LOAD_REAL_PADDED) {
     const T sc = (T)b.scale[f];
     const bool scaled = b.scale[f] != 1.0;
     for (int i = threadIdx.x; i < E; i += nt) {
-      const int s = (int)fnx.div(i), n = i - s * nx;
+      const int s = (int)qdiv<POW2>(dn, i), n = i - s * nx;
       T v = grid[(r0 + s + 1) * P + n + 1];
       if (scaled) v = v * sc;
-      lds[i] = V2<T>{v, T(0.0)};
+      lds[lp<true>(i)] = V2<T>{v, T(0.0)};
     }
   } else {
-    for (int i = threadIdx.x; i < E; i += nt) lds[i] = spec[(int64_t)r0 * nx + i];
+    for (int i = threadIdx.x; i < E; i += nt) lds[lp<true>(i)] = spec[(int64_t)r0 * nx + i];
   }
   __syncthreads();
-  fft_lds<T, INV, false>(lds, d, rows, nx, 1, nt);
+  fft_lds<T, INV, false, POW2>(lds, pl, dn, rows, nx, 1, nt);
   if (store == STORE_REAL_PADDED) {
     // copy_to_spatial + update_ghosts: interior, the two ghost columns of this
     // row, and the ghost row that mirrors it (row Ny -> row 0, row 1 -> Ny+1).
     for (int i = threadIdx.x; i < rows * P; i += nt) {
-      const int s = (int)fP.div(i), c = i - s * P;
+      const int s = (int)qdiv<false>(dP, i), c = i - s * P;
       const int n = (c == 0) ? nx - 1 : (c == P - 1 ? 0 : c - 1);
-      const T v = lds[s * nx + n].x * scale_store;
+      const T v = lds[lp<true>(s * nx + n)].x * scale_store;
       const int r = r0 + s + 1;
       grid[r * P + c] = v;
       if (r == ny) grid[c] = v;
@@ -210,7 +282,7 @@ __global__ void __launch_bounds__(512) k_fft_rows(FftBatch b, FftDim d, int ny, 
   } else {
     const bool scaled = scale_store != T(1);
     for (int i = threadIdx.x; i < E; i += nt) {
-      V2<T> v = lds[i];
+      V2<T> v = lds[lp<true>(i)];
       if (scaled) v = V2<T>{v.x * scale_store, v.y * scale_store};
       spec[(int64_t)r0 * nx + i] = v;
     }
@@ -218,12 +290,13 @@ __global__ void __launch_bounds__(512) k_fft_rows(FftBatch b, FftDim d, int ny, 
 }
 
 // Column transforms: block (bx, field) handles columns [bx*C, bx*C + C).
-template <class T, bool INV>
-__global__ void __launch_bounds__(512) k_fft_cols(FftBatch b, FftDim d, int nx, int C, T scale_store) {
+// POW2 launches have C a power of two dividing nx.
+template <class T, bool INV, bool POW2>
+__global__ void __launch_bounds__(512) k_fft_cols(FftBatch b, FftPlan pl, int nx, int C, T scale_store) {
   extern __shared__ unsigned char smem[];
   V2<T>* lds = (V2<T>*)smem;
   const int f = blockIdx.y;
-  const int ny = d.n;
+  const int ny = pl.n;
   // Blocks of C < 16 columns read parts of 128-byte lines that neighbouring
   // blocks also read: each XCD takes a contiguous run of column blocks, so the
   // neighbours share its L2.
@@ -233,16 +306,16 @@ __global__ void __launch_bounds__(512) k_fft_cols(FftBatch b, FftDim d, int nx, 
   const int E = cols * ny;
   const int nt = blockDim.x;
   V2<T>* spec = (V2<T>*)b.spec[f];
-  const FastDiv fc(cols);
+  const Div fc = make_div((uint32_t)cols);
   for (int i = threadIdx.x; i < E; i += nt) {
-    const int n = (int)fc.div(i), c = i - n * cols;
+    const int n = (int)qdiv<POW2>(fc, i), c = i - n * cols;
     lds[i] = spec[(int64_t)n * nx + c0 + c];
   }
   __syncthreads();
-  fft_lds<T, INV, true>(lds, d, cols, 1, cols, nt);
+  fft_lds<T, INV, true, POW2>(lds, pl, fc, cols, 1, cols, nt);
   const bool scaled = scale_store != T(1);
   for (int i = threadIdx.x; i < E; i += nt) {
-    const int n = (int)fc.div(i), c = i - n * cols;
+    const int n = (int)qdiv<POW2>(fc, i), c = i - n * cols;
     V2<T> v = lds[i];
     if (scaled) v = V2<T>{v.x * scale_store, v.y * scale_store};
     spec[(int64_t)n * nx + c0 + c] = v;
@@ -263,16 +336,51 @@ template <class T> __global__ void k_twiddles(V2<T>* tw, int n, int fp32_gen) {
   tw[k] = V2<T>{(T)c, (T)s};
 }
 
-template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st, bool fp32_gen) {
-  hipLaunchKernelGGL(k_twiddles<T>, dim3((d.n + 255) / 256), dim3(256), 0, st, (V2<T>*)d.tw, d.n, (int)fp32_gen);
-}
-
 static int round_threads(int e) {
   int t = (e + EPT - 1) / EPT;
   t = ((t + 63) / 64) * 64;
   return t < 64 ? 64 : (t > 512 ? 512 : t);
 }
 template <class T> static int max_elems() { return fft_max_len<T>(); }
+
+// The padded LDS of the longest row block exceeds the 64 KB default limit of
+// dynamic shared memory; gfx950 has 160 KB per workgroup.
+template <class T> static void allow_big_lds() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const int bytes = lds_slots(fft_max_len<T>()) * (int)sizeof(V2<T>);
+    // a refusal surfaces as the launch's own error
+    const void* k[4] = {(const void*)k_fft_rows<T, false, false>, (const void*)k_fft_rows<T, true, false>,
+                        (const void*)k_fft_rows<T, false, true>, (const void*)k_fft_rows<T, true, true>};
+    for (const void* f : k) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  });
+}
+
+template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st, bool fp32_gen) {
+  allow_big_lds<T>();  // at set-up, outside any stream capture
+  hipLaunchKernelGGL(k_twiddles<T>, dim3((d.n + 255) / 256), dim3(256), 0, st, (V2<T>*)d.tw, d.n, (int)fp32_gen);
+}
+
+// The pass constants of a plan; `pow2`: every radix is 2, 4 or 8.
+static FftPlan make_plan(const FftDim& d, bool& pow2) {
+  FftPlan pl{};
+  pl.tw = d.tw;
+  pl.n = d.n;
+  pl.nrad = d.nrad;
+  pow2 = true;
+  int Ns = 1;
+  for (int p = 0; p < d.nrad && p < 16; ++p) {
+    const int R = (int)((d.radix_code >> (4 * p)) & 15u);
+    pl.R[p] = R;
+    pl.nb[p] = make_div((uint32_t)(d.n / R));
+    pl.ns[p] = make_div((uint32_t)Ns);
+    pl.nsr[p] = make_div((uint32_t)(Ns * R));
+    pl.twstep[p] = make_div((uint32_t)(d.n / (Ns * R)));
+    if (R == 3) pow2 = false;
+    Ns *= R;
+  }
+  return pl;
+}
 
 template <class T>
 void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, int load, int store,
@@ -284,13 +392,18 @@ void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, i
   if (rpb > ny) rpb = ny;
   const int E = rpb * dx.n;
   const int nt = round_threads(E);
-  const size_t shm = (size_t)E * sizeof(V2<T>);
+  const size_t shm = (size_t)lds_slots(E) * sizeof(V2<T>);
+  allow_big_lds<T>();
   dim3 grid((ny + rpb - 1) / rpb, b.count);
   const T scale = (T)store_scale;
-  if (inverse)
-    hipLaunchKernelGGL((k_fft_rows<T, true>), grid, dim3(nt), shm, st, b, dx, ny, rpb, load, store, scale);
-  else
-    hipLaunchKernelGGL((k_fft_rows<T, false>), grid, dim3(nt), shm, st, b, dx, ny, rpb, load, store, scale);
+  bool pow2;
+  const FftPlan pl = make_plan(dx, pow2);
+  const Div dn = make_div((uint32_t)dx.n), dP = make_div((uint32_t)(dx.n + 2));
+  const auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(nt), shm, st, b, pl, dn, dP, ny, rpb, load, store, scale);
+  };
+  if (pow2) inverse ? go(k_fft_rows<T, true, true>) : go(k_fft_rows<T, false, true>);
+  else inverse ? go(k_fft_rows<T, true, false>) : go(k_fft_rows<T, false, false>);
 }
 
 template <class T>
@@ -305,10 +418,12 @@ void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, d
   const size_t shm = (size_t)E * sizeof(V2<T>);
   dim3 grid((nx + C - 1) / C, b.count);
   const T scale = (T)store_scale;
-  if (inverse)
-    hipLaunchKernelGGL((k_fft_cols<T, true>), grid, dim3(nt), shm, st, b, dy, nx, C, scale);
-  else
-    hipLaunchKernelGGL((k_fft_cols<T, false>), grid, dim3(nt), shm, st, b, dy, nx, C, scale);
+  bool pow2;
+  const FftPlan pl = make_plan(dy, pow2);
+  pow2 = pow2 && (C & (C - 1)) == 0 && nx % C == 0;
+  const auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(nt), shm, st, b, pl, nx, C, scale); };
+  if (pow2) inverse ? go(k_fft_cols<T, true, true>) : go(k_fft_cols<T, false, true>);
+  else inverse ? go(k_fft_cols<T, true, false>) : go(k_fft_cols<T, false, false>);
 }
 
 #define INST(T)                                                                                 \

@@ -166,6 +166,24 @@ def test_fft2d_against_fp64(prec, shape):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("prec", [4, 8])
+def test_fft2d_uneven_blocks(prec):
+    # 3 rows / 3 columns per workgroup: the last block of each direction is
+    # short and the index divisions are not shifts, on power-of-two lengths
+    ny, nx = 64, 256
+    g, _ = pair(prec, size_x=nx, size_y=ny, delta=2)
+    g.set_tuning(fft_rows=3, fft_cols=3)
+    rng = np.random.default_rng(7)
+    x = (rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))).astype(g.ct)
+    g.set_spectrum("jx", x)
+    g.fft2d("jx")
+    ref = np.fft.fft2(x.astype(np.complex128))
+    tol = (5e-7 if prec == 4 else 1e-15) * np.log2(nx * ny)
+    assert np.linalg.norm(g.spectrum("jx") - ref) / np.linalg.norm(ref) < tol
+    g.fft2d("jx", inverse=True)
+    assert np.linalg.norm(g.spectrum("jx") - x) / np.linalg.norm(x) < tol
+
+
 def test_fft2d_reference_vector():
     V = ref_vectors()
     g, _ = pair(4)
