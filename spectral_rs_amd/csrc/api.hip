@@ -273,10 +273,10 @@ template <class T> void fields_update(sp_ctx* c) {
     b.scale[6] = (double)(T(1.0) / (T)c->p.dens);  // mod.rs:80-84
     launch_fft_rows<T>(b, c->fx, ny, 0, LOAD_REAL_PADDED, STORE_COMPLEX, 1.0, c->st);
     launch_fft_cols<T>(b, c->fy, nx, 0, 1.0, c->st);
-    // transforms of real fields: keep them exactly Hermitian (fields.hip)
-    launch_hermitian<T>(b.spec, 7, nx, ny, c->st);
   }
-  launch_spectral_update<T>(c->spec, geo<T>(c), c->p.dens, c->gauss_d, c->st);  // mod.rs:105-278
+  // mod.rs:105-278, with the forward spectra (transforms of real fields) kept
+  // exactly Hermitian (fields.hip)
+  launch_spectral_update<T>(c->spec, geo<T>(c), c->p.dens, c->gauss_d, c->st, true);
   {  // mod.rs:280-307: 6 inverse transforms, spectral->spatial + ghosts fused
     FftBatch b{};
     const int ids[6] = {SP_BX, SP_BY, SP_BZ, SP_EX, SP_EY, SP_EZ};
@@ -1111,8 +1111,8 @@ int sp_fft2d(sp_ctx* c, int spec, int inverse) {
 
 int sp_spectral_update(sp_ctx* c) {
   ENTRY_PRELUDE(c);
-  if (c->f64) launch_spectral_update<double>(c->spec, geo<double>(c), c->p.dens, c->gauss_d, c->st);
-  else launch_spectral_update<float>(c->spec, geo<float>(c), c->p.dens, c->gauss_d, c->st);
+  if (c->f64) launch_spectral_update<double>(c->spec, geo<double>(c), c->p.dens, c->gauss_d, c->st, false);
+  else launch_spectral_update<float>(c->spec, geo<float>(c), c->p.dens, c->gauss_d, c->st, false);
   HIPCHK(c, hipGetLastError());
   return SP_OK;
 }

@@ -261,13 +261,14 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return v;
 }
 
+// The update of mode i = (row, col) from its spectra jx ... ez; writes E,
+// B_wrk and B of the mode (and the zeroed jx, jy at k = 0) and adds the
+// mode's Gauss sums.
 template <class T>
-__global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T cdt, T ckc, T twopi_x, T twopi_y,
-                                                  double* gauss) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  double g_pre = 0, g_post = 0, g_rho = 0;
-  if (i < (int64_t)nx * ny) {
-  const int row = (int)(i / nx), col = (int)(i - (int64_t)row * nx);
+__device__ __forceinline__ void spectral_mode(const SpecPtrs& sp, int64_t i, int row, int col, int nx, int ny, T cdt,
+                                              T ckc, T twopi_x, T twopi_y, Cx<T> jx, Cx<T> jy, Cx<T> jz, Cx<T> rho,
+                                              Cx<T> ex, Cx<T> ey, Cx<T> ez, double& g_pre, double& g_post,
+                                              double& g_rho) {
   const bool nyq = col == nx / 2 || row == ny / 2;
   T kx = T(col);
   if (col >= nx / 2 + 1) kx -= T(nx);
@@ -277,26 +278,17 @@ __global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T
   ky *= twopi_y;
   const T kn = (i == 0) ? T(0) : T(1.) / (kx * kx + ky * ky);
 
-  T* jxp = (T*)sp.p[SP_JX];
-  T* jyp = (T*)sp.p[SP_JY];
-  Cx<T> jx = ld(jxp, i), jy = ld(jyp, i);
-  const Cx<T> jz = ld((const T*)sp.p[SP_JZ], i);
-  const Cx<T> rho = ld((const T*)sp.p[SP_RHO], i);
   if (i == 0) {  // mod.rs:106-108 (j_z at k=0 kept)
     jx = Cx<T>{0, 0};
     jy = Cx<T>{0, 0};
-    st(jxp, i, jx);
-    st(jyp, i, jy);
+    st((T*)sp.p[SP_JX], i, jx);
+    st((T*)sp.p[SP_JY], i, jy);
   }
   // mod.rs:113-119: B.spectral <- B_wrk
   T* wxp = (T*)sp.p[SP_BXW];
   T* wyp = (T*)sp.p[SP_BYW];
   T* wzp = (T*)sp.p[SP_BZW];
   const Cx<T> bx = ld(wxp, i), by = ld(wyp, i), bz = ld(wzp, i);
-  T* exp_ = (T*)sp.p[SP_EX];
-  T* eyp = (T*)sp.p[SP_EY];
-  T* ezp = (T*)sp.p[SP_EZ];
-  Cx<T> ex = ld(exp_, i), ey = ld(eyp, i), ez = ld(ezp, i);
   const Cx<T> icdt{T(0.0), cdt}, micdt{T(0.0), -cdt};
   // mod.rs:130-173
   ex = add(ex, sub(mul(Cx<T>{icdt.re * ky, icdt.im * ky}, bz), scl(ckc, jx)));
@@ -313,9 +305,9 @@ __global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T
     if (!nyq) {
       const double ar = (double)kx * ex.re + (double)ky * ey.re - (double)rho.im;
       const double ai = (double)kx * ex.im + (double)ky * ey.im + (double)rho.re;
-      g_pre = (double)d.re * d.re + (double)d.im * d.im;
-      g_post = ar * ar + ai * ai;
-      g_rho = (double)rho.re * rho.re + (double)rho.im * rho.im;
+      g_pre += (double)d.re * d.re + (double)d.im * d.im;
+      g_post += ar * ar + ai * ai;
+      g_rho += (double)rho.re * rho.re + (double)rho.im * rho.im;
     }
   }
   // mod.rs:211-225: B_wrk to the next half step.
@@ -331,30 +323,94 @@ __global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T
   st(wxp, i, wx);
   st(wyp, i, wy);
   st(wzp, i, wz);
-  st(exp_, i, ex);
-  st(eyp, i, ey);
-  st(ezp, i, ez);
+  st((T*)sp.p[SP_EX], i, ex);
+  st((T*)sp.p[SP_EY], i, ey);
+  st((T*)sp.p[SP_EZ], i, ez);
   // mod.rs:270-278: B at t+1 = (B_wrk(old) + B_wrk(new)) / 2
   st((T*)sp.p[SP_BX], i, scl(T(0.5), add(bx, wx)));
   st((T*)sp.p[SP_BY], i, scl(T(0.5), add(by, wy)));
   st((T*)sp.p[SP_BZ], i, scl(T(0.5), add(bz, wz)));
+}
+
+// Block partials of the Gauss sums in a fixed order (reproducible sums).
+__device__ __forceinline__ void gauss_partials(double g_pre, double g_post, double g_rho, double* gauss) {
+  __shared__ double red[3][4];
+  g_pre = wave_sum_d(g_pre);
+  g_post = wave_sum_d(g_post);
+  g_rho = wave_sum_d(g_rho);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = g_pre;
+    red[1][w] = g_post;
+    red[2][w] = g_rho;
   }
-  if (gauss) {  // block partials in a fixed order (reproducible sums)
-    __shared__ double red[3][4];
-    g_pre = wave_sum_d(g_pre);
-    g_post = wave_sum_d(g_post);
-    g_rho = wave_sum_d(g_rho);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-      red[0][w] = g_pre;
-      red[1][w] = g_post;
-      red[2][w] = g_rho;
+  __syncthreads();
+  if (threadIdx.x < 3)
+    gauss[(int64_t)blockIdx.x * 3 + threadIdx.x] =
+        ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T cdt, T ckc, T twopi_x, T twopi_y,
+                                                  double* gauss) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double g_pre = 0, g_post = 0, g_rho = 0;
+  if (i < (int64_t)nx * ny) {
+    const int row = (int)(i / nx), col = (int)(i - (int64_t)row * nx);
+    const auto in = [&](int id) { return ld((const T*)sp.p[id], i); };
+    spectral_mode<T>(sp, i, row, col, nx, ny, cdt, ckc, twopi_x, twopi_y, in(SP_JX), in(SP_JY), in(SP_JZ),
+                     in(SP_RHO), in(SP_EX), in(SP_EY), in(SP_EZ), g_pre, g_post, g_rho);
+  }
+  if (gauss) gauss_partials(g_pre, g_post, g_rho, gauss);
+}
+
+// Hermitian projection and update in one pass: one thread per mirrored pair
+// (i, m = -i mod (Ny, Nx)) with m >= i, over rows 0..Ny/2.  The seven
+// forward spectra (Jx Jy Jz rho Ex Ey Ez) are transforms of real fields;
+// they become X[i] <- (X[i] + conj(X[m])) / 2, X[m] <- conj(X[i]) (the
+// projected J and rho are stored back), and both modes are updated by
+// spectral_mode.  Exact in exact arithmetic; in floating point it removes the
+// anti-Hermitian rounding component of the FFTs.  That component matters
+// because B_wrk persists in spectral space: its anti-Hermitian part evolves as
+// B_a <- (1 - (c dt |k|)^2) B_a each step (the matching E part is dropped by
+// the real-part extraction), which grows for c dt |k| > sqrt(2).  With
+// Hermitian inputs spectral_mode maps mirrored modes to exactly conjugate
+// outputs, so B_wrk stays exactly Hermitian.
+template <class T>
+__global__ void __launch_bounds__(256) k_spectral_pair(SpecPtrs sp, int nx, int ny, T cdt, T ckc, T twopi_x,
+                                                       T twopi_y, double* gauss) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double g_pre = 0, g_post = 0, g_rho = 0;
+  if (i < (int64_t)(ny / 2 + 1) * nx) {
+    const int row = (int)(i / nx), col = (int)(i - (int64_t)row * nx);
+    const int mrow = (ny - row) % ny, mcol = (nx - col) % nx;
+    const int64_t m = (int64_t)mrow * nx + mcol;
+    if (m >= i) {
+      const int ids[7] = {SP_JX, SP_JY, SP_JZ, SP_RHO, SP_EX, SP_EY, SP_EZ};
+      Cx<T> a[7];
+#pragma unroll
+      for (int f = 0; f < 7; ++f) {
+        const T* p = (const T*)sp.p[ids[f]];
+        const Cx<T> u = ld(p, i), v = ld(p, m);
+        a[f] = Cx<T>{T(0.5) * (u.re + v.re), T(0.5) * (u.im - v.im)};
+      }
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {  // E is rewritten by the update
+        st((T*)sp.p[ids[f]], i, a[f]);
+        if (m != i) st((T*)sp.p[ids[f]], m, Cx<T>{a[f].re, -a[f].im});
+      }
+      spectral_mode<T>(sp, i, row, col, nx, ny, cdt, ckc, twopi_x, twopi_y, a[0], a[1], a[2], a[3], a[4], a[5],
+                       a[6], g_pre, g_post, g_rho);
+      if (m != i) {
+        Cx<T> b[7];
+#pragma unroll
+        for (int f = 0; f < 7; ++f) b[f] = Cx<T>{a[f].re, -a[f].im};
+        spectral_mode<T>(sp, m, mrow, mcol, nx, ny, cdt, ckc, twopi_x, twopi_y, b[0], b[1], b[2], b[3], b[4], b[5],
+                         b[6], g_pre, g_post, g_rho);
+      }
     }
-    __syncthreads();
-    if (threadIdx.x < 3)
-      gauss[(int64_t)blockIdx.x * 3 + threadIdx.x] =
-          ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
   }
+  if (gauss) gauss_partials(g_pre, g_post, g_rho, gauss);
 }
 
 // Sums the block partials of k_spectral (one block, fixed order) into
@@ -385,40 +441,8 @@ __global__ void __launch_bounds__(1024) k_gauss_reduce(const double* part, int n
   }
 }
 
-// Hermitian projection X[k] <- (X[k] + conj(X[-k])) / 2, X[-k] <- conj(X[k])
-// of spectra that are transforms of real fields.  Exact in exact arithmetic;
-// in floating point it removes the anti-Hermitian rounding component.  That
-// component matters because B_wrk persists in spectral space: its
-// anti-Hermitian part evolves as B_a <- (1 - (c dt |k|)^2) B_a each step (the
-// matching E part is dropped by the real-part extraction), which grows for
-// c dt |k| > sqrt(2).  With Hermitian inputs the pointwise update of
-// k_spectral maps mirrored modes to exactly conjugate outputs, so B_wrk stays
-// exactly Hermitian.  One thread per mode; the lower index of each mirrored
-// pair writes both.
-template <class T> __global__ void __launch_bounds__(256) k_hermitian(SpecPtrs sp, int count, int nx, int ny) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= (int64_t)nx * ny) return;
-  const int row = (int)(i / nx), col = (int)(i - (int64_t)row * nx);
-  const int64_t m = (int64_t)((ny - row) % ny) * nx + (nx - col) % nx;
-  if (m < i) return;
-  for (int f = 0; f < count; ++f) {
-    T* p = (T*)sp.p[f];
-    const Cx<T> a = ld(p, i), b = ld(p, m);
-    const Cx<T> s{T(0.5) * (a.re + b.re), T(0.5) * (a.im - b.im)};
-    st(p, i, s);
-    if (m != i) st(p, m, Cx<T>{s.re, -s.im});
-  }
-}
-
-template <class T> void launch_hermitian(void* const* spec, int count, int nx, int ny, hipStream_t st) {
-  SpecPtrs sp{};
-  for (int i = 0; i < count && i < 14; ++i) sp.p[i] = spec[i];
-  const int64_t n = (int64_t)nx * ny;
-  hipLaunchKernelGGL(k_hermitian<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sp, count, nx, ny);
-}
-
 template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
-                                               hipStream_t st) {
+                                               hipStream_t st, bool project) {
   SpecPtrs sp{};
   for (int i = 0; i < 14; ++i) sp.p[i] = spec[i];
   const T PI = (T)3.14159265358979323846264338327950288;
@@ -426,10 +450,14 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   const T twopi_y = T(2.0) * PI / T(g.ny);
   const T ckc = g.dt / T(dens);  // mod.rs:127
   const T cdt = g.dt * g.c;      // mod.rs:128
-  const int64_t n = (int64_t)g.nx * g.ny;
+  const int64_t n = project ? (int64_t)(g.ny / 2 + 1) * g.nx : (int64_t)g.nx * g.ny;
   const int nblk = (int)((n + 255) / 256);
-  hipLaunchKernelGGL(k_spectral<T>, dim3((unsigned)nblk), dim3(256), 0, st, sp, g.nx, g.ny, cdt, ckc, twopi_x,
-                     twopi_y, gauss ? gauss + GAUSS_PARTIALS : nullptr);
+  if (project)
+    hipLaunchKernelGGL(k_spectral_pair<T>, dim3((unsigned)nblk), dim3(256), 0, st, sp, g.nx, g.ny, cdt, ckc, twopi_x,
+                       twopi_y, gauss ? gauss + GAUSS_PARTIALS : nullptr);
+  else
+    hipLaunchKernelGGL(k_spectral<T>, dim3((unsigned)nblk), dim3(256), 0, st, sp, g.nx, g.ny, cdt, ckc, twopi_x,
+                       twopi_y, gauss ? gauss + GAUSS_PARTIALS : nullptr);
   if (gauss) hipLaunchKernelGGL(k_gauss_reduce, dim3(1), dim3(1024), 0, st, gauss + GAUSS_PARTIALS, nblk, gauss);
 }
 
@@ -438,8 +466,7 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   template void launch_fold_ghosts<T>(T* const*, int, int, int, hipStream_t);                   \
   template void launch_update_ghosts<T>(T* const*, int, int, int, hipStream_t);                 \
   template void launch_binomial<T>(void**, void**, int, int, int, int, hipStream_t);              \
-  template void launch_spectral_update<T>(void* const*, const Geo<T>&, int, double*, hipStream_t);            \
-  template void launch_hermitian<T>(void* const*, int, int, int, hipStream_t);
+  template void launch_spectral_update<T>(void* const*, const Geo<T>&, int, double*, hipStream_t, bool);
 INST(float)
 INST(double)
 #undef INST

@@ -95,8 +95,10 @@ constexpr int MAX_SPECIES = 64;
 constexpr int DIAG_FIXED = 4;
 constexpr int DIAG_DOUBLES = DIAG_FIXED + 4 * MAX_SPECIES;
 inline size_t gauss_doubles(int nx, int ny) { return GAUSS_PARTIALS + 3 * (((size_t)nx * ny + 255) / 256); }
+// `project`: Hermitian-project the seven forward spectra in the same pass (the
+// step's path; see k_spectral_pair).
 template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
-                                               hipStream_t st);
+                                               hipStream_t st, bool project);
 // FP64 diagnostics: out[0] += sum over the interior of a^2 + b^2 + c^2;
 // kinetic: out[0] += sum (gamma - 1), out[1..3] += sum p.
 // sum32: accumulate in FP32 (precision study), the total still stored as FP64.
@@ -104,8 +106,6 @@ template <class T> void launch_sumsq3(const T* a, const T* b, const T* c, int nx
                                       hipStream_t st, bool sum32 = false);
 template <class T> void launch_kinetic(const SpeciesDev<T>& s, double csqinv, double* out, hipStream_t st,
                                        bool sum32 = false);
-// Hermitian projection of `count` spectra of real fields (see fields.hip).
-template <class T> void launch_hermitian(void* const* spec, int count, int nx, int ny, hipStream_t st);
 
 // FFT plan for one dimension: radices and an FP64-generated twiddle table.
 struct FftDim {

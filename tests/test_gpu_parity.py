@@ -215,6 +215,53 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
+def test_step_spectral_update_projects_and_matches_oracle(prec):
+    # The step's spectral update Hermitian-projects the seven forward spectra
+    # in the same pass.  Rebuild its inputs with this build's own forward
+    # transform (sp_fft2d, the same kernels), project them in numpy with the
+    # kernel's rounding points, and run the oracle's update on them: B_wrk
+    # and the stored J, rho spectra must agree bit for bit, and B_wrk stays
+    # exactly Hermitian when it starts so.
+    nx, ny, dens = 64, 32, 3
+    g, r = pair(prec, size_x=nx, size_y=ny, delta=2, dens=dens, n_pass=0)
+    ft = g.ft
+    rng = np.random.default_rng(12)
+    grids = {k: rng.standard_normal((ny + 2, nx + 2)).astype(ft) for k in ["ex", "ey", "ez", "jx", "jy", "jz", "rho"]}
+    bw = {}
+    for k in ["bxw", "byw", "bzw"]:
+        x = rng.standard_normal((ny, nx)).astype(ft)
+        bw[k] = np.fft.fft2(x).astype(g.ct)  # transform of a real field, nearly Hermitian
+        bw[k] = 0.5 * (bw[k] + np.conj(bw[k][(-np.arange(ny)) % ny][:, (-np.arange(nx)) % nx]))
+    for k, a in grids.items():
+        g.set_grid(k, a)
+    for k, a in bw.items():
+        g.set_spectrum(k, a)
+        r.set_spectrum(k, a)
+    g.fields_update()
+    aux = capi.GpuSim(prec=prec, **dict(TEST_SIM, size_x=nx, size_y=ny, delta=2, dens=dens, n_pass=0))
+    half = ft(0.5)
+    mi, mj = (-np.arange(ny)) % ny, (-np.arange(nx)) % nx
+    for k, a in grids.items():
+        x = a[1:-1, 1:-1]
+        if k == "rho":
+            x = x * (ft(1) / ft(dens))
+        aux.set_spectrum("jx", x.astype(g.ct))
+        aux.fft2d("jx")
+        f = aux.spectrum("jx")
+        fm = f[mi][:, mj]
+        re = half * (f.real + fm.real)
+        im = half * (f.imag - fm.imag)
+        r.set_spectrum(k, (re + 1j * im).astype(g.ct))
+    aux.close()
+    r.spectral_update()
+    for name in ["bxw", "byw", "bzw", "jx", "jy", "jz", "rho"]:
+        assert np.array_equal(g.spectrum(name), r.spectrum(name)), name
+    for name in ["bxw", "byw", "bzw"]:
+        s = g.spectrum(name)
+        assert np.array_equal(s[mi][:, mj], np.conj(s)), name
+
+
+@pytest.mark.parametrize("prec", [4, 8])
 def test_fields_update_matches_oracle(prec):
     nx, ny = 128, 64
     g, r = pair(prec, size_x=nx, size_y=ny, delta=4, dens=4, n_pass=4)
