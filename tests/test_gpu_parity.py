@@ -126,15 +126,18 @@ def test_deposit_ghosts_bit_exact(prec):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
-@pytest.mark.parametrize("shape", [(12, 24), (64, 256), (48, 96)])
-def test_binomial_filter_bit_exact_with_stale_ring(prec, shape):
+@pytest.mark.parametrize("shape,passes", [((12, 24), 4), ((64, 256), 4), ((48, 96), 4), ((96, 384), 1),
+                                          ((96, 384), 9)])
+def test_binomial_filter_bit_exact_with_stale_ring(prec, shape, passes):
+    # tiles of 16 x 64 cells: the larger grids have tiles whose first pass
+    # wraps around an edge and tiles whose first pass stays inside
     ny, nx = shape
     g, r = pair(prec, size_x=nx, size_y=ny, delta=2)
     a = np.random.default_rng(11).standard_normal((ny + 2, nx + 2)).astype(g.ft)  # non-periodic ring
     g.set_grid("jy", a)
     r.set_grid("jy", a)
-    g.filter("jy", 4)
-    for _ in range(4):
+    g.filter("jy", passes)
+    for _ in range(passes):
         r.binomial_pass("jy")
     assert np.array_equal(g.grid("jy"), r.grid("jy"))
 
