@@ -969,26 +969,35 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
   }
 }
 
-// Exclusive prefix sum of `count` into `cursor`, one workgroup.
+// Exclusive prefix sum of `count` into `cursor`, one workgroup of 16 waves.
+// Wave w owns a contiguous segment; its lanes read it 64 consecutive counters
+// at a time (coalesced).  Pass 1 sums each segment, the segment offsets come
+// from those sums, and pass 2 scans each 64-counter step with shuffles and a
+// running carry.
 __global__ void __launch_bounds__(1024) k_tile_scan(const int* __restrict__ count, int* cursor, int ntiles) {
-  __shared__ int part[1024];
-  const int tid = threadIdx.x;
-  const int per = (ntiles + 1023) / 1024;
-  const int b = tid * per, e = min(b + per, ntiles);
-  int sum = 0;
-  for (int i = b; i < e; ++i) sum += count[i];
-  part[tid] = sum;
+  __shared__ int wsum[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int seg = (ntiles + 15) / 16;
+  const int b = min(w * seg, ntiles), e = min(b + seg, ntiles);
+  int s = 0;
+  for (int i = b + lane; i < e; i += 64) s += count[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) wsum[w] = s;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
-    const int v = tid >= off ? part[tid - off] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
-  }
-  int run = part[tid] - sum;
-  for (int i = b; i < e; ++i) {
-    cursor[i] = run;
-    run += count[i];
+  int carry = 0;
+  for (int k = 0; k < w; ++k) carry += wsum[k];
+  for (int i0 = b; i0 < e; i0 += 64) {
+    const int i = i0 + lane;
+    const int v = i < e ? count[i] : 0;
+    int x = v;  // inclusive scan over the 64 lanes
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (i < e) cursor[i] = carry + x - v;
+    carry += __shfl(x, 63);
   }
 }
 

@@ -379,15 +379,17 @@ def test_gpu_initialiser_positions_and_momentum_moments():
     assert np.all(s["px"] > 0)
 
 
-def test_tile_sort_keeps_values_and_orders_by_tile():
+@pytest.mark.parametrize("nx,ny,n", [(96, 48, 40000), (1536, 512, 400000)])
+def test_tile_sort_keeps_values_and_orders_by_tile(nx, ny, n):
     # The sort changes the storage order only: after it every particle (found
     # through its tag) holds the same values, and the device order is by tile
-    # (8x8-cell tiles, tile rows in serpentine order), the key being a pure function
-    # of the cell.
-    kw = dict(size_x=96, size_y=48, delta=4, dens=4, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0, left_edge=1)
+    # (8x4-cell tiles, tile rows in serpentine order), the key being a pure function
+    # of the cell.  The larger grid has 24576 tiles: the counter scan runs
+    # several 64-counter steps per wave.
+    kw = dict(size_x=nx, size_y=ny, delta=4, dens=4, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0, left_edge=1)
     g = capi.GpuSim(prec=4, sort_interval=1, **kw)
     r = O.RefSim(prec=4, **kw)
-    sp = random_particles(40000, 96, 48, 4, np.float32, 21, pscale=4.0)
+    sp = random_particles(n, nx, ny, 4, np.float32, 21, pscale=4.0)
     g.add_species(-1.0, 1.0, **sp)
     r.add_species(-1.0, 1.0, **sp)
     both(g, r, "prestep")  # sorts, then moves
