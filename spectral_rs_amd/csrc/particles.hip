@@ -790,10 +790,21 @@ __device__ __forceinline__ int tile_of(int32_t c, int ntx, int nty, int twl, int
 // particles that changed tile since, so a workgroup holds few distinct tiles
 // and the global counters see ~1-3 atomics per workgroup instead of one per
 // wave and tile.
-constexpr int SORT_NT = 256, SORT_PPT = 8, SORT_HT = 4096;  // table: > 2x the keys of a workgroup
+#ifndef SP_SORT_HT_LOG
+#define SP_SORT_HT_LOG 11
+#endif
+#ifndef SP_SORT_PPT
+#define SP_SORT_PPT 8
+#endif
+// table: 2^SP_SORT_HT_LOG >= SORT_NT * SORT_PPT slots, so every key of a
+// workgroup finds one; the smallest such table leaves LDS for 4 workgroups
+// per CU (4096 slots: 2; measured 1.3x faster placement, 2x faster counting
+// at 4096 x 1024)
+constexpr int SORT_NT = 256, SORT_PPT = SP_SORT_PPT, SORT_HT = 1 << SP_SORT_HT_LOG;
+static_assert(SORT_HT >= SORT_NT * SORT_PPT, "one slot per particle of a workgroup at least");
 
 __device__ __forceinline__ int hash_slot(lds_ptr<int> keys, int key) {
-  int h = (int)(((unsigned)key * 2654435761u) >> 20);  // 12 bits
+  int h = (int)(((unsigned)key * 2654435761u) >> (32 - SP_SORT_HT_LOG));
   for (;;) {
     int prev = -1;  // holds the slot's old key afterwards
     __hip_atomic_compare_exchange_strong(keys + h, &prev, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
