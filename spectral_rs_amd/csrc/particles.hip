@@ -310,7 +310,8 @@ __device__ __forceinline__ void lds_addi(lds_ptr<int> p, int v) {
 template <class T>
 __device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px, T py, T pz, T inv, T q,
                                                 const Geo<T>& g, const Sink<T>& k, int& bad) {
-  if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
+  // a stencil inside the window is inside the grid (the window lies in the
+  // padded grid), so the grid test is only needed off the window
   const bool in = k.holds(ix, iy);
   if constexpr (Acc<T>::fixed) {
     if (in) {  // fixed point: contributions (wy v 2^e) wx, rounded
@@ -333,6 +334,7 @@ __device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px
       return;
     }
   }
+  if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
   const T vx = q * px * inv, vy = q * py * inv, vz = q * pz * inv;
   T w[9];
   tsc(dx, dy, w);
@@ -345,7 +347,6 @@ __device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px
 template <class T>
 __device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, T q, const Geo<T>& g, const Sink<T>& k,
                                                 int& bad) {
-  if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
   // Within a window only dens = sum w is accumulated; the flush adds q * sum w
   // to rho (one species per pass; w * q is exact for |q| = 1).
   const bool in = k.holds(ix, iy);
@@ -364,6 +365,7 @@ __device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, T q,
       return;
     }
   }
+  if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
   T w[9];
   tsc(dx, dy, w);
   if (in) {
@@ -627,23 +629,26 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
     if constexpr (DIRECT) {
       // the sort interleaves each tile's particles (k_tile_pass), so
       // consecutive particles, hence lanes, sit in different cells
-      const int i = tid + NT * j;
-      if (i < cnt) {
-        const int64_t q = base + i;
+      // the chunk's arrays from uniform bases, 32-bit lane offsets
+      const unsigned i = (unsigned)(tid + NT * j);
+      if ((int)i < cnt) {
+        int32_t* const pc = s.cell + base;
+        T *const pdx = s.dx + base, *const pdy = s.dy + base, *const ppx = s.px + base;
+        T *const ppy = s.py + base, *const ppz = s.pz + base;
         const int32_t c = creg[j];
         int ix = cell_ix(c), iy = cell_iy(c);
-        T dx = s.dx[q], dy = s.dy[q], px = s.px[q], py = s.py[q], pz = s.pz[q];
+        T dx = pdx[i], dy = pdy[i], px = ppx[i], py = ppy[i], pz = ppz[i];
         if (PUSH) {
           boris(ix, iy, dx, dy, px, py, pz, fields, g, s.alpha, s.beta, bad);
           if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
-          s.py[q] = py;
-          s.pz[q] = pz;
+          ppy[i] = py;
+          ppz[i] = pz;
         }
         move_and_deposit(ix, iy, dx, dy, px, py, pz, s.charge, g, k, bad);
-        s.cell[q] = make_cell(ix, iy);
-        s.dx[q] = dx;
-        s.dy[q] = dy;
-        s.px[q] = px;
+        pc[i] = make_cell(ix, iy);
+        pdx[i] = dx;
+        pdy[i] = dy;
+        ppx[i] = px;
       }
     } else {
       const int p = ((tid + NT * j) * 33) & (K - 1);
