@@ -145,7 +145,11 @@ int sp_move_and_deposit(sp_ctx* ctx, int id);
 /* Field::deposit_ghosts on Jx, Jy, Jz, rho and dens (src/lib.rs:174-182,
  * src/flds/field.rs:259-373). */
 int sp_deposit_ghosts(sp_ctx* ctx);
-/* Flds::update (src/flds/mod.rs:87-309). */
+/* Flds::update (src/flds/mod.rs:87-309).  The seven forward spectra (Jx Jy
+ * Jz rho Ex Ey Ez, transforms of real fields) are projected onto their
+ * Hermitian part X[k] <- (X[k] + conj(X[-k])) / 2 in the same pass as the
+ * spectral update -- exact in exact arithmetic; it keeps the persistent B_wrk
+ * free of the FFT's anti-Hermitian rounding, which the update amplifies. */
 int sp_fields_update(sp_ctx* ctx);
 /* Prtl::boris_push (src/prtls.rs:262-443) for one species. */
 int sp_boris_push(sp_ctx* ctx, int id);
@@ -155,7 +159,8 @@ int sp_filter(sp_ctx* ctx, int grid, int passes);
 /* Fft2D::fft / Fft2D::inv_fft on one spectrum (src/flds/fft_2d.rs:91-172). */
 int sp_fft2d(sp_ctx* ctx, int spec, int inverse);
 /* The pointwise spectral part of Flds::update (src/flds/mod.rs:105-278) on
- * spectra already holding the forward transforms. */
+ * spectra already holding the forward transforms, without the projection
+ * (bit-identical to the reference's arithmetic). */
 int sp_spectral_update(sp_ctx* ctx);
 
 /* The untimed pre-loop of run(): move_and_deposit of every species, then the
