@@ -89,7 +89,16 @@ template <class T, int R, bool INV> __device__ __forceinline__ void dftR(V2<T>* 
   else dft8<T, INV>(v);
 }
 
-constexpr int EPT = 16;  // elements per thread per pass
+// Elements per thread per pass: columns 16; rows SP_FFT_EPT_ROWS (8 gives a
+// 4096-point row 512 threads).  The workgroup has at least E / EPT threads.
+#ifndef SP_FFT_EPT_ROWS
+#define SP_FFT_EPT_ROWS 8
+#endif
+#ifndef SP_FFT_EPT_COLS
+#define SP_FFT_EPT_COLS 8
+#endif
+constexpr int EPT_ROWS = SP_FFT_EPT_ROWS, EPT_COLS = SP_FFT_EPT_COLS;
+constexpr int ROWS_NT_MAX = EPT_ROWS < 16 ? 1024 : 512, COLS_NT_MAX = EPT_COLS < 16 ? 1024 : 512;
 
 // LDS position of logical element i.  Row transforms (PAD) leave one pad slot
 // after every 8 elements, so the first pass's butterfly writes (stride R
@@ -156,12 +165,12 @@ template <class T> __device__ __forceinline__ V2<T> tw_load(const V2<T>* __restr
 // rows: the butterfly) so that LDS accesses stay unit-stride.  The twiddle
 // loads of all of a thread's butterflies are issued first, so their latency
 // overlaps the LDS reads.
-template <class T, int R, bool INV, bool SEQ_FAST, bool POW2>
+template <class T, int R, bool INV, bool SEQ_FAST, bool POW2, int EPTV>
 __device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, const Div& fnb, const Div& fns,
                                               const Div& fnsr, const Div& fseq, const Div& twstep, int seqs,
                                               int sstride, int estride, const V2<T>* __restrict__ tw,
                                               int nthreads) {
-  constexpr int BPT = (EPT + R - 1) / R;
+  constexpr int BPT = (EPTV + R - 1) / R;
   const int nb = N / R;
   const int total = nb * seqs;
   const Div& fq = SEQ_FAST ? fseq : fnb;
@@ -219,7 +228,7 @@ __device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, const D
   __syncthreads();
 }
 
-template <class T, bool INV, bool SEQ_FAST, bool POW2>
+template <class T, bool INV, bool SEQ_FAST, bool POW2, int EPTV>
 __device__ void fft_lds(V2<T>* lds, const FftPlan& pl, const Div& fseq, int seqs, int sstride, int estride,
                         int nthreads) {
   const V2<T>* tw = (const V2<T>*)pl.tw;
@@ -228,7 +237,8 @@ __device__ void fft_lds(V2<T>* lds, const FftPlan& pl, const Div& fseq, int seqs
     const int R = pl.R[p];
     const Div fnb = pl.nb[p], fns = pl.ns[p], fnsr = pl.nsr[p], ts = pl.twstep[p];
 #define SP_PASS(RR, P2) \
-  stockham_pass<T, RR, INV, SEQ_FAST, P2>(lds, pl.n, Ns, fnb, fns, fnsr, fseq, ts, seqs, sstride, estride, tw, nthreads)
+  stockham_pass<T, RR, INV, SEQ_FAST, P2, EPTV>(lds, pl.n, Ns, fnb, fns, fnsr, fseq, ts, seqs, sstride, estride, tw, \
+                                                 nthreads)
     if (R == 8) SP_PASS(8, POW2);
     else if (R == 4) SP_PASS(4, POW2);
     else if (R == 2) SP_PASS(2, POW2);
@@ -241,7 +251,7 @@ __device__ void fft_lds(V2<T>* lds, const FftPlan& pl, const Div& fseq, int seqs
 // Row transforms: block (bx, field) handles rows [bx*rpb, bx*rpb + rpb).
 // `dn`, `dP`: division by the row length and by the padded row length.
 template <class T, bool INV, bool POW2>
-__global__ void __launch_bounds__(512) k_fft_rows(FftBatch b, FftPlan pl, Div dn, Div dP, int ny, int rpb, int load,
+__global__ void __launch_bounds__(ROWS_NT_MAX) k_fft_rows(FftBatch b, FftPlan pl, Div dn, Div dP, int ny, int rpb, int load,
                                                   int store, T scale_store) {
   extern __shared__ unsigned char smem[];
   V2<T>* lds = (V2<T>*)smem;
@@ -266,7 +276,7 @@ __global__ void __launch_bounds__(512) k_fft_rows(FftBatch b, FftPlan pl, Div dn
     for (int i = threadIdx.x; i < E; i += nt) lds[lp<true>(i)] = spec[(int64_t)r0 * nx + i];
   }
   __syncthreads();
-  fft_lds<T, INV, false, POW2>(lds, pl, dn, rows, nx, 1, nt);
+  fft_lds<T, INV, false, POW2, EPT_ROWS>(lds, pl, dn, rows, nx, 1, nt);
   if (store == STORE_REAL_PADDED) {
     // copy_to_spatial + update_ghosts: interior, the two ghost columns of this
     // row, and the ghost row that mirrors it (row Ny -> row 0, row 1 -> Ny+1).
@@ -292,7 +302,7 @@ __global__ void __launch_bounds__(512) k_fft_rows(FftBatch b, FftPlan pl, Div dn
 // Column transforms: block (bx, field) handles columns [bx*C, bx*C + C).
 // POW2 launches have C a power of two dividing nx.
 template <class T, bool INV, bool POW2>
-__global__ void __launch_bounds__(512) k_fft_cols(FftBatch b, FftPlan pl, int nx, int C, T scale_store) {
+__global__ void __launch_bounds__(COLS_NT_MAX) k_fft_cols(FftBatch b, FftPlan pl, int nx, int C, T scale_store) {
   extern __shared__ unsigned char smem[];
   V2<T>* lds = (V2<T>*)smem;
   const int f = blockIdx.y;
@@ -312,7 +322,7 @@ __global__ void __launch_bounds__(512) k_fft_cols(FftBatch b, FftPlan pl, int nx
     lds[i] = spec[(int64_t)n * nx + c0 + c];
   }
   __syncthreads();
-  fft_lds<T, INV, true, POW2>(lds, pl, fc, cols, 1, cols, nt);
+  fft_lds<T, INV, true, POW2, EPT_COLS>(lds, pl, fc, cols, 1, cols, nt);
   const bool scaled = scale_store != T(1);
   for (int i = threadIdx.x; i < E; i += nt) {
     const int n = (int)qdiv<POW2>(fc, i), c = i - n * cols;
@@ -336,11 +346,14 @@ template <class T> __global__ void k_twiddles(V2<T>* tw, int n, int fp32_gen) {
   tw[k] = V2<T>{(T)c, (T)s};
 }
 
-static int round_threads(int e) {
-  int t = (e + EPT - 1) / EPT;
+// ceil(e / ept) rounded up to whole waves; `cap` >= e / ept for every block
+// the launchers form (e <= fft_max_len)
+static int round_threads(int e, int ept, int cap) {
+  int t = (e + ept - 1) / ept;
   t = ((t + 63) / 64) * 64;
-  return t < 64 ? 64 : (t > 512 ? 512 : t);
+  return t < 64 ? 64 : (t > cap ? cap : t);
 }
+static_assert(8192 / EPT_COLS <= COLS_NT_MAX && 8192 / EPT_ROWS <= ROWS_NT_MAX, "a block's threads cover its elements");
 template <class T> static int max_elems() { return fft_max_len<T>(); }
 
 // The padded LDS of the longest row block exceeds the 64 KB default limit of
@@ -391,7 +404,7 @@ void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, i
   if (rpb < 1) rpb = 1;
   if (rpb > ny) rpb = ny;
   const int E = rpb * dx.n;
-  const int nt = round_threads(E);
+  const int nt = round_threads(E, EPT_ROWS, ROWS_NT_MAX);
   const size_t shm = (size_t)lds_slots(E) * sizeof(V2<T>);
   allow_big_lds<T>();
   dim3 grid((ny + rpb - 1) / rpb, b.count);
@@ -414,7 +427,7 @@ void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, d
   if (C < 1) C = 1;
   if (C > nx) C = nx;
   const int E = C * dy.n;
-  const int nt = round_threads(E);
+  const int nt = round_threads(E, EPT_COLS, COLS_NT_MAX);
   const size_t shm = (size_t)E * sizeof(V2<T>);
   dim3 grid((nx + C - 1) / C, b.count);
   const T scale = (T)store_scale;
