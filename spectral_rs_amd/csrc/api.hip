@@ -227,8 +227,9 @@ void harvest_timers(sp_ctx* c) {  // after a stream sync
 
 // ---- launch sequences, templated on Float ----
 template <class T> void zero_currents(sp_ctx* c) {
-  const int ids[5] = {SP_JX, SP_JY, SP_JZ, SP_RHO, SP_DENS};
-  for (int id : ids) fill<T>((T*)c->grid[id], T(0), (int64_t)c->ngrid_elems(), c->st);
+  T* g[5] = {(T*)c->grid[SP_JX], (T*)c->grid[SP_JY], (T*)c->grid[SP_JZ], (T*)c->grid[SP_RHO],
+             (T*)c->grid[SP_DENS]};
+  fill_many<T>(g, 5, T(0), (int64_t)c->ngrid_elems(), c->st);
 }
 
 template <class T> void fold(sp_ctx* c) {

@@ -27,6 +27,35 @@ template <class T> struct Ptrs8 {
   T* p[8];
 };
 
+// v into n elements of each of `count` arrays (blockIdx.y), one launch, 16-byte
+// stores over the 16-byte-aligned body of each array.
+template <class T> __global__ void __launch_bounds__(256) k_fill_many(Ptrs8<T> gp, T v, int64_t n) {
+  constexpr int V = 16 / (int)sizeof(T);
+  typedef T vec __attribute__((ext_vector_type(V)));
+  T* p = gp.p[blockIdx.y];
+  const int64_t head = min((int64_t)((16 - ((uintptr_t)p & 15)) & 15) / (int64_t)sizeof(T), n);
+  const int64_t nv = (n - head) / V;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  vec vv;
+#pragma unroll
+  for (int k = 0; k < V; ++k) vv[k] = v;
+  vec* body = (vec*)(p + head);
+  for (int64_t i = t; i < nv; i += stride) body[i] = vv;
+  if (t < head) p[t] = v;
+  for (int64_t i = head + nv * V + t; i < n; i += stride) p[i] = v;
+}
+
+template <class T> void fill_many(T* const* ptrs, int count, T v, int64_t n, hipStream_t st) {
+  if (n <= 0 || count <= 0) return;
+  Ptrs8<T> gp{};
+  for (int i = 0; i < count && i < 8; ++i) gp.p[i] = ptrs[i];
+  int64_t b = (n / (16 / (int64_t)sizeof(T)) + 255) / 256;
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  hipLaunchKernelGGL(k_fill_many<T>, dim3((unsigned)b, (unsigned)(count < 8 ? count : 8)), dim3(256), 0, st, gp, v,
+                     n);
+}
+
 // Field::deposit_ghosts (src/flds/field.rs:259-373).  One thread per interior
 // edge cell; each cell adds its ghost contributions in the reference's order:
 // top ghost row -> row Ny, bottom ghost row -> row 1, left ghost column ->
@@ -463,6 +492,7 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
 
 #define INST(T)                                                                                 \
   template void fill<T>(T*, T, int64_t, hipStream_t);                                           \
+  template void fill_many<T>(T* const*, int, T, int64_t, hipStream_t);                          \
   template void launch_fold_ghosts<T>(T* const*, int, int, int, hipStream_t);                   \
   template void launch_update_ghosts<T>(T* const*, int, int, int, hipStream_t);                 \
   template void launch_binomial<T>(void**, void**, int, int, int, int, hipStream_t);              \

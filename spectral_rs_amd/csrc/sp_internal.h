@@ -63,6 +63,8 @@ __host__ __device__ inline int32_t make_cell(int ix, int iy) { return (iy << 16)
 
 // ---- launchers (all asynchronous on `st`) ----
 template <class T> void fill(T* p, T v, int64_t n, hipStream_t st);
+// fill of up to 8 arrays of n elements in one launch
+template <class T> void fill_many(T* const* ptrs, int count, T v, int64_t n, hipStream_t st);
 template <class T> void launch_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags,
                                             hipStream_t st);
 template <class T> void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags,
