@@ -194,7 +194,7 @@ def main():
                     help="shock: the 4096x1024 run (the metric); ensemble: 64 uneven runs stepped together")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
+        print("bench.py: runs on one GPU only; multi-GPU runs are not supported", file=sys.stderr)
         sys.exit(2)
     args.steps_given = args.steps is not None
     if args.steps is None:
@@ -215,7 +215,7 @@ def main():
           file=sys.stderr, flush=True)
     # The reference has no left boundary and asserts when a particle reaches
     # ix <= delta (src/prtls.rs:163-170), which the rear edge of the slab does
-    # in long runs; the bench uses this build's mirrored-wall left edge.
+    # in long runs; the bench uses the mirrored-wall left edge (SP_LEFT_REFLECT).
     sim = capi.GpuSim(prec=4, sort_interval=args.sort_interval, left_edge=capi.SP_LEFT_REFLECT, **cfg)
     for k, (q, m, vth) in enumerate(SPECIES):
         sim.init_species(q, m, vth, seed=0)

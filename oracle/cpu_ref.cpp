@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY.  This file is the oracle that the HIP product path
 // is checked against.  Only tests/, __graft_entry__.smoke() and bench.py's
-// cpu_baseline leg may load it, and only as the checker (or as the timed CPU
+// cpu_baseline leg may load it, and only to check results (or as the timed CPU
 // baseline); the product library never links or calls it.
 //
 // Every function restates the reference Rust source named beside it
@@ -311,7 +311,7 @@ template <class T> static inline T rust_signum(T x) {
   return std::signbit(x) ? (T)-1.0 : (T)1.0;
 }
 
-// src/prtls.rs:123-190 (apply_bc).  left_edge = 1 adds this build's mirror of
+// src/prtls.rs:123-190 (apply_bc).  left_edge = 1 adds a mirror of
 // the right wall at cell delta+1 (the reference has no left boundary and
 // asserts instead, prtls.rs:163-170).
 template <class T> void apply_bc(Ctx<T>& s, Species<T>& P) {

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/cpu_ref.cpp).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
-checker, and by bench.py's cpu_baseline leg.  The product package never
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to check
+its result, and by bench.py's cpu_baseline leg.  The product package never
 imports it.
 
 ``RefSim`` mirrors the reference's Sim + Flds + Vec<Prtl> (src/lib.rs:204-264,

@@ -794,7 +794,7 @@ int sp_create(const sp_params* params, sp_ctx** out) {
   if (p.study & ~(SP_STUDY_ABS_POS32 | SP_STUDY_TWIDDLE32 | SP_STUDY_SUM32))
     return fail(nullptr, SP_EINVAL, "unknown study flag");
   if (p.study && p.precision != SP_F32) return fail(nullptr, SP_EINVAL, "study variants are of the FP32 path");
-  // this build's limits: packed cells, LDS-resident 1D transforms
+  // limits of the GPU layout: packed cells, LDS-resident 1D transforms
   if (p.size_x + 2 > 65535 || p.size_y + 2 > 32767)
     return fail(nullptr, SP_EINVAL, "grid too large for the packed (iy << 16 | ix) cell index");
   const int maxn = p.precision == SP_F64 ? fft_max_len<double>() : fft_max_len<float>();

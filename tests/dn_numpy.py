@@ -1,4 +1,4 @@
-"""FP64 numpy restatement of the learned current denoiser (test-only checker
+"""FP64 numpy restatement of the learned current denoiser (test-only reference
 for the HIP kernels of spectral_rs_amd/csrc/denoise.hip; the reference code
 has no such network).
 

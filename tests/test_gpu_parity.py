@@ -220,7 +220,7 @@ def rel(a, b):
 @pytest.mark.parametrize("prec", [4, 8])
 def test_step_spectral_update_projects_and_matches_oracle(prec):
     # The step's spectral update Hermitian-projects the seven forward spectra
-    # in the same pass.  Rebuild its inputs with this build's own forward
+    # in the same pass.  Rebuild its inputs with the library's own forward
     # transform (sp_fft2d, the same kernels), project them in numpy with the
     # kernel's rounding points, and run the oracle's update on them: B_wrk
     # and the stored J, rho spectra must agree bit for bit, and B_wrk stays
@@ -342,7 +342,7 @@ def test_shock_many_steps_fused_path_vs_dprec(sort_interval):
     # c = 10, dt = 0.045 (0.45 cells/step), 60 steps with the fused push+deposit pass.
     # The rear edge of the slab pulls a few electrons back past delta within
     # ~30 steps, which trips the reference's assert (src/prtls.rs:163-170);
-    # both sides use this build's mirrored-wall left edge instead.
+    # both sides use the mirrored-wall left edge (left_edge=1) instead.
     kw = dict(size_x=256, size_y=32, delta=8, dens=8, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0, left_edge=1)
     g = capi.GpuSim(prec=4, sort_interval=sort_interval, **kw)
     r = O.RefSim(prec=4, **kw)
