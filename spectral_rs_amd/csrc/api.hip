@@ -96,8 +96,11 @@ namespace {
 sp_tuning default_tuning(bool f64) {
   sp_tuning t{};
   t.chunk = f64 ? 1024 : 2048;
-  t.tile_w_log = 3;
-  t.tile_h_log = f64 ? 3 : 2;  // FP32: 8x4 tiles (the autotuner's pick for the 4096x1024 shock)
+  // FP32: 16x2 tiles, the fastest shape over a whole 2000-step run of the
+  // 4096x1024 shock (6.89 ms/step vs 6.95 with 8x4, the autotuner's pick from
+  // timings taken right after a sort, and 7.30 with 8x8)
+  t.tile_w_log = f64 ? 3 : 4;
+  t.tile_h_log = f64 ? 3 : 1;
   t.fft_rows = 1;
   t.fft_cols = 8;
   t.fft_radix_x = 8;
