@@ -208,7 +208,7 @@ typedef struct sp_tuning {
   int32_t fft_radix_y;
   int32_t reserved;
 } sp_tuning;
-/* Defaults: chunk 2048 and 16x2 tiles (SP_F64: 1024 and 8x8), 1 row and 8
+/* Defaults: chunk 2048 and 8x4 tiles, 16x2 from dens >= 16 (SP_F64: 1024 and 8x8), 1 row and 8
  * columns per workgroup, radix 8. */
 int sp_set_tuning(sp_ctx* ctx, const sp_tuning* t);
 int sp_get_tuning(const sp_ctx* ctx, sp_tuning* t);

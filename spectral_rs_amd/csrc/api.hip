@@ -93,14 +93,18 @@ struct sp_ctx {
 
 namespace {
 
-sp_tuning default_tuning(bool f64) {
+sp_tuning default_tuning(bool f64, int dens) {
   sp_tuning t{};
   t.chunk = f64 ? 1024 : 2048;
-  // FP32: 16x2 tiles, the fastest shape over a whole 2000-step run of the
-  // 4096x1024 shock (6.89 ms/step vs 6.95 with 8x4, the autotuner's pick from
-  // timings taken right after a sort, and 7.30 with 8x8)
-  t.tile_w_log = f64 ? 3 : 4;
-  t.tile_h_log = f64 ? 3 : 1;
+  // FP32 sort tiles by density.  A chunk covers about chunk / dens cells; at
+  // >= 16 particles per cell (<= 128 cells) 16x2 tiles keep its gathers on the
+  // fewest cache lines (4096x1024 shock at 32 ppc, whole 2000-step run: 6.89
+  // ms/step vs 6.95 with 8x4 and 7.30 with 8x8).  Sparser runs use 8x4: a
+  // 2-row footprint 256+ cells wide outgrows the deposit window once particles
+  // spread a few rows (4 ppc: 4.1 vs 0.86 ms per particle pass).
+  const bool flat = !f64 && dens >= 16;
+  t.tile_w_log = flat ? 4 : 3;
+  t.tile_h_log = f64 ? 3 : (flat ? 1 : 2);
   t.fft_rows = 1;
   t.fft_cols = 8;
   t.fft_radix_x = 8;
@@ -828,7 +832,7 @@ int sp_create(const sp_params* params, sp_ctx** out) {
   CK(hipGetDevice(&c->device));
   CK(hipStreamCreateWithFlags(&c->own_st, hipStreamNonBlocking));
   c->st = c->own_st;
-  c->tune = default_tuning(c->f64);
+  c->tune = default_tuning(c->f64, p.dens);
   c->fx.per_block = c->tune.fft_rows;
   c->fy.per_block = c->tune.fft_cols;
   const size_t gb = c->ngrid_elems() * c->esz, sb = c->nspec_elems() * 2 * c->esz;

@@ -383,7 +383,7 @@ def test_gpu_initialiser_positions_and_momentum_moments():
 def test_tile_sort_keeps_values_and_orders_by_tile(nx, ny, n):
     # The sort changes the storage order only: after it every particle (found
     # through its tag) holds the same values, and the device order is by tile
-    # (16x2-cell tiles, tile rows in serpentine order), the key being a pure function
+    # (8x4-cell tiles at this density, tile rows in serpentine order), the key being a pure function
     # of the cell.  The larger grid has 24576 tiles: the counter scan runs
     # several 64-counter steps per wave.
     kw = dict(size_x=nx, size_y=ny, delta=4, dens=4, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0, left_edge=1)
