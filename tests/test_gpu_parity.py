@@ -151,7 +151,7 @@ def test_binomial_filter_reference_vector():
 
 
 @pytest.mark.parametrize("prec", [4, 8])
-@pytest.mark.parametrize("shape", [(12, 24), (1024, 4096), (64, 256), (6, 48), (512, 2048)])
+@pytest.mark.parametrize("shape", [(12, 24), (1024, 4096), (64, 256), (6, 48), (512, 2048), (768, 3072)])
 def test_fft2d_against_fp64(prec, shape):
     ny, nx = shape
     g, _ = pair(prec, size_x=nx, size_y=ny, delta=2)
