@@ -59,6 +59,7 @@ struct sp_ctx {
   int device = 0;
   void* grid[11] = {};
   void* work[4] = {};
+  void* dsp = nullptr;  // per-species density scratch grid (GridSet::dsp)
   void* spec[14] = {};
   FftDim fx{}, fy{};
   int* flags_d = nullptr;
@@ -169,6 +170,7 @@ template <class T> SpeciesDev<T> sdev(const sp_ctx* c, const SpeciesBuf& s) { re
 GridSet gset(const sp_ctx* c) {
   GridSet g;
   for (int i = 0; i < 11; ++i) g.g[i] = c->grid[i];
+  g.dsp = c->dsp;
   return g;
 }
 
@@ -838,6 +840,8 @@ int sp_create(const sp_params* params, sp_ctx** out) {
   const size_t gb = c->ngrid_elems() * c->esz, sb = c->nspec_elems() * 2 * c->esz;
   for (auto& g : c->grid) CK(hipMalloc(&g, gb));
   for (auto& g : c->work) CK(hipMalloc(&g, gb));
+  CK(hipMalloc(&c->dsp, gb));
+  CK(hipMemsetAsync(c->dsp, 0, gb, c->st));
   for (auto& s : c->spec) CK(hipMalloc(&s, sb));
   for (auto& g : c->grid) CK(hipMemsetAsync(g, 0, gb, c->st));
   for (auto& g : c->work) CK(hipMemsetAsync(g, 0, gb, c->st));
@@ -886,6 +890,7 @@ int sp_destroy(sp_ctx* c) {
   for (auto& s : c->species) free_species(s);
   for (auto g : c->grid) hipFree(g);
   for (auto g : c->work) hipFree(g);
+  hipFree(c->dsp);
   for (auto s : c->spec) hipFree(s);
   hipFree(c->fx.tw);
   hipFree(c->fy.tw);

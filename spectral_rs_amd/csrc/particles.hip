@@ -264,7 +264,7 @@ template <class T> struct Sink {
   int pitch;
   T scale[4];  // fixed point: 2^e per LDS plane (Jx Jy Jz dens)
   T qsj;       // charge * scale[0] (fixed point)
-  glb_ptr<T> g[5];
+  glb_ptr<T> g[4];  // Jx Jy Jz and the species density (GridSet::dsp)
   __device__ __forceinline__ bool holds(int ix, int iy) const {
     return use_lds && ix - 1 >= x0 && ix + 1 < x0 + w && iy - 1 >= y0 && iy + 1 < y0 + h;
   }
@@ -273,7 +273,7 @@ template <class T> struct Sink {
 };
 
 // Adds w[k] * v to the 3x3 stencil around (ix, iy) of one plane
-// (0..2 = Jx Jy Jz, 3 = dens in LDS; 0..4 = Jx Jy Jz rho dens globally).
+// (0..2 = Jx Jy Jz, 3 = the species density, in LDS and globally).
 // The FP32 LDS windows use the separable forms below instead.
 template <class T>
 __device__ __forceinline__ void add9(const Sink<T>& k, int lplane, int gplane, int ix, int iy, bool in_lds,
@@ -350,12 +350,13 @@ __device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px
   add9(k, 2, 2, ix, iy, in, w, vz);
 }
 
-// Sim::calc_density (src/lib.rs:395-476): rho += w q, dens += w.
+// Sim::calc_density (src/lib.rs:395-476): rho += w q, dens += w.  One species
+// per pass, so the pass accumulates only its density sum w (window and
+// fallback alike, into GridSet::dsp); the launcher then adds q * dsp to rho and
+// dsp to dens (fold_species_density).
 template <class T>
-__device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, T q, const Geo<T>& g, const Sink<T>& k,
+__device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, const Geo<T>& g, const Sink<T>& k,
                                                 int& bad) {
-  // Within a window only dens = sum w is accumulated; the flush adds q * sum w
-  // to rho (one species per pass; w * q is exact for |q| = 1).
   const bool in = k.holds(ix, iy);
   if constexpr (Acc<T>::fixed) {
     if (in) {
@@ -375,12 +376,7 @@ __device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, T q,
   if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
   T w[9];
   tsc(dx, dy, w);
-  if (in) {
-    add9(k, 3, 4, ix, iy, true, w, T(1));
-  } else {
-    add9(k, 3, 3, ix, iy, false, w, q);
-    add9(k, 3, 4, ix, iy, false, w, T(1));
-  }
+  add9(k, 3, 3, ix, iy, in, w, T(1));
 }
 
 // Prtl::boris_push for one particle (src/prtls.rs:306-440): ordered 9-term
@@ -424,7 +420,7 @@ __device__ __forceinline__ void move_and_deposit(int& ix, int& iy, T& dx, T& dy,
   move_half(ix, iy, dx, dy, px, py, c1, g, bad);
   deposit_current(ix, iy, dx, dy, px, py, pz, inv, q, g, k, bad);
   move_half(ix, iy, dx, dy, px, py, c1, g, bad);
-  deposit_density(ix, iy, dx, dy, q, g, k, bad);
+  deposit_density(ix, iy, dx, dy, g, k, bad);
 }
 
 __device__ __forceinline__ int wave_or(int v) {
@@ -524,8 +520,7 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   k.g[0] = (glb_ptr<T>)gs.g[SP_JX];
   k.g[1] = (glb_ptr<T>)gs.g[SP_JY];
   k.g[2] = (glb_ptr<T>)gs.g[SP_JZ];
-  k.g[3] = (glb_ptr<T>)gs.g[SP_RHO];
-  k.g[4] = (glb_ptr<T>)gs.g[SP_DENS];
+  k.g[3] = (glb_ptr<T>)gs.dsp;
   k.x0 = max(s_box[0] - 2, 0);
   k.y0 = max(s_box[2] - 2, 0);
   k.w = min(s_box[1] + 2, g.nx + 1) - k.x0 + 1;
@@ -701,12 +696,7 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
       T v;
       if constexpr (Acc<T>::fixed) v = (T)a * (T(1) / k.scale[pl]);
       else v = a;
-      if (pl < 3) {
-        glb_add<T>(k.g[pl] + o, v);
-      } else {
-        glb_add<T>(k.g[3] + o, s.charge * v);  // rho
-        glb_add<T>(k.g[4] + o, v);             // dens
-      }
+      glb_add<T>(k.g[pl] + o, v);
     }
   }
   bad = wave_or(bad);
@@ -1056,11 +1046,34 @@ static void launch_particles(const SpeciesDev<T>& s, const Geo<T>& g, const Grid
   }
 }
 
+// rho += q dsp, dens += dsp, dsp = 0 over the padded grid, after one
+// species' pass (ghosts included: the fold of the ring comes later).
+template <class T>
+__global__ void __launch_bounds__(256) k_fold_species_density(T* __restrict__ rho, T* __restrict__ dens,
+                                                              T* __restrict__ dsp, T q, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    const T d = dsp[i];
+    if (d != T(0)) {
+      rho[i] += q * d;
+      dens[i] += d;
+      dsp[i] = T(0);
+    }
+  }
+}
+template <class T>
+static void fold_species_density(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, hipStream_t st) {
+  const int64_t n = (int64_t)(g.ny + 2) * g.pitch;
+  hipLaunchKernelGGL(k_fold_species_density<T>, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256),
+                     0, st, (T*)gs.g[SP_RHO], (T*)gs.g[SP_DENS], (T*)gs.dsp, s.charge, n);
+}
+
 template <class T>
 void launch_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
   if (s.n == 0) return;
   if (g.direct) launch_particles<T, false, true>(s, g, gs, flags, st);
   else launch_particles<T, false, false>(s, g, gs, flags, st);
+  fold_species_density<T>(s, g, gs, st);
 }
 template <class T>
 void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
@@ -1073,6 +1086,7 @@ void launch_push_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const Gri
   if (s.n == 0) return;
   if (g.direct) launch_particles<T, true, true>(s, g, gs, flags, st);
   else launch_particles<T, true, false>(s, g, gs, flags, st);
+  fold_species_density<T>(s, g, gs, st);
 }
 template <class T>
 void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, double gamma_inj, uint64_t seed,

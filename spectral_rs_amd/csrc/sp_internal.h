@@ -46,6 +46,9 @@ template <class T> struct SpeciesDev {
 
 struct GridSet {  // device pointers of one precision, indexed by sp_grid
   void* g[11];
+  // one species' density, zero between particle passes: a pass deposits sum w
+  // here and its launcher then adds q * it to rho and it to dens (and clears it)
+  void* dsp;
 };
 
 // Workgroups are dealt to the MI355X's 8 XCDs round-robin (workgroup b runs on
