@@ -875,10 +875,14 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
                                                        int twl, int thl, int* ctr) {
   constexpr int CH = SORT_NT * SORT_PPT;
   constexpr int OUT = HIST ? 1 : CH;
-  __shared__ int s_keys[SORT_HT], s_cnt[SORT_HT], s_n[HIST ? 1 : SORT_HT];
-  __shared__ int s_gpos[OUT], s_val[OUT];  // placement: output position and value by local index
+  static_assert(CH <= SORT_HT, "the staging pairs reuse the hash table");
+  // hash keys | counts; once the output positions are known, the same LDS
+  // stages pairs of 32-bit values by local index
+  __shared__ __attribute__((aligned(16))) int s_tab[2 * SORT_HT];
+  __shared__ int s_n[HIST ? 1 : SORT_HT];
+  __shared__ int s_gpos[OUT];  // placement: output position by local index
   __shared__ int s_total;
-  lds_ptr<int> keys = (lds_ptr<int>)s_keys, cnt = (lds_ptr<int>)s_cnt;
+  lds_ptr<int> keys = (lds_ptr<int>)s_tab, cnt = (lds_ptr<int>)(s_tab + SORT_HT);
   for (int i = threadIdx.x; i < SORT_HT; i += SORT_NT) {
     keys[i] = -1;
     cnt[i] = 0;
@@ -940,29 +944,38 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
     rank[j] = loc;  // from here on: the local index
   }
   const int m = (int)min((int64_t)CH, a.n - base);
-  // each value array goes through LDS so that the global writes follow the
-  // output order (contiguous within a group)
+  // the value arrays go through LDS two at a time, so that the global writes
+  // follow the output order (contiguous within a group): (cell, tag), then for
+  // FP32 (dx, dy), (px, py), (pz, -)
+  int2* s_pair = (int2*)s_tab;
+  constexpr int ROUNDS = sizeof(T) == 4 ? 4 : 1;
 #pragma unroll
-  for (int f = 0; f < 7; ++f) {
-    __syncthreads();
+  for (int f = 0; f < ROUNDS; ++f) {
+    __syncthreads();  // round 0: every thread is done with the hash table
 #pragma unroll
     for (int j = 0; j < SORT_PPT; ++j) {
       const int64_t i = base + threadIdx.x + SORT_NT * j;
       if (i >= a.n) continue;
-      int x = 0;
-      if (f == 0) x = c[j];
-      else if (f == 6) x = tg[j];
-      else if constexpr (sizeof(T) == 4) x = __builtin_bit_cast(int, (float)v[f >= 1 && f <= 5 ? f - 1 : 0][j]);
-      s_val[rank[j]] = x;
+      int2 x = {c[j], tg[j]};
+      if constexpr (sizeof(T) == 4) {
+        if (f > 0) {
+          x.x = __builtin_bit_cast(int, (float)v[2 * f - 2][j]);
+          x.y = f < 3 ? __builtin_bit_cast(int, (float)v[2 * f - 1][j]) : 0;
+        }
+      }
+      s_pair[rank[j]] = x;
     }
     __syncthreads();
     for (int l = threadIdx.x; l < m; l += SORT_NT) {
-      const int g = s_gpos[l], x = s_val[l];
-      if (f == 0) b.cell[g] = x;
-      else if (f == 6) b.tag[g] = x;
-      else if constexpr (sizeof(T) == 4) {
-        T* dst = f == 1 ? b.dx : f == 2 ? b.dy : f == 3 ? b.px : f == 4 ? b.py : b.pz;
-        dst[g] = __builtin_bit_cast(float, x);
+      const int g = s_gpos[l];
+      const int2 x = s_pair[l];
+      if (f == 0) {
+        b.cell[g] = x.x;
+        b.tag[g] = x.y;
+      } else if constexpr (sizeof(T) == 4) {
+        T* d0 = f == 1 ? b.dx : f == 2 ? b.px : b.pz;
+        d0[g] = __builtin_bit_cast(float, x.x);
+        if (f < 3) (f == 1 ? b.dy : b.py)[g] = __builtin_bit_cast(float, x.y);
       }
     }
   }
