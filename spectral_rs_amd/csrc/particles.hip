@@ -1061,24 +1061,40 @@ static void launch_particles(const SpeciesDev<T>& s, const Geo<T>& g, const Grid
 
 // rho += q dsp, dens += dsp, dsp = 0 over the padded grid, after one
 // species' pass (ghosts included: the fold of the ring comes later).
+// 16-byte accesses over the body (the grids are separate allocations, so
+// 16-byte aligned), scalar ones over a tail.
 template <class T>
 __global__ void __launch_bounds__(256) k_fold_species_density(T* __restrict__ rho, T* __restrict__ dens,
                                                               T* __restrict__ dsp, T q, int64_t n) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
-    const T d = dsp[i];
-    if (d != T(0)) {
-      rho[i] += q * d;
-      dens[i] += d;
-      dsp[i] = T(0);
+  constexpr int V = 16 / (int)sizeof(T);
+  typedef T vec __attribute__((ext_vector_type(V)));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x, t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nv = n / V;
+  for (int64_t i = t; i < nv; i += stride) {
+    const vec d = ((const vec*)dsp)[i];
+    vec r = ((vec*)rho)[i], e = ((vec*)dens)[i];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      r[k] += q * d[k];
+      e[k] += d[k];
     }
+    ((vec*)rho)[i] = r;
+    ((vec*)dens)[i] = e;
+    ((vec*)dsp)[i] = vec{};
+  }
+  for (int64_t i = nv * V + t; i < n; i += stride) {
+    const T d = dsp[i];
+    rho[i] += q * d;
+    dens[i] += d;
+    dsp[i] = T(0);
   }
 }
 template <class T>
 static void fold_species_density(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, hipStream_t st) {
   const int64_t n = (int64_t)(g.ny + 2) * g.pitch;
-  hipLaunchKernelGGL(k_fold_species_density<T>, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256),
-                     0, st, (T*)gs.g[SP_RHO], (T*)gs.g[SP_DENS], (T*)gs.dsp, s.charge, n);
+  const int64_t blocks = std::min<int64_t>((n / (16 / (int64_t)sizeof(T)) + 255) / 256, 2048) + 1;
+  hipLaunchKernelGGL(k_fold_species_density<T>, dim3((unsigned)blocks), dim3(256), 0, st, (T*)gs.g[SP_RHO],
+                     (T*)gs.g[SP_DENS], (T*)gs.dsp, s.charge, n);
 }
 
 template <class T>
