@@ -23,9 +23,15 @@
 //     (TSC partition of unity and centre weights, uniform-E kick, pure-B
 //     rotation preserves |u|, E x B drift, charge sum = q N), an independent
 //     rotation-matrix Boris in the test, vacuum light-wave dispersion of the
-//     pseudo-spectral leapfrog, and the Gauss-law projection.
+//     pseudo-spectral leapfrog, the Gauss-law projection, the time-centred
+//     vacuum B of the half-step average (mod.rs:270-278), and the Nyquist
+//     lines zeroed in E and B_wrk only (mod.rs:230-266)
+//     (tests/test_oracle_spectral.py).
 //   * rustfft's own rounding cannot be reproduced (no Rust toolchain); the FFT
-//     here is this file's mixed-radix DIT with twiddles evaluated in double.
+//     here is this file's mixed-radix DIT with twiddles evaluated in double
+//     and a table that keeps the exact symmetries of the unit circle, so a
+//     real field has an exactly Hermitian transform (sizes 2^a 3^b) in both
+//     precisions and the dprec B_wrk stays Hermitian over long runs.
 
 #include <cmath>
 #include <cstdint>
@@ -210,16 +216,57 @@ template <class T> struct Fft1 {
     }
     tw.resize(n);
     const double twopi = 6.283185307179586476925286766559;
-    for (int64_t k = 0; k < n; ++k) {
+    // The table obeys the exact symmetries of the unit circle: only the first
+    // octant (n % 4 == 0), quadrant (n % 2 == 0) or half (n odd) is evaluated
+    // and the rest is built from it by sign changes and swaps, e.g.
+    // tw[n-k] = conj(tw[k]), tw[k+n/2] = -tw[k], tw[k+n/4] = -i tw[k].  With
+    // separate cos/sin calls for every k a double table breaks these in the
+    // last bit (-2 pi k/n and -2 pi (n-k)/n round differently), the transform
+    // of a real field is then not exactly Hermitian, and the persistent B_wrk
+    // (src/flds/mod.rs:211-225) amplifies the anti-Hermitian part by up to
+    // (c dt |k|)^2 - 1 per step.  A float table rounded from double already
+    // had the symmetries (the double's last-bit differences round away).
+    auto eval = [&](int64_t k) {
       double a = -twopi * (double)k / (double)n;
-      double cr = std::cos(a), ci = std::sin(a);
-      if ((4 * k) % n == 0) {  // exact quarter turns
-        const double q[4][2] = {{1, 0}, {0, -1}, {-1, 0}, {0, 1}};
-        cr = q[(4 * k) / n][0];
-        ci = q[(4 * k) / n][1];
+      tw[k].re = (T)std::cos(a);
+      tw[k].im = (T)std::sin(a);
+    };
+    tw[0].re = (T)1;
+    tw[0].im = (T)0;
+    if (n % 4 == 0) {
+      const int64_t q = n / 4;
+      for (int64_t k = 1; 2 * k <= q; ++k) eval(k);
+      if (q % 2 == 0 && q > 0) tw[q / 2].im = -tw[q / 2].re;  // theta = pi/4: cos = sin
+      for (int64_t k = q / 2 + 1; k <= q; ++k) {  // theta -> pi/2 - theta
+        tw[k].re = -tw[q - k].im;
+        tw[k].im = -tw[q - k].re;
       }
-      tw[k].re = (T)cr;
-      tw[k].im = (T)ci;
+      for (int64_t k = q; k < 2 * q; ++k) {  // times -i
+        tw[k].re = tw[k - q].im;
+        tw[k].im = -tw[k - q].re;
+      }
+    } else if (n % 2 == 0) {
+      const int64_t h = n / 2;
+      for (int64_t k = 1; 2 * k <= h; ++k) eval(k);
+      for (int64_t k = h / 2 + 1; k < h; ++k) {  // theta -> pi - theta
+        tw[k].re = -tw[h - k].re;
+        tw[k].im = tw[h - k].im;
+      }
+    } else {
+      for (int64_t k = 1; 2 * k < n; ++k) eval(k);
+      for (int64_t k = 1; 2 * k < n; ++k) {
+        tw[n - k].re = tw[k].re;
+        tw[n - k].im = -tw[k].im;
+      }
+    }
+    if (n % 2 == 0) {
+      const int64_t h = n / 2;
+      tw[h].re = (T)-1;
+      tw[h].im = (T)0;
+      for (int64_t k = h + 1; k < n; ++k) {  // half turn
+        tw[k].re = -tw[k - h].re;
+        tw[k].im = -tw[k - h].im;
+      }
     }
   }
   Cx<T> w(int64_t e, int sign) const {  // exp(sign 2 pi i e / n)
@@ -232,17 +279,28 @@ template <class T> struct Fft1 {
     if (len == 1) { y[0] = x[0]; return; }
     int64_t p = fac[lev], m = len / p, step = n / len;
     for (int64_t r = 0; r < p; ++r) rec(x + r * xs, xs * p, y + r * m, m, lev + 1, sign, tmp);
-    std::vector<Cx<T>> buf(p);
+    // Butterfly inputs and the p x p roots w((r q mod p) n/p): on the stack
+    // for the usual small radices.  The values are the table's own entries,
+    // read without the index modulo of w(); the arithmetic is unchanged.
+    Cx<T> small[16], rsmall[256];
+    std::vector<Cx<T>> big(p > 16 ? p : 0), rbig(p > 16 ? p * p : 0);
+    Cx<T>* buf = p > 16 ? big.data() : small;
+    Cx<T>* root = p > 16 ? rbig.data() : rsmall;
+    for (int64_t q = 0; q < p; ++q)
+      for (int64_t r = 0; r < p; ++r) root[q * p + r] = w(r * q * m * step, sign);
     for (int64_t k = 0; k < m; ++k) {
       for (int64_t r = 0; r < p; ++r) {
-        Cx<T> a = y[r * m + k], t = w(r * k * step, sign);
+        // r k step < p m step = n: a direct table index
+        Cx<T> a = y[r * m + k], t = tw[r * k * step];
+        if (sign > 0) t.im = -t.im;
         buf[r].re = a.re * t.re - a.im * t.im;
         buf[r].im = a.re * t.im + a.im * t.re;
       }
       for (int64_t q = 0; q < p; ++q) {
         T sr = 0, si = 0;
+        const Cx<T>* rq = root + q * p;
         for (int64_t r = 0; r < p; ++r) {
-          Cx<T> a = buf[r], t = w(r * q * m * step, sign);
+          Cx<T> a = buf[r], t = rq[r];
           sr += a.re * t.re - a.im * t.im;
           si += a.re * t.im + a.im * t.re;
         }
