@@ -15,9 +15,10 @@
  * Threading: one host thread per context.  Calls are queued on the context's
  * HIP stream and return before the GPU finishes, except the getters/setters
  * and sp_sync(), which wait.  Invariant violations raised by a kernel are
- * reported by the next call that waits.  Nothing is allocated after
- * sp_create()/sp_species_*() (like the reference, which allocates only in
- * Flds::new and Prtl::new).
+ * reported by the next call that waits.  No device memory is allocated after
+ * sp_create()/sp_species_*()/sp_checkpoint_load() (like the reference, which
+ * allocates only in Flds::new and Prtl::new), except by the optional
+ * features that say so (denoiser, autotuner, ensembles).
  *
  * Precision: SP_F32 is the reference's default build (Float = f32); SP_F64 is
  * `--features dprec` (Cargo.toml:27-31, src/lib.rs:15-19).  Real arrays
@@ -127,9 +128,16 @@ int sp_species_add(sp_ctx* ctx, double charge, double mass, int64_t n, const int
  * keyed by (seed, species id, particle index) instead of thread_rng. */
 int sp_species_init(sp_ctx* ctx, double charge, double mass, double vth, uint64_t seed, int* id);
 int sp_species_count(const sp_ctx* ctx, int id, int64_t* n);
-/* Copies a species back in the order it was added; psa may be NULL. */
+/* Copies a species back in the order it was added; any output may be NULL
+ * (psa is computed from the momenta whether or not px, py, pz are asked for). */
 int sp_get_species(sp_ctx* ctx, int id, int64_t n, int32_t* ix, int32_t* iy, void* dx, void* dy,
                    void* px, void* py, void* pz, void* psa);
+/* The species as it lies on the device (no reference counterpart; Prtl keeps
+ * one order, src/prtls.rs:8-23): the cell-sorted order the particle passes
+ * read, packed cells (iy << 16 | ix) and each particle's tag (its index in the
+ * order the species was added).  Any output may be NULL. */
+int sp_get_species_raw(sp_ctx* ctx, int id, int64_t n, int32_t* cell, int32_t* tag, void* dx, void* dy,
+                       void* px, void* py, void* pz);
 
 /* Padded real grids: count = (size_x+2)*(size_y+2) elements. */
 int sp_get_grid(sp_ctx* ctx, int grid, void* out, size_t count);
@@ -192,6 +200,16 @@ int sp_set_graph(sp_ctx* ctx, int on);
 /* FP64 diagnostics (no reference counterpart): energies, momentum, Gauss
  * residuals. */
 int sp_get_diag(sp_ctx* ctx, sp_diag* out);
+/* Restartable checkpoint.  save_output (src/save/mod.rs:25-81) writes species
+ * 0 and the spatial fields only; a restart also needs every species and the
+ * persistent spectral B_wrk (src/flds/mod.rs:31-33).  The file holds the
+ * parameters, launch shapes, step count, every species in device order with
+ * its tags, the 11 grids and the 14 spectra.  Save runs a pending push first.
+ * Load goes into a context created with the same sp_params and no species;
+ * stepping it continues the saved run exactly where it stopped (the learned
+ * denoiser, if any, must be set again). */
+int sp_checkpoint_save(sp_ctx* ctx, const char* path);
+int sp_checkpoint_load(sp_ctx* ctx, const char* path);
 
 /* ---- Launch shapes and the autotuner (no reference counterpart) ----
  * The knobs of one context's kernels.  They change speed only: cells, offsets

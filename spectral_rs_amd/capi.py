@@ -22,11 +22,11 @@ GRIDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5, "jx": 6, "jy": 7,
 
 # exported symbols, in the order include/spectral_rs.h declares them
 SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_species_init",
-           "sp_species_count", "sp_get_species", "sp_get_grid", "sp_set_grid", "sp_get_spectrum",
+           "sp_species_count", "sp_get_species", "sp_get_species_raw", "sp_get_grid", "sp_set_grid", "sp_get_spectrum",
            "sp_set_spectrum", "sp_zero_currents", "sp_move_and_deposit", "sp_deposit_ghosts",
            "sp_fields_update", "sp_boris_push", "sp_filter", "sp_fft2d", "sp_spectral_update",
            "sp_prestep", "sp_step", "sp_sync", "sp_set_timing", "sp_get_timers", "sp_set_graph",
-           "sp_get_diag", "sp_set_tuning", "sp_get_tuning", "sp_autotune", "sp_ensemble_create",
+           "sp_get_diag", "sp_checkpoint_save", "sp_checkpoint_load", "sp_set_tuning", "sp_get_tuning", "sp_autotune", "sp_ensemble_create",
            "sp_ensemble_destroy", "sp_ensemble_last_error", "sp_ensemble_set_graph", "sp_ensemble_step",
            "sp_ensemble_sync", "sp_ensemble_stats", "sp_dn_create", "sp_dn_destroy", "sp_dn_last_error", "sp_dn_set_params",
            "sp_dn_get_params", "sp_dn_set_loss_weights", "sp_dn_forward", "sp_dn_gradient", "sp_dn_train",
@@ -85,6 +85,9 @@ def load(path=LIB_PATH):
     L.sp_species_init.argtypes = [vp, d, d, d, ct.c_uint64, ct.POINTER(i32)]
     L.sp_species_count.argtypes = [vp, i32, ct.POINTER(i64)]
     L.sp_get_species.argtypes = [vp, i32, i64] + [vp] * 8
+    L.sp_get_species_raw.argtypes = [vp, i32, i64] + [vp] * 7
+    L.sp_checkpoint_save.argtypes = [vp, ct.c_char_p]
+    L.sp_checkpoint_load.argtypes = [vp, ct.c_char_p]
     for n in ["sp_get_grid", "sp_set_grid", "sp_get_spectrum", "sp_set_spectrum"]:
         getattr(L, n).argtypes = [vp, i32, vp, ct.c_size_t]
     for n in ["sp_zero_currents", "sp_deposit_ghosts", "sp_fields_update", "sp_spectral_update",
@@ -198,6 +201,28 @@ class GpuSim:
         self._chk(load().sp_get_species(self.h, sid, n, _p(ix), _p(iy), *[_p(a) for a in fl]))
         return dict(ix=ix.astype(np.int64), iy=iy.astype(np.int64), dx=fl[0], dy=fl[1], px=fl[2],
                     py=fl[3], pz=fl[4], psa=fl[5])
+
+    def species_raw(self, sid):
+        """The species in device order: packed cells (iy << 16 | ix), tags, offsets, momenta."""
+        n = self.count(sid)
+        cell = np.empty(n, np.int32)
+        tag = np.empty(n, np.int32)
+        fl = [np.empty(n, self.ft) for _ in range(5)]
+        self._chk(load().sp_get_species_raw(self.h, sid, n, _p(cell), _p(tag), *[_p(a) for a in fl]))
+        return dict(cell=cell, tag=tag, ix=(cell & 0xFFFF).astype(np.int64), iy=(cell >> 16).astype(np.int64),
+                    dx=fl[0], dy=fl[1], px=fl[2], py=fl[3], pz=fl[4])
+
+    def checkpoint_save(self, path):
+        self._chk(load().sp_checkpoint_save(self.h, os.fsencode(path)))
+
+    def checkpoint_load(self, path):
+        """Into a context made with the same parameters and no species."""
+        self._chk(load().sp_checkpoint_load(self.h, os.fsencode(path)))
+        n = ct.c_int64()
+        k = 0
+        while load().sp_species_count(self.h, k, ct.byref(n)) == SP_OK:
+            k += 1
+        self.nspecies = k
 
     # ---- grids ----
     def grid(self, name):

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -986,15 +987,23 @@ int sp_get_species(sp_ctx* c, int id, int64_t n, int32_t* ix, int32_t* iy, void*
   std::vector<int32_t> cell(n), tag(n);
   HIPCHK(c, hipMemcpy(cell.data(), s.cur.cell, n * 4, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(tag.data(), s.cur.tag, n * 4, hipMemcpyDeviceToHost));
-  std::vector<unsigned char> buf(n * c->esz);
+  // values in tag order; the momenta are kept whenever psa is asked for, so
+  // gamma never reads a caller array that may be NULL
+  const size_t e = c->esz;
+  std::vector<unsigned char> buf(n * e), mom[3];
   void* outs[5] = {dx, dy, px, py, pz};
   void* srcs[5] = {s.cur.dx, s.cur.dy, s.cur.px, s.cur.py, s.cur.pz};
-  const size_t e = c->esz;
   for (int k = 0; k < 5; ++k) {
+    const bool keep = psa && k >= 2;
+    if (!outs[k] && !keep) continue;
     HIPCHK(c, hipMemcpy(buf.data(), srcs[k], n * e, hipMemcpyDeviceToHost));
-    if (!outs[k]) continue;
     unsigned char* o = (unsigned char*)outs[k];
+    if (keep) {
+      mom[k - 2].resize(n * e);
+      o = mom[k - 2].data();
+    }
     for (int64_t i = 0; i < n; ++i) std::memcpy(o + (size_t)tag[i] * e, buf.data() + (size_t)i * e, e);
+    if (keep && outs[k]) std::memcpy(outs[k], o, n * e);
   }
   for (int64_t i = 0; i < n; ++i) {
     if (ix) ix[tag[i]] = cell_ix(cell[i]);
@@ -1003,16 +1012,36 @@ int sp_get_species(sp_ctx* c, int id, int64_t n, int32_t* ix, int32_t* iy, void*
   if (psa) {  // src/prtls.rs:440 in Float
     for (int64_t i = 0; i < n; ++i) {
       if (c->f64) {
-        const double* X = (const double*)px; const double* Y = (const double*)py; const double* Z = (const double*)pz;
+        const double* X = (const double*)mom[0].data(); const double* Y = (const double*)mom[1].data();
+        const double* Z = (const double*)mom[2].data();
         const double cc = c->p.c, csq = 1. / (cc * cc);
         ((double*)psa)[i] = std::sqrt(1.0 + (X[i] * X[i] + Y[i] * Y[i] + Z[i] * Z[i]) * csq);
       } else {
-        const float* X = (const float*)px; const float* Y = (const float*)py; const float* Z = (const float*)pz;
+        const float* X = (const float*)mom[0].data(); const float* Y = (const float*)mom[1].data();
+        const float* Z = (const float*)mom[2].data();
         const float cc = (float)c->p.c, csq = 1.f / (cc * cc);
         ((float*)psa)[i] = std::sqrt(1.0f + (X[i] * X[i] + Y[i] * Y[i] + Z[i] * Z[i]) * csq);
       }
     }
   }
+  return SP_OK;
+}
+
+int sp_get_species_raw(sp_ctx* c, int id, int64_t n, int32_t* cell, int32_t* tag, void* dx, void* dy, void* px,
+                       void* py, void* pz) {
+  int rc = valid_species(c, id);
+  if (rc) return rc;
+  const SpeciesBuf& s = c->species[id];
+  if (n != s.n) return fail(c, SP_EINVAL, "count does not match the species size");
+  rc = flush(c);
+  if (rc) return rc;
+  rc = sync_check(c);
+  if (rc) return rc;
+  if (n == 0) return SP_OK;
+  void* outs[7] = {cell, tag, dx, dy, px, py, pz};
+  const void* srcs[7] = {s.cur.cell, s.cur.tag, s.cur.dx, s.cur.dy, s.cur.px, s.cur.py, s.cur.pz};
+  for (int k = 0; k < 7; ++k)
+    if (outs[k]) HIPCHK(c, hipMemcpy(outs[k], srcs[k], n * (k < 2 ? 4 : c->esz), hipMemcpyDeviceToHost));
   return SP_OK;
 }
 
@@ -1278,6 +1307,178 @@ int sp_get_diag(sp_ctx* c, sp_diag* out) {
   return SP_OK;
 }
 
+
+// ---- checkpoint (src/save/mod.rs:25-81 writes species 0 and the fields only;
+// a restart also needs every species and the persistent spectral B_wrk,
+// src/flds/mod.rs:31-33) ----
+namespace {
+
+struct CkptHeader {
+  char magic[8];  // "SPCKPT\0\1"
+  int32_t esz;    // 4 (SP_F32) or 8 (SP_F64)
+  int32_t nspecies;
+  sp_params params;
+  sp_tuning tuning;
+  int64_t steps;
+  double gauss_post_max;
+};
+const char CKPT_MAGIC[8] = {'S', 'P', 'C', 'K', 'P', 'T', 0, 1};
+constexpr size_t CKPT_CHUNK = (size_t)64 << 20;  // host staging per copy
+
+bool same_params(const sp_params& a, const sp_params& b) {
+  return a.size_x == b.size_x && a.size_y == b.size_y && a.delta == b.delta && a.dens == b.dens &&
+         a.n_pass == b.n_pass && a.dt == b.dt && a.c == b.c && a.gamma_inj == b.gamma_inj &&
+         a.precision == b.precision && a.periodic_x == b.periodic_x && a.checked == b.checked &&
+         a.left_edge == b.left_edge && a.sort_interval == b.sort_interval && a.study == b.study;
+}
+
+// device <-> file in chunks through one host buffer
+int dev_to_file(sp_ctx* c, FILE* f, const void* dev, size_t bytes, std::vector<unsigned char>& stage) {
+  for (size_t off = 0; off < bytes; off += CKPT_CHUNK) {
+    const size_t k = std::min(CKPT_CHUNK, bytes - off);
+    HIPCHK(c, hipMemcpy(stage.data(), (const unsigned char*)dev + off, k, hipMemcpyDeviceToHost));
+    if (std::fwrite(stage.data(), 1, k, f) != k) return fail(c, SP_EINVAL, "checkpoint: write failed");
+  }
+  return SP_OK;
+}
+// `check` (may be empty) sees every staged chunk of an int32 array before it
+// is copied to the device and rejects the file by returning false.
+int file_to_dev(sp_ctx* c, FILE* f, void* dev, size_t bytes, std::vector<unsigned char>& stage,
+                const std::function<bool(const int32_t*, size_t)>& check = {}) {
+  for (size_t off = 0; off < bytes; off += CKPT_CHUNK) {
+    const size_t k = std::min(CKPT_CHUNK, bytes - off);
+    if (std::fread(stage.data(), 1, k, f) != k) return fail(c, SP_EINVAL, "checkpoint: file is truncated");
+    if (check && !check((const int32_t*)stage.data(), k / 4))
+      return fail(c, SP_EINVAL, "checkpoint: particle cells or tags out of range");
+    HIPCHK(c, hipMemcpy((unsigned char*)dev + off, stage.data(), k, hipMemcpyHostToDevice));
+  }
+  return SP_OK;
+}
+
+struct FileCloser {
+  FILE* f;
+  ~FileCloser() {
+    if (f) std::fclose(f);
+  }
+};
+
+}  // namespace
+
+int sp_checkpoint_save(sp_ctx* c, const char* path) {
+  if (!c || !path) return SP_EINVAL;
+  int rc = flush(c);
+  if (rc) return rc;
+  rc = sync_check(c);
+  if (rc) return rc;
+  CkptHeader h{};
+  std::memcpy(h.magic, CKPT_MAGIC, 8);
+  h.esz = (int32_t)c->esz;
+  h.nspecies = (int32_t)c->species.size();
+  h.params = c->p;
+  h.tuning = c->tune;
+  h.steps = c->steps;
+  HIPCHK(c, hipMemcpy(&h.gauss_post_max, c->gauss_d + 3, sizeof(double), hipMemcpyDeviceToHost));
+  FileCloser fc{std::fopen(path, "wb")};
+  if (!fc.f) return fail(c, SP_EINVAL, std::string("checkpoint: cannot open ") + path);
+  if (std::fwrite(&h, sizeof(h), 1, fc.f) != 1) return fail(c, SP_EINVAL, "checkpoint: write failed");
+  std::vector<unsigned char> stage(CKPT_CHUNK);
+  for (const SpeciesBuf& s : c->species) {
+    const double cm[2] = {s.charge, s.mass};
+    if (std::fwrite(cm, sizeof(cm), 1, fc.f) != 1 || std::fwrite(&s.n, sizeof(s.n), 1, fc.f) != 1)
+      return fail(c, SP_EINVAL, "checkpoint: write failed");
+    // device order (the tile-sorted order the next passes read) with the tags
+    const void* arr[7] = {s.cur.cell, s.cur.tag, s.cur.dx, s.cur.dy, s.cur.px, s.cur.py, s.cur.pz};
+    for (int k = 0; k < 7; ++k)
+      if ((rc = dev_to_file(c, fc.f, arr[k], (size_t)s.n * (k < 2 ? 4 : c->esz), stage))) return rc;
+  }
+  for (int g = 0; g < 11; ++g)
+    if ((rc = dev_to_file(c, fc.f, c->grid[g], c->ngrid_elems() * c->esz, stage))) return rc;
+  for (int g = 0; g < 14; ++g)
+    if ((rc = dev_to_file(c, fc.f, c->spec[g], c->nspec_elems() * 2 * c->esz, stage))) return rc;
+  if (std::fflush(fc.f) != 0) return fail(c, SP_EINVAL, "checkpoint: write failed");
+  return SP_OK;
+}
+
+int sp_checkpoint_load(sp_ctx* c, const char* path) {
+  if (!c || !path) return SP_EINVAL;
+  if (c->ens) return fail(c, SP_EINVAL, "the context belongs to an ensemble");
+  if (!c->species.empty()) return fail(c, SP_EINVAL, "checkpoint: load into a context without species");
+  int rc = sync_check(c);
+  if (rc) return rc;
+  FileCloser fc{std::fopen(path, "rb")};
+  if (!fc.f) return fail(c, SP_EINVAL, std::string("checkpoint: cannot open ") + path);
+  CkptHeader h{};
+  if (std::fread(&h, sizeof(h), 1, fc.f) != 1 || std::memcmp(h.magic, CKPT_MAGIC, 8) != 0)
+    return fail(c, SP_EINVAL, "checkpoint: not a checkpoint file");
+  if (h.esz != (int32_t)c->esz || !same_params(h.params, c->p))
+    return fail(c, SP_EINVAL, "checkpoint: parameters differ from the context's");
+  if (h.nspecies < 0 || h.nspecies > MAX_SPECIES) return fail(c, SP_EINVAL, "checkpoint: bad species count");
+  if ((rc = check_tuning(c, h.tuning))) return rc;
+  std::vector<unsigned char> stage(CKPT_CHUNK);
+  for (int i = 0; i < h.nspecies; ++i) {
+    double cm[2];
+    int64_t n = 0;
+    if (std::fread(cm, sizeof(cm), 1, fc.f) != 1 || std::fread(&n, sizeof(n), 1, fc.f) != 1 || n < 0 ||
+        n > INT32_MAX) {
+      for (auto& s : c->species) free_species(s);
+      c->species.clear();
+      return fail(c, SP_EINVAL, "checkpoint: bad species record");
+    }
+    SpeciesBuf s;
+    s.charge = cm[0];
+    s.mass = cm[1];
+    rc = alloc_species(c, s, n);
+    if (rc) {
+      free_species(s);
+      for (auto& t : c->species) free_species(t);
+      c->species.clear();
+      return rc;
+    }
+    c->species.push_back(s);
+    void* arr[7] = {s.cur.cell, s.cur.tag, s.cur.dx, s.cur.dy, s.cur.px, s.cur.py, s.cur.pz};
+    // cells inside the padded grid (the kernels index the grids with them),
+    // tags a permutation of 0..n-1 (the getters place values by tag)
+    const int nxp = c->p.size_x + 1, nyp = c->p.size_y + 1;
+    auto cells_ok = [&](const int32_t* v, size_t m) {
+      for (size_t j = 0; j < m; ++j) {
+        const int ix = cell_ix(v[j]), iy = cell_iy(v[j]);
+        if (ix > nxp || iy < 0 || iy > nyp) return false;
+      }
+      return true;
+    };
+    std::vector<bool> seen((size_t)n, false);
+    auto tags_ok = [&](const int32_t* v, size_t m) {
+      for (size_t j = 0; j < m; ++j) {
+        if (v[j] < 0 || v[j] >= n || seen[(size_t)v[j]]) return false;
+        seen[(size_t)v[j]] = true;
+      }
+      return true;
+    };
+    for (int k = 0; k < 7; ++k) {
+      const std::function<bool(const int32_t*, size_t)> chk =
+          k == 0 ? std::function<bool(const int32_t*, size_t)>(cells_ok)
+                 : (k == 1 ? std::function<bool(const int32_t*, size_t)>(tags_ok) : nullptr);
+      if ((rc = file_to_dev(c, fc.f, arr[k], (size_t)n * (k < 2 ? 4 : c->esz), stage, chk))) {
+        for (auto& t : c->species) free_species(t);
+        c->species.clear();
+        return rc;
+      }
+    }
+  }
+  for (int g = 0; g < 11 && !rc; ++g) rc = file_to_dev(c, fc.f, c->grid[g], c->ngrid_elems() * c->esz, stage);
+  for (int g = 0; g < 14 && !rc; ++g) rc = file_to_dev(c, fc.f, c->spec[g], c->nspec_elems() * 2 * c->esz, stage);
+  if (rc) {
+    for (auto& t : c->species) free_species(t);
+    c->species.clear();
+    return rc;
+  }
+  apply_tuning(c, h.tuning);
+  c->steps = h.steps;
+  c->push_pending = false;
+  drop_graph(c);
+  HIPCHK(c, hipMemcpy(c->gauss_d + 3, &h.gauss_post_max, sizeof(double), hipMemcpyHostToDevice));
+  return SP_OK;
+}
 
 int sp_set_tuning(sp_ctx* c, const sp_tuning* t) {
   if (!c || !t) return SP_EINVAL;

@@ -13,7 +13,13 @@
 //              (unchecked = !checked), left_edge = "assert" | "reflect",
 //              sort_interval = 20 (steps between tile sorts of the particles;
 //              0 = never), autotune = false (time the launch shapes first)
-//   [output]   verbose = false (print the reference's per-step timer lines)
+//   [output]   verbose = false (print the reference's per-step timer lines),
+//              checkpoint_interval = 0 (every that many steps write
+//              output/checkpoint_NNNNN.ckpt, the restartable state of
+//              sp_checkpoint_save; 0 = never)
+//   [setup]    restart = "path" (continue the run saved in that checkpoint
+//              instead of initialising the particles; the loop resumes at
+//              the saved step)
 //
 // With several config files (spectral_rs a/config.toml b/config.toml ...)
 // the runs are independent members of one ensemble stepped together on the
@@ -98,6 +104,8 @@ struct Config {  // src/lib.rs:34-66 plus the optional extensions
   bool f64 = false, periodic = false, checked = true, autotune = false;
   int left_edge = SP_LEFT_ASSERT;
   int64_t sort_interval = 20;
+  int64_t checkpoint_interval = 0;
+  std::string restart;             // checkpoint to continue from
   std::string out_dir = "output";  // where save_output writes
 };
 
@@ -166,8 +174,10 @@ bool load_config(const std::string& path, Config& c, std::string& err) {
             get_num(t, "params.vth_ion", c.vth_ion, false, err) && get_num(t, "params.vth_lec", c.vth_lec, false, err) &&
             get_bool(t, "params.periodic", c.periodic, false, err) && get_bool(t, "params.checked", c.checked, false, err) &&
             get_bool(t, "params.autotune", c.autotune, false, err) &&
-            get_int(t, "params.sort_interval", c.sort_interval, false, err);
+            get_int(t, "params.sort_interval", c.sort_interval, false, err) &&
+            get_int(t, "output.checkpoint_interval", c.checkpoint_interval, false, err);
   if (!ok) return false;
+  c.restart = get_str(t, "setup.restart", "");
   int64_t seed = 0;
   if (!get_int(t, "params.seed", seed, false, err)) return false;
   c.seed = (uint64_t)seed;
@@ -316,7 +326,8 @@ template <class F> int save_output(sp_ctx* ctx, const Config& cfg, uint32_t t, s
   } while (0)
 
 // Flds::new + Sim::new + the two Prtl::new of run() (src/lib.rs:119-142):
-// context, species, pre-loop move & deposit; optionally the autotuner.
+// context, species, pre-loop move & deposit; optionally the autotuner.  With
+// a restart checkpoint the context takes the saved state instead.
 int setup(const Config& cfg, sp_ctx** out) {
   sp_params p{};
   p.size_x = (int32_t)cfg.size_x;
@@ -336,6 +347,12 @@ int setup(const Config& cfg, sp_ctx** out) {
   if (sp_create(&p, &ctx) != SP_OK) {
     std::printf("Application error: %s\n", sp_last_error(nullptr));
     return 1;
+  }
+  if (!cfg.restart.empty()) {
+    CHECK(sp_checkpoint_load(ctx, cfg.restart.c_str()));
+    std::printf("restarted from %s\n", cfg.restart.c_str());
+    *out = ctx;
+    return 0;
   }
   std::printf("initialzing  prtls\n");
   int sid = 0;
@@ -364,12 +381,25 @@ int save(sp_ctx* ctx, const Config& cfg, uint32_t t) {
 int run(const Config& cfg) {
   sp_ctx* ctx = nullptr;
   if (setup(cfg, &ctx)) return 1;
+  sp_diag d0{};
+  CHECK(sp_get_diag(ctx, &d0));  // 0, or the step a restart resumes at
+  const uint32_t t0 = (uint32_t)d0.steps;
   CHECK(sp_set_timing(ctx, 1));
   double ms[4];
-  for (uint32_t t = 0; t <= cfg.t_final; ++t) {
+  for (uint32_t t = t0; t <= cfg.t_final; ++t) {
     if (save(ctx, cfg, t)) {
       sp_destroy(ctx);
       return 1;
+    }
+    if (cfg.checkpoint_interval > 0 && t > t0 && t % cfg.checkpoint_interval == 0) {
+      char name[64];
+      std::snprintf(name, sizeof name, "/checkpoint_%05u.ckpt", (unsigned)t);
+      if (!mkdirs(cfg.out_dir)) {
+        std::printf("Application error: Unable to create output directory\n");
+        sp_destroy(ctx);
+        return 1;
+      }
+      CHECK(sp_checkpoint_save(ctx, (cfg.out_dir + name).c_str()));
     }
     CHECK(sp_step(ctx, 1));
     if (cfg.verbose) {
@@ -382,7 +412,7 @@ int run(const Config& cfg) {
   CHECK(sp_sync(ctx));
   CHECK(sp_get_timers(ctx, ms, 1));
   std::printf("done: %u steps; dep %.3f ms, solve %.3f ms, push %.3f ms (totals over the un-reported steps)\n",
-              cfg.t_final + 1, ms[0], ms[1], ms[2]);
+              cfg.t_final + 1 - t0, ms[0], ms[1], ms[2]);
   sp_destroy(ctx);
   return 0;
 }
@@ -458,6 +488,10 @@ int main(int argc, char** argv) {  // src/main.rs:3-13
     if (nfiles > 1) {  // each member writes beside its config file
       const size_t slash = path.find_last_of('/');
       cfg.out_dir = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/output";
+      if (!cfg.restart.empty()) {
+        std::printf("Problem parsing arguments: restart is for a single run, not an ensemble member\n");
+        return 1;
+      }
       if (!cfgs.empty() && cfg.t_final != cfgs[0].t_final) {
         std::printf("Problem parsing arguments: ensemble members must share t_final\n");
         return 1;

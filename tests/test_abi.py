@@ -4,6 +4,7 @@ Config::new rejects (src/lib.rs:75-113) before touching a GPU."""
 import ctypes as ct
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -62,3 +63,37 @@ def test_null_arguments_are_rejected():
     assert L.sp_create(None, None) == capi.SP_EINVAL
     assert L.sp_sync(None) == capi.SP_EINVAL
     assert L.sp_step(None, 1) == capi.SP_EINVAL
+
+
+def _c_layout(struct, fields, tmp_path):
+    """sizeof and offsetof of a header struct, from a C compiler."""
+    src = tmp_path / "layout.c"
+    body = "".join(f'  printf("%zu\\n", offsetof({struct}, {f}));\n' for f in fields)
+    src.write_text(f'#include <stdio.h>\n#include <stddef.h>\n#include "spectral_rs.h"\n'
+                   f'int main(void) {{\n  printf("%zu\\n", sizeof({struct}));\n{body}  return 0;\n}}\n')
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    out = [int(v) for v in subprocess.check_output([str(exe)]).split()]
+    return out[0], out[1:]
+
+
+@pytest.mark.parametrize("struct,cls", [("sp_params", capi.SpParams), ("sp_diag", capi.SpDiag),
+                                        ("sp_tuning", capi.SpTuning)])
+def test_struct_layouts_match_the_header(struct, cls, tmp_path):
+    names = [n for n, _ in cls._fields_]
+    size, offs = _c_layout(struct, names, tmp_path)
+    assert ct.sizeof(cls) == size
+    assert [getattr(cls, n).offset for n in names] == offs
+
+
+def test_documented_rust_binding_mirrors_sp_params():
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    body = re.search(r"pub struct SpParams \{(.*?)\}", txt, flags=re.S).group(1)
+    rust = re.findall(r"pub (\w+): (\[i32; \d+\]|i32|f64)", body)
+    sizes = {"i32": 4, "f64": 8}
+    ctype = {ct.c_int32: "i32", ct.c_double: "f64"}
+    want = []
+    for n, t in capi.SpParams._fields_:
+        want.append((n, ctype[t] if t in ctype else f"[i32; {t._length_}]"))
+    assert rust == want
+    assert sum(sizes.get(t, 4 * int(t[6:-1]) if t.startswith("[") else 0) for _, t in rust) + 4 == ct.sizeof(capi.SpParams)

@@ -120,3 +120,48 @@ def test_host_ensemble_writes_each_members_output(tmp_path):
             x = np.load(d / "output" / f"dat_{t:05d}" / "x.npy")
             assert x.size == n
         assert np.load(d / "output" / "dat_00003" / "flds" / "Bz.npy").size in (48 * 24, 96 * 32)
+
+
+def test_restart_is_refused_for_ensemble_members(tmp_path):
+    a = write_cfg(tmp_path / "a")
+    b = write_cfg(tmp_path / "b")
+    txt = open(b).read().replace("[setup]", '[setup]\nrestart = "x.ckpt"')
+    open(b, "w").write(txt)
+    if not os.path.exists(B.HOST):
+        B.build_host()
+    r = subprocess.run([B.HOST, a, b], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "restart is for a single run" in r.stdout
+
+
+@pytest.mark.gpu
+def test_host_checkpoint_restart_continues_the_run(tmp_path):
+    # FP64: a run that writes checkpoints every 5 steps, and a second run that
+    # restarts from step 5; their outputs of steps 5..12 agree (cells exactly,
+    # the rest to the FP64 deposit's summation order)
+    extra = "n_pass = 4\nprecision = \"f64\"\nsort_interval = 3"
+    a = write_cfg(tmp_path / "a", t_final=12)
+    open(a, "w").write(open(a).read().replace("n_pass = 4", extra).replace(
+        "istep = 1", "istep = 1\ncheckpoint_interval = 5"))
+    if not os.path.exists(B.HOST):
+        B.build_host()
+    r = subprocess.run([B.HOST, a], cwd=tmp_path / "a", capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    ck = tmp_path / "a" / "output" / "checkpoint_00005.ckpt"
+    assert ck.exists() and (tmp_path / "a" / "output" / "checkpoint_00010.ckpt").exists()
+    b = write_cfg(tmp_path / "b", t_final=12)
+    open(b, "w").write(open(b).read().replace("n_pass = 4", extra).replace("[setup]", f'[setup]\nrestart = "{ck}"'))
+    r = subprocess.run([B.HOST, b], cwd=tmp_path / "b", capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    assert "restarted from" in r.stdout
+    assert not (tmp_path / "b" / "output" / "dat_00004").exists()
+    for t in (5, 9, 12):
+        da, db = (tmp_path / d / "output" / f"dat_{t:05d}" for d in ("a", "b"))
+        for f in ("x", "y"):
+            xa, xb = np.load(da / f"{f}.npy"), np.load(db / f"{f}.npy")
+            assert xa.dtype == np.float64
+            assert np.array_equal(np.floor(xa), np.floor(xb)) and np.max(np.abs(xa - xb)) < 1e-11
+        for f in ("u", "gam"):
+            assert np.max(np.abs(np.load(da / f"{f}.npy") - np.load(db / f"{f}.npy"))) < 1e-11
+        for f in ("Ex", "By", "Jx", "Dens"):
+            fa, fb = np.load(da / "flds" / f"{f}.npy"), np.load(db / "flds" / f"{f}.npy")
+            assert np.max(np.abs(fa - fb)) <= 1e-11 * max(np.max(np.abs(fa)), 1e-30), (t, f)
