@@ -84,10 +84,15 @@ def test_fp64_continued_run_equals_the_uninterrupted_run(tmp_path):
     assert a.diag()["steps"] == b.diag()["steps"] == 18
 
 
-def test_fp32_continued_run_matches_to_summation_order(tmp_path):
-    # FP32 windows sum in fixed point (order-free), but the window flush adds
-    # to the grids with float atomics in a varying order: after 3 steps the two
-    # runs differ by FP32 rounding of the deposit only.
+def test_fp32_continued_step_matches_to_summation_order(tmp_path):
+    # One step after the load: the move reads only saved state, so cells and
+    # offsets are bit-equal; the deposit windows sum in fixed point (order-
+    # free) but are flushed to the grids with float atomics in a varying
+    # order, so J, rho and dens agree to FP32 rounding of the summed
+    # contributions: each species' current is up to |q| c times its density
+    # (|v| < c), ~80 per cell here, while ion and electron currents cancel to
+    # ~0.04 net, so the bound is a few FP32 ulps of c max(dens), not of |J|.
+    # (The fields are judged through J for the same reason.)
     a = shock(4)
     a.step(5)
     path = os.fspath(tmp_path / "run.ckpt")
@@ -95,10 +100,17 @@ def test_fp32_continued_run_matches_to_summation_order(tmp_path):
     b = capi.GpuSim(prec=4, **KW)
     b.checkpoint_load(path)
     for s in (a, b):
-        s.step(3)
-    for n in ("ex", "bz", "jx", "rho"):
+        s.step(1)
+    for sid in range(2):
+        x, y = a.species_raw(sid), b.species_raw(sid)
+        for k in ("cell", "tag", "dx", "dy"):
+            assert np.array_equal(x[k], y[k]), (sid, k)
+    eps = float(np.finfo(np.float32).eps)
+    dmax = float(np.max(a.grid("dens")))
+    for n in ("jx", "jy", "jz", "rho", "dens"):
         x, y = a.grid(n).astype(np.float64), b.grid(n).astype(np.float64)
-        assert np.max(np.abs(x - y)) <= 1e-4 * max(np.max(np.abs(x)), 1e-30), n
+        scale = dmax * (KW["c"] if n.startswith("j") else 1.0)
+        assert np.max(np.abs(x - y)) <= 8 * eps * scale, (n, np.max(np.abs(x - y)) / (eps * scale))
 
 
 def test_load_rejects_mismatched_or_damaged_files(tmp_path):

@@ -140,8 +140,8 @@ def test_host_checkpoint_restart_continues_the_run(tmp_path):
     # the rest to the FP64 deposit's summation order)
     extra = "n_pass = 4\nprecision = \"f64\"\nsort_interval = 3"
     a = write_cfg(tmp_path / "a", t_final=12)
-    open(a, "w").write(open(a).read().replace("n_pass = 4", extra).replace(
-        "istep = 1", "istep = 1\ncheckpoint_interval = 5"))
+    txt = open(a).read().replace("n_pass = 4", extra).replace("istep = 1", "istep = 1\ncheckpoint_interval = 5")
+    open(a, "w").write(txt)
     if not os.path.exists(B.HOST):
         B.build_host()
     r = subprocess.run([B.HOST, a], cwd=tmp_path / "a", capture_output=True, text=True, timeout=300)
@@ -149,7 +149,8 @@ def test_host_checkpoint_restart_continues_the_run(tmp_path):
     ck = tmp_path / "a" / "output" / "checkpoint_00005.ckpt"
     assert ck.exists() and (tmp_path / "a" / "output" / "checkpoint_00010.ckpt").exists()
     b = write_cfg(tmp_path / "b", t_final=12)
-    open(b, "w").write(open(b).read().replace("n_pass = 4", extra).replace("[setup]", f'[setup]\nrestart = "{ck}"'))
+    txt = open(b).read().replace("n_pass = 4", extra).replace("[setup]", f'[setup]\nrestart = "{ck}"')
+    open(b, "w").write(txt)
     r = subprocess.run([B.HOST, b], cwd=tmp_path / "b", capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout
     assert "restarted from" in r.stdout
