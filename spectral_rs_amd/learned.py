@@ -26,6 +26,7 @@ import time
 import numpy as np
 
 from spectral_rs_amd import capi
+from spectral_rs_amd.analysis import profile, profile_error, spectrum_error, xspectrum
 
 SHOCK = dict(n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0)
 SPECIES = [(1.0, 16.0, 0.0025), (-1.0, 1.0, 0.01)]  # ions then electrons
@@ -171,17 +172,6 @@ def evaluate(p, X, T):
     return {"channels": CH, "binomial_rel_mse": list(base / n / ms), "learned_rel_mse": list(learned / n / ms)}
 
 
-def profile(sim, ppc):
-    """Density averaged over y, relative to the upstream ppc."""
-    return sim.grid("dens")[1:-1, 1:-1].astype(np.float64).mean(axis=0) / ppc
-
-
-def xspectrum(sim, name):
-    """Power of the field along x, averaged over rows: |FFT_x|^2 for kx = 0..Nx/2."""
-    f = sim.grid(name)[1:-1, 1:-1].astype(np.float64)
-    return np.mean(np.abs(np.fft.rfft(f, axis=1)) ** 2, axis=0)
-
-
 def run_case(size_x, size_y, ppc, steps, params=None, seed=0, max_rel=1.0, mask=0xF):
     sim = shock_sim(size_x, size_y, ppc, seed)
     if params is not None:
@@ -201,18 +191,6 @@ def run_case(size_x, size_y, ppc, steps, params=None, seed=0, max_rel=1.0, mask=
     stats = sim.denoiser_stats() if params is not None else None
     sim.close()
     return el, prof, stats, spec
-
-
-def profile_error(prof, truth, delta):
-    a, b = prof[delta:-delta], truth[delta:-delta]
-    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
-
-
-def spectrum_error(spec, truth):
-    """Relative L2 error of a power spectrum over kx = 1 .. Nx/4 (the modes the
-    binomial filter leaves)."""
-    k = slice(1, (truth.size - 1) // 2 + 1)
-    return float(np.linalg.norm(spec[k] - truth[k]) / np.linalg.norm(truth[k]))
 
 
 def main():
