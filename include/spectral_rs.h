@@ -164,6 +164,17 @@ int sp_boris_push(sp_ctx* ctx, int id);
 /* `passes` calls of Field::binomial_filter_2_d on one grid
  * (src/flds/field.rs:375-426). */
 int sp_filter(sp_ctx* ctx, int grid, int passes);
+/* sp_filter without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  `passes` calls of
+ * Field::binomial_filter_2_d on a caller's padded (size_y+2) x (size_x+2)
+ * device grid of float (SP_F32) or double (SP_F64), in place, queued on
+ * `stream` (a hipStream_t; NULL = the null stream) without waiting.  The
+ * workspace is a device array inside the library (33,685,504 bytes per
+ * device, held until the process ends): grids larger than that return
+ * SP_EINVAL.  Calls that overlap on one device share the workspace and race.
+ * It calls no hipMalloc or hipFree; errors read with sp_last_error(NULL). */
+int sp_filter_noalloc(void* grid, int32_t size_x, int32_t size_y, int precision, int passes, void* stream);
 /* Fft2D::fft / Fft2D::inv_fft on one spectrum (src/flds/fft_2d.rs:91-172). */
 int sp_fft2d(sp_ctx* ctx, int spec, int inverse);
 /* The pointwise spectral part of Flds::update (src/flds/mod.rs:105-278) on

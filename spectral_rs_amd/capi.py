@@ -24,7 +24,7 @@ GRIDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5, "jx": 6, "jy": 7,
 SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_species_init",
            "sp_species_count", "sp_get_species", "sp_get_species_raw", "sp_get_grid", "sp_set_grid", "sp_get_spectrum",
            "sp_set_spectrum", "sp_zero_currents", "sp_move_and_deposit", "sp_deposit_ghosts",
-           "sp_fields_update", "sp_boris_push", "sp_filter", "sp_fft2d", "sp_spectral_update",
+           "sp_fields_update", "sp_boris_push", "sp_filter", "sp_filter_noalloc", "sp_fft2d", "sp_spectral_update",
            "sp_prestep", "sp_step", "sp_sync", "sp_set_timing", "sp_get_timers", "sp_set_graph",
            "sp_get_diag", "sp_checkpoint_save", "sp_checkpoint_load", "sp_set_tuning", "sp_get_tuning", "sp_autotune", "sp_ensemble_create",
            "sp_ensemble_destroy", "sp_ensemble_last_error", "sp_ensemble_set_graph", "sp_ensemble_step",
@@ -96,6 +96,7 @@ def load(path=LIB_PATH):
     for n in ["sp_move_and_deposit", "sp_boris_push", "sp_step", "sp_set_timing", "sp_set_graph"]:
         getattr(L, n).argtypes = [vp, i32]
     L.sp_filter.argtypes = [vp, i32, i32]
+    L.sp_filter_noalloc.argtypes = [vp, ct.c_int32, ct.c_int32, i32, i32, vp]
     L.sp_fft2d.argtypes = [vp, i32, i32]
     L.sp_get_timers.argtypes = [vp, vp, i32]
     L.sp_get_diag.argtypes = [vp, ct.POINTER(SpDiag)]

@@ -1129,6 +1129,71 @@ int sp_filter(sp_ctx* c, int grid, int passes) {
   return SP_OK;
 }
 
+}  // extern "C"
+
+// ---- sp_filter without allocation ----
+// The filter's workspace (the grid each fused launch writes) lives in a
+// file-scope device array instead of a context's work grid: one padded grid
+// of up to SCRATCH_BYTES, the 4096 x 1024 grid in either precision.  Each
+// device holds its own copy (33.7 MB) for the life of the process; its
+// address is taken once per device with that device current.  Every call on
+// a device shares that device's copy, so calls that overlap on one device
+// (other streams or host threads) race.
+namespace {
+constexpr size_t SCRATCH_BYTES = (size_t)4210688 * 8;  // >= 4098 x 1026 doubles
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[SCRATCH_MAX_DEVICES] = {};
+
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(nullptr, SP_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail(nullptr, SP_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(nullptr, SP_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return SP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sp_filter_noalloc(void* grid, int32_t size_x, int32_t size_y, int precision, int passes, void* stream) {
+  if (!grid) return fail(nullptr, SP_EINVAL, "null grid");
+  if (size_x <= 0 || size_y <= 0 || size_x % 2 != 0 || size_y % 2 != 0)
+    return fail(nullptr, SP_EINVAL, "grid sizes must be positive and even");
+  if (precision != SP_F32 && precision != SP_F64) return fail(nullptr, SP_EINVAL, "unknown precision");
+  if (passes < 0) return fail(nullptr, SP_EINVAL, "negative pass count");
+  const size_t esz = precision == SP_F64 ? 8 : 4;
+  const size_t bytes = (size_t)(size_x + 2) * (size_t)(size_y + 2) * esz;
+  if (bytes > SCRATCH_BYTES)
+    return fail(nullptr, SP_EINVAL,
+                "grid of " + std::to_string(bytes) + " bytes exceeds the " + std::to_string(SCRATCH_BYTES) +
+                    "-byte workspace of sp_filter_noalloc");
+  void* work = nullptr;
+  int rc = scratch_address(&work);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  void* g = grid;
+  if (precision == SP_F64) launch_binomial<double>(&g, &work, 1, size_x, size_y, passes, st);
+  else launch_binomial<float>(&g, &work, 1, size_x, size_y, passes, st);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(nullptr, SP_EHIP, std::string("filter launch: ") + hipGetErrorString(e));
+  if (g != grid) {  // an odd number of launches left the result in the workspace
+    e = hipMemcpyAsync(grid, g, bytes, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return fail(nullptr, SP_EHIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+  }
+  return SP_OK;
+}
+
 int sp_fft2d(sp_ctx* c, int spec, int inverse) {
   ENTRY_PRELUDE(c);
   if (spec < 0 || spec >= 14) return fail(c, SP_EINVAL, "unknown spectrum id");
