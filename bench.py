@@ -25,21 +25,50 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 
-C2 = dict(size_x=4096, size_y=1024, delta=16, dens=32, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0)
-SPECIES = [(1.0, 16.0, 0.0025), (-1.0, 1.0, 0.01)]  # (q, m, vth): ions then electrons (lib.rs:124-126)
+from spectral_rs_amd.workloads import C2, SPECIES, ensemble_runs, particles  # noqa: E402
 
 
-def particles(cfg):
-    return 2 * cfg["dens"] * (cfg["size_x"] - 2 * cfg["delta"]) * cfg["size_y"]
+def crop_error():
+    """The FP32 GPU path on the 512 x 128 slab of this shock (32 ppc per
+    species, 3,932,160 particles, 1000 steps) against the FP64 (dprec)
+    oracle's snapshots at steps 100, 500 and 1000 (tests/golden, written by
+    tests/golden/make_shock_snapshots.py), next to the FP32 oracle's own
+    errors.  Untimed; None when the snapshots are absent."""
+    import numpy as np
+
+    from spectral_rs_amd import capi
+    from spectral_rs_amd.analysis import HORIZON_CASES, SHOCK_PHYS, horizon_errors, inputs_digest, shock_inputs
+
+    path = os.path.join(ROOT, "tests", "golden", "shock_oracle_snapshots.npz")
+    if not os.path.exists(path):
+        return None
+    k = HORIZON_CASES["c2_crop"]
+    sp = shock_inputs("c2_crop")
+    with np.load(path, allow_pickle=False) as z:
+        if inputs_digest(sp) != str(z["c2_crop_inputs_sha256"]):
+            return None
+    g = capi.GpuSim(prec=4, sort_interval=20, left_edge=k["left_edge"], size_x=k["size_x"], size_y=k["size_y"],
+                    delta=k["delta"], dens=k["dens"], **SHOCK_PHYS)
+    for q, m, a in sp:
+        g.add_species(q, m, **a)
+    g.prestep()
+    res = horizon_errors(g, "c2_crop", path)
+    g.close()
+    return {"workload": "512x128 slab of this shock, 32 ppc/species, identical seeded inputs on both sides",
+            "reference": "FP64 (dprec) oracle",
+            "metrics": "relative L2: y-averaged density profile, interior Ex Ey Bz, x power spectra of Ex Bz",
+            "steps": {str(r["step"]): {"gpu_fp32": r["errors"], "oracle_fp32": r["oracle_fp32_errors"]}
+                      for r in res}}
 
 
 def cpu_baseline(seconds_hint=15.0):
     """The oracle (restated reference CPU path, f32, OpenMP like rayon) on a
-    512 x 128 crop of the same physics with 32 ppc per species."""
+    512 x 128 crop of the same physics with 32 ppc per species, on the host's
+    threads and again on one thread."""
     import numpy as np
 
     from oracle import oracle as O
-    from tests.helpers import shock_species
+    from spectral_rs_amd.inputs import shock_species
 
     cfg = dict(C2, size_x=512, size_y=128)
     r = O.RefSim(prec=4, **cfg)
@@ -61,29 +90,18 @@ def cpu_baseline(seconds_hint=15.0):
     except AttributeError:
         pass
     thr = int(os.environ.get("OMP_NUM_THREADS", cores))
+    # one thread: the reference's rayon regions serialised
+    O.set_threads(1)
+    steps1 = max(2, steps // 4)
+    t0 = time.perf_counter()
+    r.step(steps1)
+    el1 = time.perf_counter() - t0
+    O.set_threads(thr)
     return {"value": n * steps / el, "unit": "particle-steps/s", "cores": thr, "kind": "port",
             "sample": f"oracle f32, 512x128 crop, 32 ppc/species ({n} particles), {steps} steps, "
-                      f"{el:.1f} s; serial mover/deposit, threaded push/filter/FFT as the reference"}
-
-
-def ensemble_runs(count=64, seed=0):
-    """The batched-ensemble workload: `count` independent shock runs drawn with
-    `seed` -- grid from {128,256,512,1024} x {32,64,128,256}, ppc from
-    {4,8,16,32}, gamma0 uniform in [2,20], temperature T log-uniform in
-    [1e-5,1e-3] (electron vth = sqrt(T), ion vth = sqrt(T/16))."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    runs = []
-    for _ in range(count):
-        nx = int(rng.choice([128, 256, 512, 1024]))
-        ny = int(rng.choice([32, 64, 128, 256]))
-        ppc = int(rng.choice([4, 8, 16, 32]))
-        g0 = float(rng.uniform(2.0, 20.0))
-        temp = float(np.exp(rng.uniform(np.log(1e-5), np.log(1e-3))))
-        runs.append(dict(size_x=nx, size_y=ny, delta=8 if nx <= 256 else 16, dens=ppc, n_pass=4, dt=0.045,
-                         c=10.0, gamma_inj=g0, temp=temp))
-    return runs
+                      f"{el:.1f} s; serial mover/deposit, threaded push/filter/FFT as the reference",
+            "single_thread": {"value": n * steps1 / el1, "cores": 1, "sample": f"the same, {steps1} more steps, "
+                                                                              f"{el1:.1f} s"}}
 
 
 def bench_ensemble(args):
@@ -183,6 +201,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-crop", action="store_true", help="skip the 512x128 comparison with the FP64 oracle")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--size", default=None, help="override grid, e.g. 1024x256 (not the metric workload)")
     ap.add_argument("--sort-interval", type=int, default=20, help="steps between tile sorts of the particles")
@@ -289,6 +308,10 @@ def main():
                    "particles": n, "grid": [cfg["size_x"], cfg["size_y"]],
                    "tuning": tuning or sim.tuning()},
         "phases_ms_per_step": {k: v / args.steps for k, v in tm.items()},
+        "window": (f"{args.steps} steps: {args.steps // args.sort_interval if args.sort_interval else 0} tile sorts "
+                   "(one per sort interval, as in a long run), and one stand-alone Boris push at the end (the "
+                   "last step's push, otherwise fused into the next step's particle pass); the roofline "
+                   "duration is the fused particle pass alone"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_particles<float, true> (fused push + move + deposit, one species per launch)",
@@ -296,6 +319,10 @@ def main():
                      "ms_per_launch": part_ms},
     }
     sim.close()
+    if not args.size and not args.no_crop:
+        ce = crop_error()
+        if ce is not None:
+            out["fp64_oracle_error"] = ce
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -71,3 +71,54 @@ def load_snapshots(path, case):
     snaps = [{k: d[k][i] for k in ("profile", "spec_ex", "spec_bz", "ex", "ey", "bz", "energy")}
              for i in range(len(steps))]
     return steps, snaps, d
+
+
+# ---- long-horizon comparisons against stored snapshots of another run ----
+# The shock workloads of the stored FP64 snapshots (reference units: c = 10,
+# dt = 0.045 -> 0.45 cells/step, 10-cell skin depth; gamma0 = 10; 4 filter
+# passes; ions q = +1, m = 16, vth 0.0025, then electrons q = -1, m = 1,
+# vth 0.01, inputs from spectral_rs_amd.inputs.shock_species with seeds 0, 1).
+SHOCK_PHYS = dict(n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0)
+SHOCK_SPECIES = [(1.0, 16.0, 0.0025, 0), (-1.0, 1.0, 0.01, 1)]
+HORIZON_CASES = {
+    # the 512 x 128 slab of the 4096 x 1024 shock (32 ppc, 3,932,160 particles)
+    "c2_crop": dict(size_x=512, size_y=128, delta=16, dens=32, steps=[100, 500, 1000], left_edge=1),
+    # the 256 x 64 shock at 16 ppc (491,520 particles), to the left-edge assert
+    "c1": dict(size_x=256, size_y=64, delta=8, dens=16, steps=[100, 300], left_edge=0, until_assert=1000),
+}
+
+
+def shock_inputs(case, ft=np.float32):
+    """[(charge, mass, arrays)] of a HORIZON_CASES workload."""
+    from spectral_rs_amd.inputs import shock_species
+    k = HORIZON_CASES[case]
+    return [(q, m, shock_species(k["size_x"], k["size_y"], k["delta"], k["dens"], SHOCK_PHYS["c"],
+                                 SHOCK_PHYS["gamma_inj"], vth, seed=sd, ft=ft)) for q, m, vth, sd in SHOCK_SPECIES]
+
+
+def inputs_digest(sp):
+    import hashlib
+    h = hashlib.sha256()
+    for _, _, a in sp:
+        for key in ("ix", "iy", "dx", "dy", "px", "py", "pz"):
+            h.update(np.ascontiguousarray(a[key]).tobytes())
+    return h.hexdigest()
+
+
+def horizon_errors(sim, case, path):
+    """Steps `sim` (species added, pre-step done, at step 0) through the
+    snapshot steps of `case` stored in `path` and returns, per step, its
+    errors against them (snapshot_errors keys) and the stored FP32-oracle
+    errors of the same step."""
+    k = HORIZON_CASES[case]
+    steps, snaps, d = load_snapshots(path, case)
+    out, t = [], 0
+    for i, st in enumerate(steps):
+        sim.step(st - t)
+        t = st
+        m = shock_metrics(sim, k["dens"])
+        f = {n: sim.grid(n)[1:-1, 1:-1] for n in ("ex", "ey", "bz")}
+        e = snapshot_errors(snaps[i], m, f, k["delta"])
+        ref = {key: float(d[f"fp32_oracle_{key}"][i]) for key in e}
+        out.append({"step": st, "errors": e, "oracle_fp32_errors": ref})
+    return out

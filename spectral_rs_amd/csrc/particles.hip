@@ -79,6 +79,10 @@ __device__ __forceinline__ void move_half(int& ix, int& iy, T& dx, T& dy, T& px,
   dx += c1 * px;
   dy += c1 * py;
   if (sizeof(T) == 4 && g.abs_pos32) {
+    // A wave-uniform branch taken only by the precision study: the empty
+    // volatile asm keeps the compiler from computing the rounding on every
+    // position update and selecting it away (16 VALU per particle).
+    __asm__ volatile("" ::: "memory");
     // precision study: the position held as one absolute FP32 coordinate
     // (x = ix + dx rounded to float), the offset read back exactly
     dx = (T(ix) + dx) - T(ix);
@@ -156,7 +160,7 @@ template <class T>
 __device__ __forceinline__ void gather6(const GridSet& gs, const Geo<T>& g, int ix, int iy, const T (&w)[9],
                                         T (&f)[6]) {
   const int P = g.pitch;
-  const int base = (iy - 1) * P + ix - 1;
+  const int base = __mul24(iy - 1, P) + ix - 1;
   const int off[9] = {0, 1, 2, P, P + 1, P + 2, 2 * P, 2 * P + 1, 2 * P + 2};
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
@@ -173,7 +177,7 @@ __device__ __forceinline__ void gather6(const FieldBufs& fb, const Geo<float>& g
   // grid has < 2^29 bytes; the mask states that and tells the compiler that vo + 8
   // cannot wrap, so the column offsets fold into the load's immediate and each
   // stencil row becomes one 3-dword load
-  const int vo = (((iy - 1) * g.pitch + ix - 1) * (int)sizeof(float)) & 0x1FFFFFFC;
+  const int vo = ((__mul24(iy - 1, g.pitch) + ix - 1) * (int)sizeof(float)) & 0x1FFFFFFC;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     float a[9];
@@ -269,7 +273,9 @@ template <class T> struct Sink {
     return use_lds && ix - 1 >= x0 && ix + 1 < x0 + w && iy - 1 >= y0 && iy + 1 < y0 + h;
   }
   // window cell of the stencil's corner (ix - 1, iy - 1)
-  __device__ __forceinline__ int corner(int ix, int iy) const { return (iy - 1 - y0) * w + (ix - 1 - x0); }
+  // 24-bit multiply (full rate; a 32-bit one is quarter rate): |iy - 1 - y0| and
+  // w stay below 2^16
+  __device__ __forceinline__ int corner(int ix, int iy) const { return __mul24(iy - 1 - y0, w) + (ix - 1 - x0); }
 };
 
 // Adds w[k] * v to the 3x3 stencil around (ix, iy) of one plane
@@ -285,7 +291,7 @@ __device__ __forceinline__ void add9(const Sink<T>& k, int lplane, int gplane, i
 #pragma unroll
       for (int c = 0; c < 3; ++c) lds_add<typename Acc<T>::type>(a + r * k.w + c, w[3 * r + c] * v);
   } else {
-    glb_ptr<T> a = k.g[gplane] + (iy - 1) * k.pitch + (ix - 1);
+    glb_ptr<T> a = k.g[gplane] + __mul24(iy - 1, k.pitch) + (ix - 1);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -589,7 +595,7 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
         const int32_t c = cell_of(j);
         const int cx = cell_ix(c) - k.x0, cy = cell_iy(c) - k.y0;
         if (cx >= 0 && cx < k.w && cy >= 0 && cy < k.h)  // particles outside deposit globally
-          __hip_atomic_fetch_add(cntg + cy * k.w + cx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(cntg + __mul24(cy, k.w) + cx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       __syncthreads();
       int m = 0;

@@ -24,7 +24,6 @@ and the energies; and the FP32 oracle's own errors against it -- the yardstick
 for the GPU's FP32 errors, which come from the deposit's summation order and
 the FFT instead of the serial loop.
 """
-import hashlib
 import os
 import sys
 import time
@@ -35,30 +34,21 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 
 from oracle import oracle as O  # noqa: E402
-from spectral_rs_amd.analysis import shock_metrics, snapshot_errors  # noqa: E402
-from tests.helpers import energies, shock_species  # noqa: E402
+from spectral_rs_amd.analysis import (HORIZON_CASES, SHOCK_PHYS, inputs_digest, shock_inputs,  # noqa: E402
+                                     shock_metrics, snapshot_errors)
+from tests.helpers import energies  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "shock_oracle_snapshots.npz")
-SPECIES = [(1.0, 16.0, 0.0025, 0), (-1.0, 1.0, 0.01, 1)]
-CASES = {
-    "c2_crop": dict(size_x=512, size_y=128, delta=16, dens=32, steps=[100, 500, 1000], left_edge=1),
-    "c1": dict(size_x=256, size_y=64, delta=8, dens=16, steps=[100, 300], left_edge=0, until_assert=1000),
-}
-PHYS = dict(n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0)
+CASES = HORIZON_CASES
+PHYS = SHOCK_PHYS
 
 
 def inputs(case):
-    k = CASES[case]
-    return [(q, m, shock_species(k["size_x"], k["size_y"], k["delta"], k["dens"], PHYS["c"], PHYS["gamma_inj"],
-                                 vth, seed=sd)) for q, m, vth, sd in SPECIES]
+    return shock_inputs(case)
 
 
 def digest(sp):
-    h = hashlib.sha256()
-    for _, _, a in sp:
-        for key in ("ix", "iy", "dx", "dy", "px", "py", "pz"):
-            h.update(np.ascontiguousarray(a[key]).tobytes())
-    return h.hexdigest()
+    return inputs_digest(sp)
 
 
 def oracle_run(case, prec, sp):
@@ -67,6 +57,10 @@ def oracle_run(case, prec, sp):
               left_edge=k["left_edge"])
     s = O.RefSim(prec=prec, **kw)
     for q, m, a in sp:
+        # gamma (psa) from the momenta in the run's own precision, as
+        # initialize_velocities does (src/prtls.rs:248-249): the float32 psa
+        # of the inputs would put a float32 rounding into the dprec run
+        a = {kk: v for kk, v in a.items() if kk != "psa"}
         s.add_species(q, m, **({kk: (v.astype(np.float64) if v.dtype == np.float32 else v) for kk, v in a.items()}
                                if prec == 8 else a))
     s.prestep()

@@ -1,0 +1,157 @@
+"""The C3, C4 and C5 workloads at their stated sizes (marked gpu), checked
+through properties that hold at any size and against reference runs:
+
+* C3, the precision-study shock (2048 x 512, 64 ppc per species, 132,120,576
+  particles): conservation properties of the product path, and its density
+  profile against the FP64 path next to an all-FP32 code's (the FP64 GPU path
+  itself follows the dprec oracle to ~1e-9 in the profile, test_gpu_horizon);
+* C4, 64 independent runs drawn from the stated distribution, 1000 steps in
+  one ensemble with captured graphs and autotuned shapes: every member's
+  particle count, density sum, Gauss law and energy, and a sample of members
+  against the same runs stepped alone;
+* C5, inference at 4096 x 1024 with 4 ppc per species (33,292,288 particles):
+  the trained network in the loop, its application and fallback counts, and
+  the run's conservation properties.
+
+Properties: every particle deposits a total TSC weight of 1 (the folded
+density grid sums to the particle count); the Gauss correction leaves FP32
+round-off; no reference assert fires (the left edge is the mirrored wall,
+which these slabs reach within their run lengths).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from spectral_rs_amd import capi
+from spectral_rs_amd.analysis import profile, profile_error
+from spectral_rs_amd.workloads import C3, C5, SPECIES, ensemble_runs, particles
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def total_energy(d):
+    return d["energy_e"] + d["energy_b"] + sum(d["energy_kin"])
+
+
+def check_properties(sim, n_particles, steps):
+    sim.sync()  # raises SP_EINVARIANT if a particle left the grid
+    dens = sim.grid("dens")[1:-1, 1:-1].astype(np.float64).sum()
+    assert abs(dens - n_particles) <= 1e-6 * n_particles, dens
+    d = sim.diag()
+    assert d["steps"] == steps
+    assert d["gauss_post_max"] < 1e-6, d
+    assert np.isfinite(total_energy(d))
+    return d
+
+
+def test_c3_precision_shock_at_full_size():
+    steps = 200
+    sims = {}
+    for name, prec, study in (("mixed", 4, 0), ("f64", 8, 0),
+                              ("all32", 4, capi.SP_STUDY_ABS_POS32 | capi.SP_STUDY_TWIDDLE32 | capi.SP_STUDY_SUM32)):
+        s = capi.GpuSim(prec=prec, sort_interval=20, left_edge=capi.SP_LEFT_REFLECT, study=study, **C3)
+        for q, m, vth in SPECIES:
+            s.init_species(q, m, vth, seed=0)
+        sims[name] = s
+    # one initial state for all three: the device initialiser's positions are
+    # exact in both precisions and its momenta are drawn in FP64 and rounded
+    # to the run's precision
+    n = particles(C3)
+    assert n == 132120576
+    prof = {}
+    e0 = {}
+    for name, s in sims.items():
+        s.prestep()
+        e0[name] = total_energy(s.diag())
+        s.step(steps)
+        d = check_properties(s, n, steps)
+        drift = total_energy(d) / e0[name] - 1
+        print(f"C3 {name}: energy drift {drift:.3e} over {steps} steps, gauss post max {d['gauss_post_max']:.3e}",
+              file=sys.stderr, flush=True)
+        assert abs(drift) < 1e-3
+        prof[name] = profile(s, C3["dens"])
+        s.close()
+    em = profile_error(prof["mixed"], prof["f64"], C3["delta"])
+    ea = profile_error(prof["all32"], prof["f64"], C3["delta"])
+    print(f"C3 profile error vs FP64 at step {steps}: mixed {em:.3e}, all-FP32 {ea:.3e}", file=sys.stderr, flush=True)
+    assert em < ea
+
+
+def test_c4_ensemble_of_64_runs_for_1000_steps():
+    runs = ensemble_runs()
+    sims, counts = [], []
+    for r in runs:
+        kw = {k: v for k, v in r.items() if k != "temp"}
+        s = capi.GpuSim(prec=4, sort_interval=20, left_edge=capi.SP_LEFT_REFLECT, **kw)
+        for q, m, _ in SPECIES:
+            s.init_species(q, m, (r["temp"] / m) ** 0.5, seed=0)
+        sims.append(s)
+        counts.append(particles(r))
+    assert sum(counts) == 115175424
+    tunings = [s.autotune(trials=2, use_cache=True) for s in sims]
+    for s in sims:
+        s.prestep()
+    ens = capi.Ensemble(sims, branches=4, unit=2)
+    ens.step(1000)
+    ens.sync()
+    st = ens.stats()
+    ens.close()
+    assert st["kept"] >= 1
+    worst = 0.0
+    for s, n in zip(sims, counts):
+        d = check_properties(s, n, 1000)
+        worst = max(worst, d["gauss_post_max"])
+    print(f"C4: 64 runs x 1000 steps, {sum(counts)} particles, graphs {st}, worst gauss post {worst:.3e}",
+          file=sys.stderr, flush=True)
+    # a sample of members against the same runs stepped alone (same inputs
+    # and tuning): FP32 deposits flush with float atomics in a varying order,
+    # so the runs agree to that rounding, amplified by the dynamics over 1000
+    # steps -- judged on the density profile and the energy
+    for i in (0, 21, 42, 63):
+        r = runs[i]
+        kw = {k: v for k, v in r.items() if k != "temp"}
+        alone = capi.GpuSim(prec=4, sort_interval=20, left_edge=capi.SP_LEFT_REFLECT, **kw)
+        for q, m, _ in SPECIES:
+            alone.init_species(q, m, (r["temp"] / m) ** 0.5, seed=0)
+        alone.set_tuning(**tunings[i])
+        alone.prestep()
+        alone.step(1000)
+        pa, pm = profile(alone, r["dens"]), profile(sims[i], r["dens"])
+        ea, em = total_energy(alone.diag()), total_energy(sims[i].diag())
+        print(f"C4 member {i} ({r['size_x']}x{r['size_y']}, {r['dens']} ppc): profile vs alone "
+              f"{profile_error(pm, pa, r['delta']):.3e}, energy {em / ea - 1:.3e}", file=sys.stderr, flush=True)
+        assert profile_error(pm, pa, r["delta"]) < 1e-2
+        assert abs(em / ea - 1) < 1e-2
+        alone.close()
+    for s in sims:
+        s.close()
+
+
+def test_c5_learned_inference_at_full_size():
+    params = np.load(os.path.join(os.path.dirname(HERE), "spectral_rs_amd", "denoiser_shock.npy"))
+    steps = 300
+    out = {}
+    for name, p in (("plain", None), ("learned", params)):
+        s = capi.GpuSim(prec=4, sort_interval=20, left_edge=capi.SP_LEFT_REFLECT, **C5)
+        for q, m, vth in SPECIES:
+            s.init_species(q, m, vth, seed=0)
+        if p is not None:
+            s.set_denoiser(p, max_rel=1.0)
+            s.set_denoiser_channels(0x7)
+        s.prestep()
+        s.step(steps)
+        check_properties(s, particles(C5), steps)
+        if p is not None:
+            applied, fallbacks = s.denoiser_stats()
+            print(f"C5: network applied in {applied} solves, fallback in {fallbacks}", file=sys.stderr, flush=True)
+            assert applied + fallbacks == steps
+            assert applied > 0
+        out[name] = profile(s, C5["dens"])
+        s.close()
+    e = profile_error(out["learned"], out["plain"], C5["delta"])
+    print(f"C5: learned vs plain 4 ppc density profile {e:.3e} at step {steps}", file=sys.stderr, flush=True)
+    assert e < 0.05
