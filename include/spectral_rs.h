@@ -307,11 +307,21 @@ int sp_dn_gradient(sp_dn* dn, const float* x, const float* t, float* grad, doubl
  * loss_hist (iters entries, may be NULL) gets each batch's mean loss. */
 int sp_dn_train(sp_dn* dn, const float* xs, const float* ts, int nsamples, int iters, float lr, uint64_t seed,
                 double* loss_hist);
-/* Apply a trained network after the binomial filter in every field solve of
- * the context (FP32 only).  Fallback to the binomial-filtered currents when
- * |r| > max_rel |x| or r is not finite, or when the previous solve's relative
- * pre-correction Gauss residual exceeds max_gauss.  params NULL disables. */
+/* Apply a trained network in every field solve of the context (FP32 only),
+ * after the binomial filter on its output, or in its place (see
+ * sp_set_denoiser_input).  Fallback to the binomial-filtered currents when
+ * |r| > max_rel |x| or r is not finite, when (max_gauss > 0) the previous
+ * solve's relative pre-correction Gauss residual exceeds max_gauss, or when
+ * the spectrum check of sp_set_denoiser_input fails.  params NULL disables. */
 int sp_set_denoiser(sp_ctx* ctx, const float* params, int n, float max_rel, float max_gauss);
+/* The network's input: raw = 0, the binomial-filtered currents (default);
+ * raw = 1, the unfiltered deposit, the network then standing in for the
+ * filter (src/flds/mod.rs:90-94) whose result remains the fallback (needs
+ * n_pass <= 8).  max_hp > 0 adds a spectrum check on the current solve's
+ * data: for every written channel, the high-pass energy sum |lap y|^2 of the
+ * network's output y (5-point Laplacian, periodic) must not exceed max_hp
+ * times that of the binomial-filtered grid; 0 switches it off. */
+int sp_set_denoiser_input(sp_ctx* ctx, int raw, float max_hp);
 /* Channels the network's output is written to when it is applied: bit c
  * for Jx, Jy, Jz, rho (default 0xF).  With rho left out, the Gauss-law
  * projection of the solve (src/flds/mod.rs:185-201) keeps the longitudinal

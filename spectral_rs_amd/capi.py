@@ -30,7 +30,7 @@ SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_spe
            "sp_ensemble_destroy", "sp_ensemble_last_error", "sp_ensemble_set_graph", "sp_ensemble_step",
            "sp_ensemble_sync", "sp_ensemble_stats", "sp_dn_create", "sp_dn_destroy", "sp_dn_last_error", "sp_dn_set_params",
            "sp_dn_get_params", "sp_dn_set_loss_weights", "sp_dn_forward", "sp_dn_gradient", "sp_dn_train",
-           "sp_set_denoiser", "sp_set_denoiser_channels",
+           "sp_set_denoiser", "sp_set_denoiser_input", "sp_set_denoiser_channels",
            "sp_get_denoiser_stats"]
 DN_NTRAIN, DN_NPARAMS = 30084, 30088
 DN_LAYERS, DN_CH = 5, 32  # the denoiser: 5 convolutions, 32 hidden channels
@@ -113,6 +113,7 @@ def load(path=LIB_PATH):
     L.sp_set_denoiser.argtypes = [vp, vp, i32, ct.c_float, ct.c_float]
     L.sp_get_denoiser_stats.argtypes = [vp, ct.POINTER(i32), ct.POINTER(i32)]
     L.sp_set_denoiser_channels.argtypes = [vp, i32]
+    L.sp_set_denoiser_input.argtypes = [vp, i32, ct.c_float]
     L.sp_set_tuning.argtypes = [vp, ct.POINTER(SpTuning)]
     L.sp_get_tuning.argtypes = [vp, ct.POINTER(SpTuning)]
     L.sp_autotune.argtypes = [vp, i32, i32, ct.POINTER(SpTuning)]
@@ -283,6 +284,12 @@ class GpuSim:
             return
         p = np.ascontiguousarray(params, np.float32)
         self._chk(load().sp_set_denoiser(self.h, _p(p), p.size, max_rel, max_gauss))
+
+    def set_denoiser_input(self, raw=False, max_hp=0.0):
+        """raw: the network reads the unfiltered deposit (standing in for the
+        binomial filter, which stays the fallback); max_hp > 0: spectrum check
+        of its output against the filter's (high-pass energy ratio bound)."""
+        self._chk(load().sp_set_denoiser_input(self.h, int(raw), float(max_hp)))
 
     def set_denoiser_channels(self, mask):
         """Bit c: write the network's output to Jx, Jy, Jz, rho (c = 0..3)."""

@@ -75,6 +75,8 @@ struct sp_ctx {
   int* dn_stats = nullptr;
   float dn_tol = 0, dn_gtol = 0;
   int dn_mask = 0xF;  // channels the correction is written to (bit c: Jx Jy Jz rho)
+  bool dn_raw = false;  // the network reads the unfiltered deposit (it replaces the filter)
+  float dn_hp = 0;      // spectrum check: high-pass energy bound relative to the filter's (0 = off)
   // step-loop graph (sp_set_graph): the captured period and the buffers it uses
   bool graph_on = false;
   hipGraphExec_t gexec = nullptr;
@@ -265,10 +267,14 @@ template <class T> void fields_update(sp_ctx* c) {
   static_assert(SP_JY == SP_JX + 1 && SP_JZ == SP_JX + 2 && SP_RHO == SP_JX + 3, "grid ids");
   launch_binomial<T>(&c->grid[SP_JX], c->work, 4, nx, ny, c->p.n_pass, c->st);
   if constexpr (sizeof(T) == 4) {
-    if (c->dn_on) {  // learned correction of the filtered currents, or the fallback
+    if (c->dn_on) {  // the learned currents, or the fallback: the filtered ones
       float* g4[4] = {(float*)c->grid[SP_JX], (float*)c->grid[SP_JY], (float*)c->grid[SP_JZ], (float*)c->grid[SP_RHO]};
-      dn_apply_sim(dn_params(c->dn_buf), dn_tensors(c), g4, nx, ny, c->p.dens, c->dn_sums, c->gauss_d, c->dn_tol,
-                   c->dn_gtol, c->dn_mask, c->dn_stats, c->st);
+      // after the filter's pointer swap the work grids hold its input, the
+      // unfiltered deposit (one filter launch: n_pass <= 8, checked when set)
+      float* w4[4] = {(float*)c->work[0], (float*)c->work[1], (float*)c->work[2], (float*)c->work[3]};
+      const bool raw = c->dn_raw && c->p.n_pass > 0;
+      dn_apply_sim(dn_params(c->dn_buf), dn_tensors(c), raw ? w4 : g4, g4, nx, ny, c->p.dens, c->dn_sums, c->gauss_d,
+                   c->dn_tol, c->dn_gtol, c->dn_hp, c->dn_mask, c->dn_stats, c->st);
       launch_update_ghosts<float>(g4, 4, nx, ny, c->st);
     }
   }
@@ -1270,11 +1276,11 @@ int sp_set_denoiser(sp_ctx* c, const float* params, int n, float max_rel, float 
   }
   if (c->f64) return fail(c, SP_EINVAL, "the denoiser runs in FP32 contexts only");
   if (n != DN_NPARAMS) return fail(c, SP_EINVAL, "expected SP_DN_NPARAMS parameters");
-  if (!(max_rel > 0) || !(max_gauss > 0)) return fail(c, SP_EINVAL, "tolerances must be positive");
+  if (!(max_rel > 0) || !(max_gauss >= 0)) return fail(c, SP_EINVAL, "max_rel must be positive, max_gauss >= 0");
   if (!c->dn_buf) {
     const size_t planes = DN_IN * c->nspec_elems();
     HIPCHK(c, hipMalloc(&c->dn_buf, (DN_NPARAMS + DN_PACK_SHORTS / 2 + planes) * sizeof(float)));
-    HIPCHK(c, hipMalloc(&c->dn_sums, 2 * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->dn_sums, DN_SUMS * sizeof(double)));
     HIPCHK(c, hipMalloc(&c->dn_stats, 2 * sizeof(int)));
   }
   HIPCHK(c, hipMemcpyAsync(c->dn_buf, params, DN_NPARAMS * sizeof(float), hipMemcpyHostToDevice, c->st));
@@ -1283,6 +1289,17 @@ int sp_set_denoiser(sp_ctx* c, const float* params, int n, float max_rel, float 
   c->dn_tol = max_rel;
   c->dn_gtol = max_gauss;
   c->dn_on = true;
+  return SP_OK;
+}
+
+int sp_set_denoiser_input(sp_ctx* c, int raw, float max_hp) {
+  if (!c) return SP_EINVAL;
+  if (!(max_hp >= 0)) return fail(c, SP_EINVAL, "max_hp must be >= 0");
+  if (raw && c->p.n_pass > 8) return fail(c, SP_EINVAL, "the unfiltered input needs n_pass <= 8 (one filter launch)");
+  c->dn_raw = raw != 0;
+  c->dn_hp = max_hp;
+  c->tune_gen++;  // captured steps hold these as kernel arguments
+  drop_graph(c);
   return SP_OK;
 }
 

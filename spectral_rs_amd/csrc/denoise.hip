@@ -300,19 +300,37 @@ struct Grids4 {
   float* g[4];
 };
 
-// Adds dens s_c r to the interior of grid c (for the channels set in chmask)
-// when the correction passes the checks: |r| <= tol |x| and r finite (sums,
-// reduced by the fused kernel), and the previous solve's pre-correction
-// Gauss residual (gauss[0] / gauss[2], relative) is below gauss_tol.
-// Otherwise the binomial-filtered grids stay (the fallback).  stats[0]
+// Writes out_c = in_c + dens s_c r to the interior of the output grids (for
+// the channels set in chmask) when the correction passes the checks:
+//  * |r| <= tol |x| and r finite (sums[0..1], reduced by the fused kernel);
+//  * (gauss_tol > 0) the previous solve's pre-correction Gauss residual
+//    (gauss[0] / gauss[2], relative) is below gauss_tol;
+//  * (hp_tol > 0) a spectrum check on this solve's data: for every written
+//    channel, the high-pass energy sum |lap(in + dens s r)|^2 (sums[2 + 2c],
+//    k_dn_hp) is at most hp_tol times that of the binomial-filtered grid
+//    (sums[3 + 2c]); the filter's symbol (cos^2(kx/2) cos^2(ky/2))^n_pass
+//    leaves little at high k, and an output that keeps more is noise the
+//    network failed to remove.
+// Otherwise the outputs keep the binomial-filtered values (the fallback).
+// `in` is the network's input: the filtered grids themselves (in == out), or
+// the unfiltered deposit when the network replaces the filter.  stats[0]
 // counts applications, stats[1] fallbacks.  Grid: x over the columns, y over
 // (channel, row) pairs with a stride.
-__global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float dens, const float* __restrict__ r,
-                           int nx, int ny, const double* __restrict__ sums, const double* __restrict__ gauss, float tol,
-                           float gauss_tol, int chmask, int* stats) {
+__global__ void k_dn_apply(Grids4 in, Grids4 out, const float* __restrict__ scale, float dens,
+                           const float* __restrict__ r, int nx, int ny, const double* __restrict__ sums,
+                           const double* __restrict__ gauss, float tol, float gauss_tol, float hp_tol, int chmask,
+                           int* stats) {
   const double ratio2 = sums[0] > 0 ? sums[1] / sums[0] : 0.0;
-  const double g2 = gauss[2] > 0 ? gauss[0] / gauss[2] : 0.0;
-  const bool ok = ratio2 <= (double)tol * tol && g2 <= (double)gauss_tol * gauss_tol;
+  bool ok = ratio2 <= (double)tol * tol;
+  if (gauss_tol > 0) {
+    const double g2 = gauss[2] > 0 ? gauss[0] / gauss[2] : 0.0;
+    ok = ok && g2 <= (double)gauss_tol * gauss_tol;
+  }
+  if (hp_tol > 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if ((chmask >> c) & 1) ok = ok && sums[2 + 2 * c] <= (double)hp_tol * sums[3 + 2 * c];
+  }
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) atomicAdd(stats + (ok ? 0 : 1), 1);
   if (!ok) return;
   const int xx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -320,8 +338,55 @@ __global__ void k_dn_apply(Grids4 gr, const float* __restrict__ scale, float den
   for (int cr = blockIdx.y; cr < 4 * ny; cr += gridDim.y) {
     const int c = cr / ny, yy = cr - c * ny;
     if (!((chmask >> c) & 1)) continue;
-    float* g = gr.g[c] + (int64_t)(yy + 1) * (nx + 2) + xx + 1;
-    *g = *g + dens * scale[c] * r[(int64_t)cr * nx + xx];
+    const int64_t o = (int64_t)(yy + 1) * (nx + 2) + xx + 1;
+    out.g[c][o] = in.g[c][o] + dens * scale[c] * r[(int64_t)cr * nx + xx];
+  }
+}
+
+// High-pass energies for the spectrum check of k_dn_apply: for each channel,
+// sums[2 + 2c] += sum |lap(in + dens s_c r)|^2 and sums[3 + 2c] +=
+// sum |lap(out)|^2 over the interior, lap = the 5-point Laplacian with
+// periodic neighbours (its symbol 4 sin^2(kx/2) + 4 sin^2(ky/2) weights the
+// highest wavenumbers most).  FP64 sums, one atomic per block and quantity.
+__global__ void __launch_bounds__(256) k_dn_hp(Grids4 in, Grids4 out, const float* __restrict__ scale, float dens,
+                                               const float* __restrict__ r, int nx, int ny, double* sums) {
+  __shared__ double red[2][4];
+  const int c = blockIdx.z;
+  const int xx = blockIdx.x * blockDim.x + threadIdx.x;
+  double hy = 0.0, hf = 0.0;
+  const int P = nx + 2;
+  if (xx < nx) {
+    const int xl = xx == 0 ? nx - 1 : xx - 1, xr = xx == nx - 1 ? 0 : xx + 1;
+    const float sd = dens * scale[c];
+    for (int yy = blockIdx.y; yy < ny; yy += gridDim.y) {
+      const int yd = yy == 0 ? ny - 1 : yy - 1, yu = yy == ny - 1 ? 0 : yy + 1;
+      const float* a = in.g[c];
+      const float* f = out.g[c];
+      const float* rc = r + (int64_t)c * ny * nx;
+      auto y = [&](int j, int i) {
+        return a[(int64_t)(j + 1) * P + i + 1] + sd * rc[(int64_t)j * nx + i];
+      };
+      auto g = [&](int j, int i) { return f[(int64_t)(j + 1) * P + i + 1]; };
+      const double ly = 4.0 * y(yy, xx) - y(yy, xl) - y(yy, xr) - y(yd, xx) - y(yu, xx);
+      const double lf = 4.0 * g(yy, xx) - g(yy, xl) - g(yy, xr) - g(yd, xx) - g(yu, xx);
+      hy += ly * ly;
+      hf += lf * lf;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    hy += __shfl_xor(hy, off);
+    hf += __shfl_xor(hf, off);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = hy;
+    red[1][w] = hf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(sums + 2 + 2 * c, red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+    atomicAdd(sums + 3 + 2 * c, red[1][0] + red[1][1] + red[1][2] + red[1][3]);
   }
 }
 
@@ -667,19 +732,27 @@ void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, in
   hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, p, g, m1, m2, n, lr, t);
 }
 
-void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, int nx, int ny, int dens, double* sums,
-                  const double* gauss, float tol, float gauss_tol, int chmask, int* stats, hipStream_t st) {
+void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* inputs, float* const* grids, int nx, int ny,
+                  int dens, double* sums, const double* gauss, float tol, float gauss_tol, float hp_tol, int chmask,
+                  int* stats, hipStream_t st) {
   FusedIn in{};
-  for (int c = 0; c < 4; ++c) in.gr.g[c] = grids[c];
+  Grids4 out{};
+  for (int c = 0; c < 4; ++c) {
+    in.gr.g[c] = inputs[c];
+    out.g[c] = grids[c];
+  }
   in.scale = P.scale;
   in.inv_dens = 1.0f / (float)dens;
   in.sums = sums;
-  fill<double>(sums, 0.0, 2, st);
+  fill<double>(sums, 0.0, DN_SUMS, st);
   hipLaunchKernelGGL(k_dn_pack_weights, dim3((DN_PACK_SHORTS + 255) / 256), dim3(256), 0, st, P, T.packed);
   dim3 grid((nx + FX - 1) / FX, (ny + FY - 1) / FY);
   hipLaunchKernelGGL((k_dn_fused<true>), grid, dim3(FNT), 0, st, in, P, (const short*)T.packed, T.r, ny, nx);
+  if (hp_tol > 0)
+    hipLaunchKernelGGL(k_dn_hp, dim3((nx + 255) / 256, (unsigned)std::min(ny, 256), 4), dim3(256), 0, st, in.gr, out,
+                       P.scale, (float)dens, T.r, nx, ny, sums);
   hipLaunchKernelGGL(k_dn_apply, dim3((nx + 255) / 256, (unsigned)std::min(4 * ny, 4096)), dim3(256), 0, st, in.gr,
-                     P.scale, (float)dens, T.r, nx, ny, sums, gauss, tol, gauss_tol, chmask, stats);
+                     out, P.scale, (float)dens, T.r, nx, ny, sums, gauss, tol, gauss_tol, hp_tol, chmask, stats);
 }
 
 }  // namespace sp

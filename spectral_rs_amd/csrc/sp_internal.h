@@ -189,11 +189,16 @@ void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, in
 // All layers of one image [4][H][W] in one launch, y periodic (the
 // simulation's inference path); same results as dn_forward(periodic_y).
 void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipStream_t st);
-// Runs the fused network on the 4 filtered grids (read as grid / dens / s_c)
-// and, when the checks pass, adds dens s_c r to the grids for the channels
-// set in chmask (bit c = grid c of Jx Jy Jz rho); T.packed and T.r are its
-// buffers; stats[0] counts applications, stats[1] fallbacks.
-void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* grids, int nx, int ny, int dens, double* sums,
-                  const double* gauss, float tol, float gauss_tol, int chmask, int* stats, hipStream_t st);
+// Runs the fused network on the 4 input grids (read as grid / dens / s_c:
+// the binomial-filtered grids, or the unfiltered deposit when the network
+// replaces the filter) and, when the checks pass, writes input + dens s_c r
+// into the filtered grids for the channels set in chmask (bit c = grid c of
+// Jx Jy Jz rho); otherwise the filtered grids stay.  T.packed and T.r are its
+// buffers; sums holds DN_SUMS doubles; stats[0] counts applications,
+// stats[1] fallbacks.  gauss_tol <= 0 and hp_tol <= 0 switch those checks off.
+constexpr int DN_SUMS = 10;  // |x|^2, |r|^2, then per channel the high-pass energies of output and filter
+void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* inputs, float* const* grids, int nx, int ny,
+                  int dens, double* sums, const double* gauss, float tol, float gauss_tol, float hp_tol, int chmask,
+                  int* stats, hipStream_t st);
 
 }  // namespace sp

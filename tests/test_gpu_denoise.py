@@ -106,13 +106,15 @@ def test_training_reduces_loss_on_a_denoising_task():
     dn.close()
 
 
-def shock(denoise=None, max_rel=0.5):
+def shock(denoise=None, max_rel=0.5, raw=False, max_hp=0.0, mask=0xF, max_gauss=1.0):
     kw = dict(size_x=128, size_y=32, delta=8, dens=4, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0, left_edge=1)
     g = capi.GpuSim(prec=4, sort_interval=5, **kw)
     for q, m, vth, sd in [(1.0, 16.0, 0.0025, 3), (-1.0, 1.0, 0.01, 4)]:
         g.add_species(q, m, **shock_species(128, 32, 8, 4, 10.0, 10.0, vth, seed=sd))
     if denoise is not None:
-        g.set_denoiser(denoise, max_rel=max_rel)
+        g.set_denoiser(denoise, max_rel=max_rel, max_gauss=max_gauss)
+        g.set_denoiser_input(raw, max_hp)
+        g.set_denoiser_channels(mask)
     g.prestep()
     g.step(10)
     g.sync()
@@ -144,26 +146,51 @@ def test_in_loop_identity_network_and_fallback():
         s.close()
 
 
-def test_trained_network_gains_over_the_binomial_filter():
-    # Fresh 4-ppc / 64-ppc pairs from a run the network was not trained on: the
-    # training size (1024 x 256) with gamma0 10 and seed 5 (training runs: seeds
-    # 0-5, gamma0 drawn from [2, 20]), sampled over the run as in training
-    # (steps 100 to 1000).  The committed network brings the filtered 4-ppc
-    # currents closer to the 64-ppc ones than the binomial filter alone on every
-    # channel; measured: Jx 0.50x, Jy 0.80x, Jz 0.08x (almost pure noise), rho
-    # 0.55x of the filter's error.  Jy is the weak channel: on another held-out
-    # run (gamma0 12.9) the network was 1.5x worse than the filter there, which
-    # is why the time-to-solution study writes back J only.  Relative errors
-    # are summed over the whole run, as the loss is.
+def test_unfiltered_input_mode_and_the_spectrum_check():
+    # With the network reading the unfiltered deposit, an all-zero network
+    # passes the raw deposit through: its high-pass energy is far above the
+    # binomial filter's, so the spectrum check (max_hp) rejects it in every
+    # solve and the run is the plain one; with the check off it is applied
+    # and the written channels carry the unfiltered currents.
+    plain, plain2 = shock(), shock()
+    floor = {n: rel(plain2.grid(n), plain.grid(n)) for n in ["jx", "rho", "ex", "bz"]}
+    p0 = np.zeros(D.NPARAMS, np.float32)
+    p0[-4:] = 1.0
+    # (the previous solve's Gauss gate is off: max_gauss = 0)
+    chk = shock(p0, max_rel=10.0, raw=True, max_hp=2.0, mask=0x5, max_gauss=0.0)
+    assert chk.denoiser_stats() == (0, 10)
+    for name, f in floor.items():
+        assert rel(chk.grid(name), plain.grid(name)) <= 3 * f + 1e-6, (name, f)
+    off = shock(p0, max_rel=10.0, raw=True, max_hp=0.0, mask=0x5, max_gauss=0.0)
+    assert off.denoiser_stats() == (10, 0)
+    # Jx is written unfiltered: far from the plain run's filtered Jx
+    assert rel(off.grid("jx"), plain.grid("jx")) > 10 * floor["jx"] + 1e-3
+    for s in (plain, plain2, chk, off):
+        s.close()
+
+
+@pytest.mark.parametrize("gamma,seed", [(4.0, 100), (12.9, 101), (18.0, 102)])
+def test_trained_network_gains_over_the_binomial_filter(gamma, seed):
+    # Fresh 4-ppc / 64-ppc pairs at the training size (1024 x 256) from runs
+    # the network never saw (training: seeds 0-11, gamma0 drawn from
+    # [2, 20]), sampled over the run as in training (steps 100-850).  The
+    # committed network reads the unfiltered 4-ppc deposit (it stands in for
+    # the binomial filter, C5); against the 64-ppc filtered target it beats the
+    # binomial-filtered 4-ppc currents on Jx, Jz and rho at every gamma0
+    # (measured: Jx 0.96/0.79/0.58x, Jz 0.08/0.08/0.07x, rho 0.87/0.87/0.55x of
+    # the filter's error at gamma0 4/12.9/18).  Jy is not written back in the
+    # simulation (channel mask 5 = Jx, Jz): on some held-out frames the network
+    # is worse than the filter there (4.9x at gamma0 12.9, 57x at 18, while
+    # 0.93x on training frames), so the filter keeps it.  Relative errors are
+    # summed over the run, as the loss is.
     import os
     import sys
     from spectral_rs_amd import learned
     p = np.load(os.path.join(os.path.dirname(capi.__file__), "denoiser_shock.npy"))
-    X, T = learned.pairs(1024, 256, n=8, t0=100, every=125, seed=5)
-    ev = learned.evaluate(p, X, T)
-    print("held-out 1024x256 gamma0 10:", ev, file=sys.stderr)
+    Xr, Xf, T = learned.pairs(1024, 256, n=6, t0=100, every=150, seed=seed, gamma=gamma)
+    ev = learned.evaluate(p, Xr, Xf, T)
+    print(f"held-out 1024x256 gamma0 {gamma}:", ev, file=sys.stderr)
     b, l = ev["binomial_rel_mse"], ev["learned_rel_mse"]
-    assert l[0] < 0.65 * b[0], ev       # jx
-    assert l[1] < 1.0 * b[1], ev        # jy
-    assert l[2] < 0.15 * b[2], ev       # jz
-    assert l[3] < 0.7 * b[3], ev        # rho
+    assert l[0] < b[0], ev         # jx
+    assert l[2] < 0.15 * b[2], ev  # jz (almost pure noise at 4 ppc)
+    assert l[3] < b[3], ev         # rho

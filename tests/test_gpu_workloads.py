@@ -10,8 +10,9 @@ through properties that hold at any size and against reference runs:
   particle count, density sum, Gauss law and energy, and a sample of members
   against the same runs stepped alone;
 * C5, inference at 4096 x 1024 with 4 ppc per species (33,292,288 particles):
-  the trained network in the loop, its application and fallback counts, and
-  the run's conservation properties.
+  the trained network in the loop on the unfiltered deposit (Jx, Jz written,
+  spectrum check on), its application and fallback counts, and the run's
+  conservation properties.
 
 Properties: every particle deposits a total TSC weight of 1 (the folded
 density grid sums to the particle count); the Gauss correction leaves FP32
@@ -139,9 +140,10 @@ def test_c5_learned_inference_at_full_size():
         s = capi.GpuSim(prec=4, sort_interval=20, left_edge=capi.SP_LEFT_REFLECT, **C5)
         for q, m, vth in SPECIES:
             s.init_species(q, m, vth, seed=0)
-        if p is not None:
-            s.set_denoiser(p, max_rel=1.0)
-            s.set_denoiser_channels(0x7)
+        if p is not None:  # as the time-to-solution study runs it
+            s.set_denoiser(p, max_rel=1.0, max_gauss=0.0)
+            s.set_denoiser_input(True, 2.0)
+            s.set_denoiser_channels(0x5)
         s.prestep()
         s.step(steps)
         check_properties(s, particles(C5), steps)
