@@ -748,8 +748,10 @@ void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* inputs, f
   hipLaunchKernelGGL(k_dn_pack_weights, dim3((DN_PACK_SHORTS + 255) / 256), dim3(256), 0, st, P, T.packed);
   dim3 grid((nx + FX - 1) / FX, (ny + FY - 1) / FY);
   hipLaunchKernelGGL((k_dn_fused<true>), grid, dim3(FNT), 0, st, in, P, (const short*)T.packed, T.r, ny, nx);
+  // a few hundred rows per block: 2 FP64 atomics per block on 8 addresses
+  // (one block per row made them serialise: 0.35 ms at 4096 x 1024)
   if (hp_tol > 0)
-    hipLaunchKernelGGL(k_dn_hp, dim3((nx + 255) / 256, (unsigned)std::min(ny, 256), 4), dim3(256), 0, st, in.gr, out,
+    hipLaunchKernelGGL(k_dn_hp, dim3((nx + 255) / 256, (unsigned)std::min(ny, 16), 4), dim3(256), 0, st, in.gr, out,
                        P.scale, (float)dens, T.r, nx, ny, sums);
   hipLaunchKernelGGL(k_dn_apply, dim3((nx + 255) / 256, (unsigned)std::min(4 * ny, 4096)), dim3(256), 0, st, in.gr,
                      out, P.scale, (float)dens, T.r, nx, ny, sums, gauss, tol, gauss_tol, hp_tol, chmask, stats);
