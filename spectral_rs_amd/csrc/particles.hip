@@ -713,10 +713,16 @@ template <class T>
 __global__ void __launch_bounds__(256) k_push(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int bad = 0;
+  // FP32: the raw-buffer gather of the particle pass (one address register
+  // per particle, stencil columns as immediates)
+  const auto fields = [&] {
+    if constexpr (sizeof(T) == 4) return field_bufs(gs, g);
+    else return gs;
+  }();
   if (i < s.n) {
     const int32_t c = s.cell[i];
     T px = s.px[i], py = s.py[i], pz = s.pz[i];
-    boris(cell_ix(c), cell_iy(c), s.dx[i], s.dy[i], px, py, pz, gs, g, s.alpha, s.beta, bad);
+    boris(cell_ix(c), cell_iy(c), s.dx[i], s.dy[i], px, py, pz, fields, g, s.alpha, s.beta, bad);
     if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
     s.px[i] = px;
     s.py[i] = py;
