@@ -10,6 +10,8 @@
 // summation order only.
 #pragma clang fp contract(off)
 
+#include <type_traits>
+
 #include "sp_internal.h"
 
 namespace sp {
@@ -132,12 +134,15 @@ template <class T> __device__ __forceinline__ bool inside(int ix, int iy, const 
   return ix >= 1 && ix <= g.nx && iy >= 1 && iy <= g.ny;
 }
 
-// Raw buffer resources of the six field grids (Ex Ey Ez Bx By Bz).  The
-// gather then needs one address register per particle: the column offset
-// is an instruction immediate and the row offset a scalar register, where
-// global loads would need a 64-bit address add per stencil row and field.
-// Word 3 0x00020000 is the gfx9 raw-buffer format; reads past `bytes`
-// return 0 (the callers only read inside the padded grid).
+template <class T> using lds_ptr = __attribute__((address_space(3))) T*;
+template <class T> using glb_ptr = __attribute__((address_space(1))) T*;
+
+// Raw buffer resources of the six field grids (Ex Ey Ez Bx By Bz), for the
+// stand-alone push.  The gather then needs one address register per particle:
+// the column offset is an instruction immediate and the row offset a scalar
+// register, where global loads would need a 64-bit address add per stencil row
+// and field.  Word 3 0x00020000 is the gfx9 raw-buffer format; reads past
+// `bytes` return 0 (the callers only read inside the padded grid).
 struct FieldBufs {
   __amdgpu_buffer_rsrc_t r[6];
   int row_bytes;
@@ -192,6 +197,50 @@ __device__ __forceinline__ void gather6(const FieldBufs& fb, const Geo<float>& g
   }
 }
 
+// The six field grids over one chunk's cell box + 1 cell (every stencil of a
+// particle in the box), staged in LDS by the FP32 fused pass.  They sit at
+// fixed offsets in the tails of the deposit planes (fields 0-3 in the last
+// FWIN words of planes 0-3, fields 4-5 in the FWIN words before those of
+// planes 0-1), so the window is used when it has at most FWIN cells and the
+// deposit copies fit in the first WMAX - 2 FWIN words of each plane.  Gathers
+// from LDS replace 18 three-dword vector loads per particle, whose per-lane
+// L1 lookups (lanes of a wave in scattered cells) bound the pass before.
+#ifndef SP_FWIN
+#define SP_FWIN 400
+#endif
+struct FieldWin {
+  lds_ptr<int> p;  // base of the deposit planes
+  int x0, y0, w, h;
+  bool use;
+};
+template <int WMAX> __device__ __forceinline__ constexpr int fwin_off(int k) {
+  return k < 4 ? k * WMAX + WMAX - SP_FWIN : (k - 4) * WMAX + WMAX - 2 * SP_FWIN;
+}
+// The same ordered 9-term sums as gather6, from the window when the stencil
+// lies in it, else from the grids with global loads (plain pointers: buffer
+// descriptors would hold 24 scalar registers through the particle loop).
+template <int WMAX>
+__device__ __forceinline__ void gather6w(const GridSet& gs, const FieldWin& l, const Geo<float>& g, int ix, int iy,
+                                         const float (&w)[9], float (&f)[6]) {
+  if (l.use && ix - 1 >= l.x0 && ix + 1 < l.x0 + l.w && iy - 1 >= l.y0 && iy + 1 < l.y0 + l.h) {
+    const lds_ptr<int> p = l.p + (__mul24(iy - 1 - l.y0, l.w) + (ix - 1 - l.x0));
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      float a[9];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[3 * r + c] = __builtin_bit_cast(float, (int)p[fwin_off<WMAX>(k) + r * l.w + c]);
+      float s = w[0] * a[0];
+#pragma unroll
+      for (int m = 1; m < 9; ++m) s += w[m] * a[m];
+      f[k] = s;
+    }
+  } else {
+    gather6(gs, g, ix, iy, w, f);
+  }
+}
+
 // Deposit target of one workgroup: a window [x0, x0+w) x [y0, y0+h) of the
 // padded grids held in LDS (planes Jx Jy Jz rho dens, w*h cells each), plus
 // the global grids for stencils that leave the window.  use_lds == false sends
@@ -199,8 +248,6 @@ __device__ __forceinline__ void gather6(const FieldBufs& fb, const Geo<float>& g
 // Pointers with their address space spelled out, so every atomic lowers to
 // the native LDS (ds_add_f32/f64) or global (global_atomic_add_f32/f64)
 // instruction rather than a flat atomic.
-template <class T> using lds_ptr = __attribute__((address_space(3))) T*;
-template <class T> using glb_ptr = __attribute__((address_space(1))) T*;
 template <class T> __device__ __forceinline__ void lds_add(lds_ptr<T> p, T v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -387,15 +434,19 @@ __device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, cons
 
 // Prtl::boris_push for one particle (src/prtls.rs:306-440): ordered 9-term
 // TSC gather of E and B, then the relativistic Boris rotation.
-// `Fields` is a GridSet (global loads) or, for FP32, FieldBufs.
+// `Fields` is a GridSet (FP32: with the LDS window `fw`, global loads outside
+// it; FP64: global loads) or, for the FP32 stand-alone push, FieldBufs.
 template <class T, class Fields>
 __device__ __forceinline__ void boris(int ix, int iy, T dx, T dy, T& px, T& py, T& pz, const Fields& fs,
-                                      const Geo<T>& g, T alpha, T beta, int& bad) {
+                                      const FieldWin& fw, const Geo<T>& g, T alpha, T beta, int& bad) {
   if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
   T w[9];
   tsc(dx, dy, w);
-  T f[6];
-  gather6(fs, g, ix, iy, w, f);  // Ex Ey Ez Bx By Bz
+  T f[6];  // Ex Ey Ez Bx By Bz
+  if constexpr (sizeof(T) == 4 && std::is_same<Fields, GridSet>::value)
+    gather6w<ChunkShape<float>::WMAX>(fs, fw, g, ix, iy, w, f);
+  else
+    gather6(fs, g, ix, iy, w, f);
   const T ext = f[0] * beta, eyt = f[1] * beta, ezt = f[2] * beta;
   T bxt = f[3] * alpha, byt = f[4] * alpha, bzt = f[5] * alpha;
   const T ux = px + ext, uy = py + eyt, uz = pz + ezt;
@@ -445,17 +496,22 @@ __device__ __forceinline__ int wave_or(int v) {
 //     cell only receives contributions from particles that start within 2
 //     cells of it, each bounded by |q| c for J (|v| < c) and 1 for dens, so
 //     with m the largest particle count of any cell of the chunk a cell's
-//     |sum| <= 25 m |q| c (resp. 25 m); the scale 2^e keeps that below 2^30;
-//  3. each thread processes PPT particles, visiting slot (s * 33) mod K for
-//     s = tid + NT j: consecutive lanes get particles ~33 slots apart, i.e. in
-//     different cells of a cell-sorted chunk, so LDS atomics rarely collide;
-//     optionally the Boris push of the previous step runs first (PUSH);
-//  4. state goes back through LDS with coalesced stores and the window is
-//     added to HBM with one atomic per non-zero cell and grid.
-// When the box of the whole chunk would not fit, the window covers the
+//     |sum| <= 25 m |q| c (resp. 25 m); the scale 2^e keeps that below 2^30.
+//     With the push (FP32), the six field grids over the box + 1 are staged
+//     in LDS too (FieldWin);
+//  3. each thread processes PPT particles: DIRECT (the default) takes chunk
+//     slot tid + NT j, whose neighbours the tile sort put in neighbouring
+//     cells; the staged variant visits slot (s * 33) mod K for s = tid + NT j,
+//     particles ~33 slots apart, i.e. in different cells of a cell-sorted
+//     chunk.  Either way LDS atomics rarely collide.  Optionally the Boris push
+//     of the previous step runs first (PUSH);
+//  4. state goes back with coalesced stores and the window is added to HBM
+//     with one atomic per non-zero cell and grid.
+// When the box of the whole chunk would not fit, the windows cover the
 // particles near the chunk's middle particle.  Particles whose stencil leaves
-// the window (periodic wrap, unsorted input) use global atomics directly, so
-// the result never depends on the particle order; only the speed does.
+// a window (periodic wrap, unsorted input) gather with global loads and
+// deposit with global atomics, so the result never depends on the particle
+// order; only the speed does.
 template <class T, bool PUSH, bool DIRECT, int K>
 __global__ void __launch_bounds__(SP_PARTICLE_NT) __attribute__((amdgpu_waves_per_eu(SP_PARTICLE_WAVES_PER_EU)))
 k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
@@ -618,20 +674,40 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   // chunks) get more copies; the copy stride is odd so that one cell of
   // consecutive copies falls into different banks.
   int cs = k.wh | 1, nc = 1;
-  if (cs <= WMAX) nc = min(NCOPY, WMAX / cs);
+  FieldWin fw;
+  fw.p = (lds_ptr<int>)s_acc;
+  fw.use = false;
+  if constexpr (PUSH && sizeof(T) == 4) {
+    // the field window: the box + 1 clipped to the padded grid
+    fw.x0 = max(s_box[0] - 1, 0);
+    fw.y0 = max(s_box[2] - 1, 0);
+    fw.w = min(s_box[1] + 1, g.nx + 1) - fw.x0 + 1;
+    fw.h = min(s_box[3] + 1, g.ny + 1) - fw.y0 + 1;
+    fw.use = use_lds && fw.w > 0 && fw.h > 0 && fw.w * fw.h <= SP_FWIN && cs <= WMAX - 2 * SP_FWIN;
+  }
+  const int room = fw.use ? WMAX - 2 * SP_FWIN : WMAX;
+  if (cs <= room) nc = min(NCOPY, room / cs);
   else cs = k.wh;
   k.lds = (lds_ptr<A>)s_acc + ((tid & 63) % nc) * cs;
   for (int i = tid; i < nc * cs; i += NT) {
 #pragma unroll
     for (int pl = 0; pl < 4; ++pl) s_acc[pl * WMAX + i] = A(0);
   }
+  if constexpr (PUSH && sizeof(T) == 4) {
+    if (fw.use) {
+      const int n = fw.w * fw.h;
+      for (int i = tid; i < n; i += NT) {
+        const int r = i / fw.w, c = i - r * fw.w;
+        const int o = __mul24(fw.y0 + r, g.pitch) + fw.x0 + c;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+          fw.p[fwin_off<WMAX>(f) + i] = __builtin_bit_cast(int, ((const float*)gs.g[f])[o]);
+      }
+    }
+  }
   __syncthreads();
 
   int bad = 0;
-  const auto fields = [&] {
-    if constexpr (sizeof(T) == 4) return field_bufs(gs, g);
-    else return gs;
-  }();
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     if constexpr (DIRECT) {
@@ -647,7 +723,7 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
         int ix = cell_ix(c), iy = cell_iy(c);
         T dx = pdx[i], dy = pdy[i], px = ppx[i], py = ppy[i], pz = ppz[i];
         if (PUSH) {
-          boris(ix, iy, dx, dy, px, py, pz, fields, g, s.alpha, s.beta, bad);
+          boris(ix, iy, dx, dy, px, py, pz, gs, fw, g, s.alpha, s.beta, bad);
           if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
           ppy[i] = py;
           ppz[i] = pz;
@@ -665,7 +741,7 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
         int ix = cell_ix(c), iy = cell_iy(c);
         T dx = s_dx[p], dy = s_dy[p], px = s_px[p], py = s_py[p], pz = s_pz[p];
         if (PUSH) {
-          boris(ix, iy, dx, dy, px, py, pz, fields, g, s.alpha, s.beta, bad);
+          boris(ix, iy, dx, dy, px, py, pz, gs, fw, g, s.alpha, s.beta, bad);
           if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
           s_py[p] = py;
           s_pz[p] = pz;
@@ -722,7 +798,9 @@ __global__ void __launch_bounds__(256) k_push(SpeciesDev<T> s, Geo<T> g, GridSet
   if (i < s.n) {
     const int32_t c = s.cell[i];
     T px = s.px[i], py = s.py[i], pz = s.pz[i];
-    boris(cell_ix(c), cell_iy(c), s.dx[i], s.dy[i], px, py, pz, fields, g, s.alpha, s.beta, bad);
+    FieldWin none{};
+    none.use = false;
+    boris(cell_ix(c), cell_iy(c), s.dx[i], s.dy[i], px, py, pz, fields, none, g, s.alpha, s.beta, bad);
     if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
     s.px[i] = px;
     s.py[i] = py;
@@ -787,10 +865,11 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs
 // ------------------------------------------------------------- tile sort
 // Counting sort of a species by tile (TILE_W x TILE_H cells, tile rows in
 // serpentine order: even rows left to right, odd rows right to left, so
-// consecutive tiles are always neighbours); the order inside a tile is
-// whatever the scatter's atomics give.  The key of a particle is a pure
-// function of its cell, so the keys are bit-exact; only the order within a
-// tile is not reproducible.
+// consecutive tiles are always neighbours); inside a tile, each workgroup's
+// share is interleaved over the tile's cells (k_tile_pass), ties within a cell
+// in the order the atomics give.  The key of a particle is a pure function of
+// its cell, so the keys are bit-exact; only the order within a cell is not
+// reproducible.
 __device__ __forceinline__ int tile_of(int32_t c, int ntx, int nty, int twl, int thl) {
   const int tx = min(max((cell_ix(c) - 1) >> twl, 0), ntx - 1);
   const int ty = min(max((cell_iy(c) - 1) >> thl, 0), nty - 1);
@@ -877,11 +956,13 @@ __device__ __forceinline__ int gcd_int(int a, int b) {
 }
 
 // Counts (HIST) or places (!HIST) one workgroup's particles.  Placement
-// interleaves each group of n particles that share a tile: rank r goes to
-// position (r * u^-1) mod n for a stride u ~ 33 coprime with n, so position p
-// holds rank (p * u) mod n and consecutive positions hold particles about a
-// cell's worth apart in the input order -- in different cells, so the lanes
-// of a wave of the particle pass rarely hit the same deposit addresses.
+// interleaves each group of n particles that share a tile: the group is ranked
+// by cell (row-major inside the tile), and rank r goes to position
+// (r * u^-1) mod n for a stride u ~ n / (cells per tile) coprime with n, so
+// position p holds rank (p * u) mod n: consecutive positions, hence the lanes
+// of a wave of the particle pass, hold particles in consecutive cells of the
+// tile (round robin over its cells).  Lanes then rarely hit the same deposit
+// address, and neighbouring lanes read neighbouring field values.
 template <class T, bool HIST>
 __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty,
                                                        int twl, int thl, int* ctr) {
@@ -893,13 +974,16 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
   __shared__ __attribute__((aligned(16))) int s_tab[2 * SORT_HT];
   __shared__ int s_n[HIST ? 1 : SORT_HT];
   __shared__ int s_gpos[OUT];  // placement: output position by local index
-  __shared__ int s_total;
+  __shared__ int s_total, s_groups;
   lds_ptr<int> keys = (lds_ptr<int>)s_tab, cnt = (lds_ptr<int>)(s_tab + SORT_HT);
   for (int i = threadIdx.x; i < SORT_HT; i += SORT_NT) {
     keys[i] = -1;
     cnt[i] = 0;
   }
-  if (threadIdx.x == 0) s_total = 0;
+  if (threadIdx.x == 0) {
+    s_total = 0;
+    s_groups = 0;
+  }
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * CH;
   // all loads first, so that they are in flight together
@@ -936,20 +1020,60 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
       atomicAdd(ctr + key, n);
     } else {
       cnt[sl] = atomicAdd(ctr + key, n);  // the group's first output position
-      int u = 33;
-      while (gcd_int(u, n) != 1) u += 2;
-      s_n[sl] = n | (inv_mod(u % n, n) << 16);  // n, u^-1 mod n  (both < 2^12)
-      keys[sl] = atomicAdd(&s_total, n);        // the group's first local index
+      keys[sl] = atomicAdd(&s_total, n);  // the group's first local index
+      const int d = atomicAdd(&s_groups, 1);
+      // stride: the group's mean particle count per cell of the tile (n <=
+      // 2048 < 2^12, u^-1 < n, group index d < 512 whenever it is used)
+      int u = max(n >> (twl + thl), 1);
+      while (gcd_int(u, n) != 1) ++u;
+      s_n[sl] = n | (n > 1 ? inv_mod(u % n, n) : 0) << 12 | (d & 511) << 23;
     }
   }
   if (HIST) return;
   __syncthreads();
+  // rank of each particle in its group's cell order (cell index inside the
+  // tile, then arrival): per-group cell histograms in s_gpos, exclusive scans,
+  // and the arrival rank inside the cell from the histogram's atomics.  With
+  // more groups than the histograms fit (input far from sorted) the rank stays
+  // the arrival order.
+  const int C = 1 << (twl + thl);
+  if (s_groups * C <= CH) {
+    lds_ptr<int> hist = (lds_ptr<int>)s_gpos;
+    for (int i = threadIdx.x; i < s_groups * C; i += SORT_NT) hist[i] = 0;
+    __syncthreads();
+    int hb[SORT_PPT];
+#pragma unroll
+    for (int j = 0; j < SORT_PPT; ++j) {
+      const int64_t i = base + threadIdx.x + SORT_NT * j;
+      if (i >= a.n) continue;
+      const int cl = ((cell_ix(c[j]) - 1) & ((1 << twl) - 1)) | ((cell_iy(c[j]) - 1) & ((1 << thl) - 1)) << twl;
+      hb[j] = (int)((unsigned)s_n[slot[j]] >> 23) * C + cl;
+      rank[j] = __hip_atomic_fetch_add(hist + hb[j], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < s_groups; d += SORT_NT) {
+      int sum = 0;
+      for (int q = 0; q < C; ++q) {
+        const int t = hist[d * C + q];
+        hist[d * C + q] = sum;
+        sum += t;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SORT_PPT; ++j) {
+      const int64_t i = base + threadIdx.x + SORT_NT * j;
+      if (i >= a.n) continue;
+      rank[j] += hist[hb[j]];
+    }
+    __syncthreads();  // s_gpos is rewritten below
+  }
   // local index of each particle: groups contiguous, interleaved inside
 #pragma unroll
   for (int j = 0; j < SORT_PPT; ++j) {
     const int64_t i = base + threadIdx.x + SORT_NT * j;
     if (i >= a.n) continue;
-    const int nu = s_n[slot[j]], n = nu & 0xFFFF, inv = nu >> 16;
+    const int nu = s_n[slot[j]], n = nu & 0xFFF, inv = (nu >> 12) & 0x7FF;
     const int r = n > 1 ? (rank[j] * inv) % n : 0;
     const int loc = keys[slot[j]] + r;
     s_gpos[loc] = cnt[slot[j]] + r;
