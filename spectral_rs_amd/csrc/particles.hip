@@ -71,11 +71,13 @@ template <class T> __device__ __forceinline__ T gamma_of(T px, T py, T pz, T csq
 }
 
 // Prtl::update_position then Prtl::apply_bc for one particle
-// (src/prtls.rs:85-121, :123-190).  `c1` = (0.5 dt) * (1/psa).
-template <class T>
+// (src/prtls.rs:85-121, :123-190).  `c1` = (0.5 dt) * (1/psa).  Q: the
+// particle's chunk is quiet (see k_particles): no boundary is within reach this
+// step, so apply_bc leaves it alone and only the cell crossing remains.
+template <bool Q, class T>
 __device__ __forceinline__ void move_half(int& ix, int& iy, T& dx, T& dy, T& px, T py, T c1, const Geo<T>& g,
                                           int& bad) {
-  if (ix <= 0 || iy <= 0) bad |= F_CELL;
+  if (!Q && (ix <= 0 || iy <= 0)) bad |= F_CELL;
   // a crossing moves by one cell: d - 1 above 0.5, d - (-1) = d + 1 below
   // -0.5, d - 0 = d otherwise (all exact, as the reference's branches)
   dx += c1 * px;
@@ -96,6 +98,7 @@ __device__ __forceinline__ void move_half(int& ix, int& iy, T& dx, T& dy, T& px,
   const int sy = (dy >= T(0.5)) - (dy < T(-0.5));
   dy -= T(sy);
   iy += sy;
+  if (Q) return;
   iy += (iy < 1 ? g.ny : 0) - (iy > g.ny ? g.ny : 0);
   if (g.periodic_x) {
     if (ix < 1) ix += g.nx;
@@ -219,10 +222,10 @@ template <int WMAX> __device__ __forceinline__ constexpr int fwin_off(int k) {
 // The same ordered 9-term sums as gather6, from the window when the stencil
 // lies in it, else from the grids with global loads (plain pointers: buffer
 // descriptors would hold 24 scalar registers through the particle loop).
-template <int WMAX>
+template <bool Q, int WMAX>
 __device__ __forceinline__ void gather6w(const GridSet& gs, const FieldWin& l, const Geo<float>& g, int ix, int iy,
                                          const float (&w)[9], float (&f)[6]) {
-  if (l.use && ix - 1 >= l.x0 && ix + 1 < l.x0 + l.w && iy - 1 >= l.y0 && iy + 1 < l.y0 + l.h) {
+  if (Q || (l.use && ix - 1 >= l.x0 && ix + 1 < l.x0 + l.w && iy - 1 >= l.y0 && iy + 1 < l.y0 + l.h)) {
     const lds_ptr<int> p = l.p + (__mul24(iy - 1 - l.y0, l.w) + (ix - 1 - l.x0));
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -367,12 +370,13 @@ __device__ __forceinline__ void lds_addi(lds_ptr<int> p, int v) {
 
 // Sim::deposit_current (src/lib.rs:266-393): v = q p / gamma, J += w v on the
 // 3x3 stencil around (ix, iy), ghosts included.  `inv` = 1 / psa.
-template <class T>
+template <bool Q, class T>
 __device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px, T py, T pz, T inv, T q,
                                                 const Geo<T>& g, const Sink<T>& k, int& bad) {
   // a stencil inside the window is inside the grid (the window lies in the
-  // padded grid), so the grid test is only needed off the window
-  const bool in = k.holds(ix, iy);
+  // padded grid), so the grid test is only needed off the window; in a quiet
+  // chunk every stencil is inside it
+  const bool in = Q || k.holds(ix, iy);
   if constexpr (Acc<T>::fixed) {
     if (in) {  // fixed point: contributions (wy v 2^e) wx, rounded
       constexpr int W = ChunkShape<T>::WMAX;
@@ -394,7 +398,7 @@ __device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px
       return;
     }
   }
-  if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
+  if (!Q && !inside(ix, iy, g)) { bad |= F_CELL; return; }
   const T vx = q * px * inv, vy = q * py * inv, vz = q * pz * inv;
   T w[9];
   tsc(dx, dy, w);
@@ -407,10 +411,10 @@ __device__ __forceinline__ void deposit_current(int ix, int iy, T dx, T dy, T px
 // per pass, so the pass accumulates only its density sum w (window and
 // fallback alike, into GridSet::dsp); the launcher then adds q * dsp to rho and
 // dsp to dens (fold_species_density).
-template <class T>
+template <bool Q, class T>
 __device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, const Geo<T>& g, const Sink<T>& k,
                                                 int& bad) {
-  const bool in = k.holds(ix, iy);
+  const bool in = Q || k.holds(ix, iy);
   if constexpr (Acc<T>::fixed) {
     if (in) {
       constexpr int W = ChunkShape<T>::WMAX;
@@ -426,7 +430,7 @@ __device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, cons
       return;
     }
   }
-  if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
+  if (!Q && !inside(ix, iy, g)) { bad |= F_CELL; return; }
   T w[9];
   tsc(dx, dy, w);
   add9(k, 3, 3, ix, iy, in, w, T(1));
@@ -436,15 +440,15 @@ __device__ __forceinline__ void deposit_density(int ix, int iy, T dx, T dy, cons
 // TSC gather of E and B, then the relativistic Boris rotation.
 // `Fields` is a GridSet (FP32: with the LDS window `fw`, global loads outside
 // it; FP64: global loads) or, for the FP32 stand-alone push, FieldBufs.
-template <class T, class Fields>
+template <bool Q, class T, class Fields>
 __device__ __forceinline__ void boris(int ix, int iy, T dx, T dy, T& px, T& py, T& pz, const Fields& fs,
                                       const FieldWin& fw, const Geo<T>& g, T alpha, T beta, int& bad) {
-  if (!inside(ix, iy, g)) { bad |= F_CELL; return; }
+  if (!Q && !inside(ix, iy, g)) { bad |= F_CELL; return; }
   T w[9];
   tsc(dx, dy, w);
   T f[6];  // Ex Ey Ez Bx By Bz
   if constexpr (sizeof(T) == 4 && std::is_same<Fields, GridSet>::value)
-    gather6w<ChunkShape<float>::WMAX>(fs, fw, g, ix, iy, w, f);
+    gather6w<Q, ChunkShape<float>::WMAX>(fs, fw, g, ix, iy, w, f);
   else
     gather6(fs, g, ix, iy, w, f);
   const T ext = f[0] * beta, eyt = f[1] * beta, ezt = f[2] * beta;
@@ -468,16 +472,16 @@ template <class T> __device__ __forceinline__ bool finite3(T a, T b, T c) {
 }
 
 // Sim::move_and_deposit (src/lib.rs:478-489) for one particle.
-template <class T>
+template <bool Q, class T>
 __device__ __forceinline__ void move_and_deposit(int& ix, int& iy, T& dx, T& dy, T& px, T py, T pz, T q,
                                                  const Geo<T>& g, const Sink<T>& k, int& bad) {
   const T psa = gamma_of(px, py, pz, g.csqinv);
   const T inv = rcp_exact(psa);
   const T c1 = g.half_dt * inv;
-  move_half(ix, iy, dx, dy, px, py, c1, g, bad);
-  deposit_current(ix, iy, dx, dy, px, py, pz, inv, q, g, k, bad);
-  move_half(ix, iy, dx, dy, px, py, c1, g, bad);
-  deposit_density(ix, iy, dx, dy, g, k, bad);
+  move_half<Q>(ix, iy, dx, dy, px, py, c1, g, bad);
+  deposit_current<Q>(ix, iy, dx, dy, px, py, pz, inv, q, g, k, bad);
+  move_half<Q>(ix, iy, dx, dy, px, py, c1, g, bad);
+  deposit_density<Q>(ix, iy, dx, dy, g, k, bad);
 }
 
 __device__ __forceinline__ int wave_or(int v) {
@@ -587,7 +591,8 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   k.y0 = max(s_box[2] - 2, 0);
   k.w = min(s_box[1] + 2, g.nx + 1) - k.x0 + 1;
   k.h = min(s_box[3] + 2, g.ny + 1) - k.y0 + 1;
-  if (g.lds_ok && cnt > 0 && k.w * k.h > WMAX) {
+  const bool whole_box = k.w * k.h <= WMAX;  // the windows cover every particle of the chunk
+  if (g.lds_ok && cnt > 0 && !whole_box) {
     // A few particles far from the rest (crossed the periodic y boundary since
     // the last sort, or a chunk that spans the end of the tile order): box
     // only the particles within RBOX cells of the chunk's middle particle;
@@ -708,8 +713,25 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   __syncthreads();
 
   int bad = 0;
-#pragma unroll
-  for (int j = 0; j < PPT; ++j) {
+  // A quiet chunk: every particle's cell stays in the grid interior this step
+  // (one crossing at most, c dt < 1) and away from the wall, the left edge
+  // and, for periodic x, the x ends; the box is whole and both windows are in
+  // LDS unclipped.  Then no boundary or assert can act, every stencil lies in
+  // the windows, and the particles take a path without those tests (the
+  // arithmetic is the same).
+  bool quiet = use_lds && whole_box && s_box[2] >= 2 && s_box[3] <= g.ny - 1;
+  if constexpr (PUSH && sizeof(T) == 4) quiet = quiet && fw.use;
+  {
+    int xlo = 2, xhi = g.nx - 1;
+    if (!g.periodic_x) {
+      xhi = min(xhi, g.wall - 2);
+      if (g.left_edge == SP_LEFT_REFLECT) xlo = max(xlo, g.lwall + 2);
+      else if (g.checked) xlo = max(xlo, g.delta + 2);
+    }
+    quiet = quiet && s_box[0] >= xlo && s_box[1] <= xhi;
+  }
+  const auto particle = [&](auto q, int j) {
+    constexpr bool Q = decltype(q)::value;
     if constexpr (DIRECT) {
       // the sort interleaves each tile's particles (k_tile_pass), so
       // consecutive particles, hence lanes, sit in different cells
@@ -723,12 +745,12 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
         int ix = cell_ix(c), iy = cell_iy(c);
         T dx = pdx[i], dy = pdy[i], px = ppx[i], py = ppy[i], pz = ppz[i];
         if (PUSH) {
-          boris(ix, iy, dx, dy, px, py, pz, gs, fw, g, s.alpha, s.beta, bad);
+          boris<Q>(ix, iy, dx, dy, px, py, pz, gs, fw, g, s.alpha, s.beta, bad);
           if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
           ppy[i] = py;
           ppz[i] = pz;
         }
-        move_and_deposit(ix, iy, dx, dy, px, py, pz, s.charge, g, k, bad);
+        move_and_deposit<Q>(ix, iy, dx, dy, px, py, pz, s.charge, g, k, bad);
         pc[i] = make_cell(ix, iy);
         pdx[i] = dx;
         pdy[i] = dy;
@@ -741,18 +763,25 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
         int ix = cell_ix(c), iy = cell_iy(c);
         T dx = s_dx[p], dy = s_dy[p], px = s_px[p], py = s_py[p], pz = s_pz[p];
         if (PUSH) {
-          boris(ix, iy, dx, dy, px, py, pz, gs, fw, g, s.alpha, s.beta, bad);
+          boris<Q>(ix, iy, dx, dy, px, py, pz, gs, fw, g, s.alpha, s.beta, bad);
           if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
           s_py[p] = py;
           s_pz[p] = pz;
         }
-        move_and_deposit(ix, iy, dx, dy, px, py, pz, s.charge, g, k, bad);
+        move_and_deposit<Q>(ix, iy, dx, dy, px, py, pz, s.charge, g, k, bad);
         s_cell[p] = make_cell(ix, iy);
         s_dx[p] = dx;
         s_dy[p] = dy;
         s_px[p] = px;
       }
     }
+  };
+  if (quiet) {
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) particle(std::true_type{}, j);
+  } else {
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) particle(std::false_type{}, j);
   }
   __syncthreads();
   if constexpr (!DIRECT) {
@@ -800,7 +829,7 @@ __global__ void __launch_bounds__(256) k_push(SpeciesDev<T> s, Geo<T> g, GridSet
     T px = s.px[i], py = s.py[i], pz = s.pz[i];
     FieldWin none{};
     none.use = false;
-    boris(cell_ix(c), cell_iy(c), s.dx[i], s.dy[i], px, py, pz, fields, none, g, s.alpha, s.beta, bad);
+    boris<false>(cell_ix(c), cell_iy(c), s.dx[i], s.dy[i], px, py, pz, fields, none, g, s.alpha, s.beta, bad);
     if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
     s.px[i] = px;
     s.py[i] = py;
