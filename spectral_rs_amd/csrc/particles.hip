@@ -10,6 +10,7 @@
 // summation order only.
 #pragma clang fp contract(off)
 
+#include <climits>
 #include <type_traits>
 
 #include "sp_internal.h"
@@ -199,6 +200,11 @@ __device__ __forceinline__ void gather6(const FieldBufs& fb, const Geo<float>& g
     f[k] = s;
   }
 }
+
+// i / w for 0 <= i < 2^22 and w >= 1 from rw = 1/w in FP32: (i + 1/2) / w lies
+// at least 1/(2w) from an integer, and (i + 1/2) * rw (rw and the product each
+// rounded once) is within 2^-23 (i + 1/2) / w < 1/(2w) of it.
+__device__ __forceinline__ int row_of(int i, float rw) { return (int)(((float)i + 0.5f) * rw); }
 
 // The six field grids over one chunk's cell box + 1 cell (every stencil of a
 // particle in the box), staged in LDS by the FP32 fused pass.  They sit at
@@ -540,8 +546,13 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   }
   __syncthreads();
   // Particle i = tid + NT j of the chunk; DIRECT keeps its cell in a register.
+  // The box is reduced on the packed cells (ix, iy) as 16-bit halves with
+  // packed min/max (grids are at most 8192 cells a side; a corrupt cell with a
+  // half >= 2^15 reads as negative, which only widens the box and keeps the
+  // chunk off the quiet path).
   int32_t creg[DIRECT ? PPT : 1];
-  int xmn = INT_MAX, xmx = INT_MIN, ymn = INT_MAX, ymx = INT_MIN;
+  typedef short pk16 __attribute__((ext_vector_type(2)));
+  pk16 bmn = {SHRT_MAX, SHRT_MAX}, bmx = {SHRT_MIN, SHRT_MIN};
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     const int i = tid + NT * j;
@@ -557,11 +568,8 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
       s_py[i] = s.py[base + i];
       s_pz[i] = s.pz[base + i];
     }
-    const int ix = cell_ix(c), iy = cell_iy(c);
-    xmn = min(xmn, ix);
-    xmx = max(xmx, ix);
-    ymn = min(ymn, iy);
-    ymx = max(ymx, iy);
+    bmn = __builtin_elementwise_min(bmn, __builtin_bit_cast(pk16, c));
+    bmx = __builtin_elementwise_max(bmx, __builtin_bit_cast(pk16, c));
   }
   const auto cell_of = [&](int j) -> int32_t {
     if constexpr (DIRECT) return creg[j];
@@ -569,16 +577,14 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   };
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    xmn = min(xmn, __shfl_xor(xmn, off));
-    xmx = max(xmx, __shfl_xor(xmx, off));
-    ymn = min(ymn, __shfl_xor(ymn, off));
-    ymx = max(ymx, __shfl_xor(ymx, off));
+    bmn = __builtin_elementwise_min(bmn, __builtin_bit_cast(pk16, __shfl_xor(__builtin_bit_cast(int, bmn), off)));
+    bmx = __builtin_elementwise_max(bmx, __builtin_bit_cast(pk16, __shfl_xor(__builtin_bit_cast(int, bmx), off)));
   }
   if ((tid & 63) == 0) {
-    atomicMin(&s_box[0], xmn);
-    atomicMax(&s_box[1], xmx);
-    atomicMin(&s_box[2], ymn);
-    atomicMax(&s_box[3], ymx);
+    atomicMin(&s_box[0], (int)bmn.x);
+    atomicMax(&s_box[1], (int)bmx.x);
+    atomicMin(&s_box[2], (int)bmn.y);
+    atomicMax(&s_box[3], (int)bmx.y);
   }
   __syncthreads();
   Sink<T> k;
@@ -591,7 +597,10 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   k.y0 = max(s_box[2] - 2, 0);
   k.w = min(s_box[1] + 2, g.nx + 1) - k.x0 + 1;
   k.h = min(s_box[3] + 2, g.ny + 1) - k.y0 + 1;
-  const bool whole_box = k.w * k.h <= WMAX;  // the windows cover every particle of the chunk
+  // the windows cover every particle of the chunk, and no cell was clipped
+  // (so the unsigned ix of every cell lies in the box as well)
+  const bool whole_box = k.w * k.h <= WMAX && s_box[0] >= 0 && s_box[1] <= g.nx + 1 && s_box[2] >= 0 &&
+                         s_box[3] <= g.ny + 1;
   if (g.lds_ok && cnt > 0 && !whole_box) {
     // A few particles far from the rest (crossed the periodic y boundary since
     // the last sort, or a chunk that spans the end of the tile order): box
@@ -650,13 +659,23 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
       lds_ptr<int> cntg = (lds_ptr<int>)s_acc;
       for (int i = tid; i < k.wh; i += NT) cntg[i] = 0;
       __syncthreads();
+      if (whole_box) {  // every particle's cell is in the window
 #pragma unroll
-      for (int j = 0; j < PPT; ++j) {
-        if (tid + NT * j >= cnt) break;
-        const int32_t c = cell_of(j);
-        const int cx = cell_ix(c) - k.x0, cy = cell_iy(c) - k.y0;
-        if (cx >= 0 && cx < k.w && cy >= 0 && cy < k.h)  // particles outside deposit globally
-          __hip_atomic_fetch_add(cntg + __mul24(cy, k.w) + cx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (int j = 0; j < PPT; ++j) {
+          if (tid + NT * j >= cnt) break;
+          const int32_t c = cell_of(j);
+          __hip_atomic_fetch_add(cntg + __mul24(cell_iy(c) - k.y0, k.w) + (cell_ix(c) - k.x0), 1, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+          if (tid + NT * j >= cnt) break;
+          const int32_t c = cell_of(j);
+          const int cx = cell_ix(c) - k.x0, cy = cell_iy(c) - k.y0;
+          if (cx >= 0 && cx < k.w && cy >= 0 && cy < k.h)  // particles outside deposit globally
+            __hip_atomic_fetch_add(cntg + __mul24(cy, k.w) + cx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
       }
       __syncthreads();
       int m = 0;
@@ -701,8 +720,9 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   if constexpr (PUSH && sizeof(T) == 4) {
     if (fw.use) {
       const int n = fw.w * fw.h;
+      const float rw = 1.0f / (float)fw.w;
       for (int i = tid; i < n; i += NT) {
-        const int r = i / fw.w, c = i - r * fw.w;
+        const int r = row_of(i, rw), c = i - r * fw.w;
         const int o = __mul24(fw.y0 + r, g.pitch) + fw.x0 + c;
 #pragma unroll
         for (int f = 0; f < 6; ++f)
@@ -796,8 +816,9 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
       }
     }
   }
+  const float rk = 1.0f / (float)max(k.w, 1);
   for (int i = tid; i < k.wh; i += NT) {
-    const int r = i / k.w, cc = i - r * k.w;
+    const int r = row_of(i, rk), cc = i - r * k.w;
     const int64_t o = (int64_t)(k.y0 + r) * g.pitch + k.x0 + cc;
 #pragma unroll
     for (int pl = 0; pl < 4; ++pl) {
