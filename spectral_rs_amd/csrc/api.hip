@@ -278,6 +278,12 @@ template <class T> void fields_update(sp_ctx* c) {
       launch_update_ghosts<float>(g4, 4, nx, ny, c->st);
     }
   }
+  // Every field transformed here is real and nx, ny are even (sp_create), so
+  // the step runs a real-field pipeline: rows two at a time into half spectra,
+  // columns kx = 0 .. nx/2 (forward: plus their conjugate mirrors, so the
+  // stored J, rho spectra are full and exactly Hermitian), the update on those,
+  // and back.  (With even sizes the Nyquist column nx/2 and row ny/2 that the
+  // update zeroes, mod.rs:230-266, are their own mirrors.)
   {  // mod.rs:97-103, 122-124: 7 forward transforms, spatial->spectral fused
     FftBatch b{};
     const int ids[7] = {SP_JX, SP_JY, SP_JZ, SP_EX, SP_EY, SP_EZ, SP_RHO};
@@ -288,8 +294,8 @@ template <class T> void fields_update(sp_ctx* c) {
       b.scale[i] = 1.0;
     }
     b.scale[6] = (double)(T(1.0) / (T)c->p.dens);  // mod.rs:80-84
-    launch_fft_rows<T>(b, c->fx, ny, 0, LOAD_REAL_PADDED, STORE_COMPLEX, 1.0, c->st);
-    launch_fft_cols<T>(b, c->fy, nx, 0, 1.0, c->st);
+    launch_fft_rows_r2c<T>(b, c->fx, ny, c->st);
+    launch_fft_cols<T>(b, c->fy, nx, 0, 1.0, c->st, nx / 2 + 1, true);
   }
   // mod.rs:105-278, with the forward spectra (transforms of real fields) kept
   // exactly Hermitian (fields.hip)
@@ -304,8 +310,8 @@ template <class T> void fields_update(sp_ctx* c) {
       b.scale[i] = 1.0;
     }
     const T norm = T(1) / (T)((int64_t)nx * ny);  // fft_2d.rs:170
-    launch_fft_cols<T>(b, c->fy, nx, 1, 1.0, c->st);
-    launch_fft_rows<T>(b, c->fx, ny, 1, LOAD_COMPLEX, STORE_REAL_PADDED, (double)norm, c->st);
+    launch_fft_cols<T>(b, c->fy, nx, 1, 1.0, c->st, nx / 2 + 1, false);
+    launch_fft_rows_c2r<T>(b, c->fx, ny, (double)norm, c->st);
   }
 }
 

@@ -299,10 +299,108 @@ __global__ void __launch_bounds__(ROWS_NT_MAX) k_fft_rows(FftBatch b, FftPlan pl
   }
 }
 
-// Column transforms: block (bx, field) handles columns [bx*C, bx*C + C).
-// POW2 launches have C a power of two dividing nx.
-template <class T, bool INV, bool POW2>
-__global__ void __launch_bounds__(COLS_NT_MAX) k_fft_cols(FftBatch b, FftPlan pl, int nx, int C, T scale_store) {
+// Real-field rows, forward (the step's pipeline, even nx and ny): complex row
+// q of block (bx, field) is z = a + i b for the real interior rows a = 2q,
+// b = 2q + 1 of the padded grid (times the load scale).  After its transform
+// Z, the transforms of the two real rows are A[k] = (Z[k] + conj(Z[-k])) / 2
+// and B[k] = (Z[k] - conj(Z[-k])) / 2i; their halves k = 0 .. nx/2 go to the
+// spectrum rows a and b (the other halves are their conjugate mirrors, which
+// the column pass writes).
+template <class T, bool POW2>
+__global__ void __launch_bounds__(ROWS_NT_MAX) k_fft_rows_r2c(FftBatch b, FftPlan pl, Div dn, Div dh, int nq, int rpb) {
+  extern __shared__ unsigned char smem[];
+  V2<T>* lds = (V2<T>*)smem;
+  const int f = blockIdx.y;
+  const int nx = pl.n, P = nx + 2, H = nx / 2 + 1;
+  const int q0 = blockIdx.x * rpb;
+  const int rows = min(rpb, nq - q0);
+  const int E = rows * nx;
+  const int nt = blockDim.x;
+  V2<T>* spec = (V2<T>*)b.spec[f];
+  const T* grid = (const T*)b.grid[f];
+  const T sc = (T)b.scale[f];
+  const bool scaled = b.scale[f] != 1.0;
+  for (int i = threadIdx.x; i < E; i += nt) {
+    const int s = (int)qdiv<POW2>(dn, i), n = i - s * nx;
+    const int64_t o = (int64_t)(2 * (q0 + s) + 1) * P + n + 1;  // padded row 2q + 1 = interior row 2q
+    T va = grid[o], vb = grid[o + P];
+    if (scaled) {
+      va = va * sc;
+      vb = vb * sc;
+    }
+    lds[lp<true>(i)] = V2<T>{va, vb};
+  }
+  __syncthreads();
+  fft_lds<T, false, false, POW2, EPT_ROWS>(lds, pl, dn, rows, nx, 1, nt);
+  const T h = T(0.5);
+  for (int i = threadIdx.x; i < rows * H; i += nt) {
+    const int s = (int)qdiv<false>(dh, i), k = i - s * H;
+    const V2<T> zk = lds[lp<true>(s * nx + k)], zm = lds[lp<true>(s * nx + (k == 0 ? 0 : nx - k))];
+    const int64_t o = (int64_t)(2 * (q0 + s)) * nx + k;
+    spec[o] = V2<T>{h * (zk.x + zm.x), h * (zk.y - zm.y)};
+    spec[o + nx] = V2<T>{h * (zk.y + zm.y), h * (zm.x - zk.x)};
+  }
+}
+
+// Real-field rows, inverse: the spectrum rows a = 2q and b = 2q + 1 hold the
+// halves k = 0 .. nx/2 of the row transforms of two real rows (the column
+// pass has run on those columns only); the other halves are their conjugate
+// mirrors.  Z = A + i B is inverse-transformed once: Re Z -> row a, Im Z -> row
+// b, as copy_to_spatial + update_ghosts (the real part, times the norm, the
+// ghost columns and the mirrored ghost rows).  The imaginary parts of A and B
+// at k = 0 and nx/2 are dropped: in the reference's complex inverse they only
+// reach the imaginary part that copy_to_spatial discards.
+template <class T, bool POW2>
+__global__ void __launch_bounds__(ROWS_NT_MAX) k_fft_rows_c2r(FftBatch b, FftPlan pl, Div dn, Div dP, int ny, int rpb,
+                                                          T scale_store) {
+  extern __shared__ unsigned char smem[];
+  V2<T>* lds = (V2<T>*)smem;
+  const int f = blockIdx.y;
+  const int nx = pl.n, P = nx + 2, half = nx / 2, nq = ny / 2;
+  const int q0 = blockIdx.x * rpb;
+  const int rows = min(rpb, nq - q0);
+  const int E = rows * nx;
+  const int nt = blockDim.x;
+  const V2<T>* spec = (const V2<T>*)b.spec[f];
+  T* grid = (T*)b.grid[f];
+  for (int i = threadIdx.x; i < E; i += nt) {
+    const int s = (int)qdiv<POW2>(dn, i), k = i - s * nx;
+    const int64_t o = (int64_t)(2 * (q0 + s)) * nx;
+    V2<T> z;
+    if (k <= half) {
+      V2<T> a = spec[o + k], c = spec[o + nx + k];
+      if (k == 0 || k == half) a.y = c.y = T(0);
+      z = V2<T>{a.x - c.y, a.y + c.x};
+    } else {  // A[k] = conj(A[nx - k]), B[k] = conj(B[nx - k])
+      const V2<T> a = spec[o + nx - k], c = spec[o + nx + nx - k];
+      z = V2<T>{a.x + c.y, c.x - a.y};
+    }
+    lds[lp<true>(i)] = z;
+  }
+  __syncthreads();
+  fft_lds<T, true, false, POW2, EPT_ROWS>(lds, pl, dn, rows, nx, 1, nt);
+  for (int i = threadIdx.x; i < rows * P; i += nt) {
+    const int s = (int)qdiv<false>(dP, i), c = i - s * P;
+    const int n = (c == 0) ? nx - 1 : (c == P - 1 ? 0 : c - 1);
+    const V2<T> z = lds[lp<true>(s * nx + n)];
+    const T va = z.x * scale_store, vb = z.y * scale_store;
+    const int ra = 2 * (q0 + s) + 1, rb = ra + 1;  // padded rows of the pair
+    grid[(int64_t)ra * P + c] = va;
+    grid[(int64_t)rb * P + c] = vb;
+    if (ra == 1) grid[(int64_t)(ny + 1) * P + c] = va;
+    if (rb == ny) grid[c] = vb;
+  }
+}
+
+// Column transforms: block (bx, field) handles columns [bx*C, bx*C + C) of the
+// first ncols columns.  POW2 launches have C a power of two and every block's
+// column count a power of two.  MIRROR (forward transforms of real fields, on
+// columns 0 .. nx/2): column nx - c is written as the conjugate mirror of
+// column c, F[-ky][nx - c] = conj(F[ky][c]), except for the self-mirrored
+// columns 0 and nx/2.
+template <class T, bool INV, bool POW2, bool MIRROR>
+__global__ void __launch_bounds__(COLS_NT_MAX) k_fft_cols(FftBatch b, FftPlan pl, int nx, int ncols, int C,
+                                                          T scale_store) {
   extern __shared__ unsigned char smem[];
   V2<T>* lds = (V2<T>*)smem;
   const int f = blockIdx.y;
@@ -312,7 +410,7 @@ __global__ void __launch_bounds__(COLS_NT_MAX) k_fft_cols(FftBatch b, FftPlan pl
   // neighbours share its L2.
   const int bx = SP_FFT_XCD_REMAP ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   const int c0 = bx * C;
-  const int cols = min(C, nx - c0);
+  const int cols = min(C, ncols - c0);
   const int E = cols * ny;
   const int nt = blockDim.x;
   V2<T>* spec = (V2<T>*)b.spec[f];
@@ -329,6 +427,10 @@ __global__ void __launch_bounds__(COLS_NT_MAX) k_fft_cols(FftBatch b, FftPlan pl
     V2<T> v = lds[i];
     if (scaled) v = V2<T>{v.x * scale_store, v.y * scale_store};
     spec[(int64_t)n * nx + c0 + c] = v;
+    if constexpr (MIRROR) {
+      const int cc = c0 + c;
+      if (cc != 0 && 2 * cc != nx) spec[(int64_t)(n == 0 ? 0 : ny - n) * nx + (nx - cc)] = V2<T>{v.x, -v.y};
+    }
   }
 }
 
@@ -363,8 +465,10 @@ template <class T> static void allow_big_lds() {
   std::call_once(once, [] {
     const int bytes = lds_slots(fft_max_len<T>()) * (int)sizeof(V2<T>);
     // a refusal surfaces as the launch's own error
-    const void* k[4] = {(const void*)k_fft_rows<T, false, false>, (const void*)k_fft_rows<T, true, false>,
-                        (const void*)k_fft_rows<T, false, true>, (const void*)k_fft_rows<T, true, true>};
+    const void* k[8] = {(const void*)k_fft_rows<T, false, false>, (const void*)k_fft_rows<T, true, false>,
+                        (const void*)k_fft_rows<T, false, true>,  (const void*)k_fft_rows<T, true, true>,
+                        (const void*)k_fft_rows_r2c<T, false>,    (const void*)k_fft_rows_r2c<T, true>,
+                        (const void*)k_fft_rows_c2r<T, false>,    (const void*)k_fft_rows_c2r<T, true>};
     for (const void* f : k) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   });
 }
@@ -419,30 +523,78 @@ void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, i
   else inverse ? go(k_fft_rows<T, true, false>) : go(k_fft_rows<T, false, false>);
 }
 
+// Rows per workgroup of a row launch over `nrows` transforms: the plan's
+// choice (default 1), within LDS and the grid.
+template <class T> static int rows_per_block(const FftDim& dx, int nrows) {
+  int rpb = dx.per_block > 0 ? dx.per_block : 1;
+  if (rpb > max_elems<T>() / dx.n) rpb = max_elems<T>() / dx.n;
+  if (rpb < 1) rpb = 1;
+  if (rpb > nrows) rpb = nrows;
+  return rpb;
+}
+
+template <class T> void launch_fft_rows_r2c(const FftBatch& b, const FftDim& dx, int ny, hipStream_t st) {
+  const int nq = ny / 2, rpb = rows_per_block<T>(dx, nq);
+  const int E = rpb * dx.n;
+  const int nt = round_threads(E, EPT_ROWS, ROWS_NT_MAX);
+  const size_t shm = (size_t)lds_slots(E) * sizeof(V2<T>);
+  allow_big_lds<T>();
+  dim3 grid((nq + rpb - 1) / rpb, b.count);
+  bool pow2;
+  const FftPlan pl = make_plan(dx, pow2);
+  const Div dn = make_div((uint32_t)dx.n), dh = make_div((uint32_t)(dx.n / 2 + 1));
+  if (pow2) hipLaunchKernelGGL((k_fft_rows_r2c<T, true>), grid, dim3(nt), shm, st, b, pl, dn, dh, nq, rpb);
+  else hipLaunchKernelGGL((k_fft_rows_r2c<T, false>), grid, dim3(nt), shm, st, b, pl, dn, dh, nq, rpb);
+}
+
 template <class T>
-void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, double store_scale, hipStream_t st) {
+void launch_fft_rows_c2r(const FftBatch& b, const FftDim& dx, int ny, double store_scale, hipStream_t st) {
+  const int nq = ny / 2, rpb = rows_per_block<T>(dx, nq);
+  const int E = rpb * dx.n;
+  const int nt = round_threads(E, EPT_ROWS, ROWS_NT_MAX);
+  const size_t shm = (size_t)lds_slots(E) * sizeof(V2<T>);
+  allow_big_lds<T>();
+  dim3 grid((nq + rpb - 1) / rpb, b.count);
+  bool pow2;
+  const FftPlan pl = make_plan(dx, pow2);
+  const Div dn = make_div((uint32_t)dx.n), dP = make_div((uint32_t)(dx.n + 2));
+  const T scale = (T)store_scale;
+  if (pow2) hipLaunchKernelGGL((k_fft_rows_c2r<T, true>), grid, dim3(nt), shm, st, b, pl, dn, dP, ny, rpb, scale);
+  else hipLaunchKernelGGL((k_fft_rows_c2r<T, false>), grid, dim3(nt), shm, st, b, pl, dn, dP, ny, rpb, scale);
+}
+
+// ncols: the first ncols columns (nx for a full transform, nx/2 + 1 for the
+// real-field pipeline); mirror: write the conjugate mirrors (forward only).
+template <class T>
+void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, double store_scale, hipStream_t st,
+                     int ncols, bool mirror) {
+  if (ncols <= 0) ncols = nx;
   // columns per workgroup: the plan's choice (default 8), within LDS and the grid
   int C = dy.per_block > 0 ? dy.per_block : 8;
   if (C > max_elems<T>() / dy.n) C = max_elems<T>() / dy.n;
   if (C < 1) C = 1;
-  if (C > nx) C = nx;
+  if (C > ncols) C = ncols;
   const int E = C * dy.n;
   const int nt = round_threads(E, EPT_COLS, COLS_NT_MAX);
   const size_t shm = (size_t)E * sizeof(V2<T>);
-  dim3 grid((nx + C - 1) / C, b.count);
+  dim3 grid((ncols + C - 1) / C, b.count);
   const T scale = (T)store_scale;
   bool pow2;
   const FftPlan pl = make_plan(dy, pow2);
-  pow2 = pow2 && (C & (C - 1)) == 0 && nx % C == 0;
-  const auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(nt), shm, st, b, pl, nx, C, scale); };
-  if (pow2) inverse ? go(k_fft_cols<T, true, true>) : go(k_fft_cols<T, false, true>);
-  else inverse ? go(k_fft_cols<T, true, false>) : go(k_fft_cols<T, false, false>);
+  const int rem = ncols % C;
+  pow2 = pow2 && (C & (C - 1)) == 0 && (rem & (rem - 1)) == 0;
+  const auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(nt), shm, st, b, pl, nx, ncols, C, scale); };
+  if (mirror && !inverse) pow2 ? go(k_fft_cols<T, false, true, true>) : go(k_fft_cols<T, false, false, true>);
+  else if (pow2) inverse ? go(k_fft_cols<T, true, true, false>) : go(k_fft_cols<T, false, true, false>);
+  else inverse ? go(k_fft_cols<T, true, false, false>) : go(k_fft_cols<T, false, false, false>);
 }
 
 #define INST(T)                                                                                 \
   template void fft_make_twiddles<T>(FftDim&, hipStream_t, bool);                               \
   template void launch_fft_rows<T>(const FftBatch&, const FftDim&, int, int, int, int, double, hipStream_t); \
-  template void launch_fft_cols<T>(const FftBatch&, const FftDim&, int, int, double, hipStream_t);
+  template void launch_fft_cols<T>(const FftBatch&, const FftDim&, int, int, double, hipStream_t, int, bool); \
+  template void launch_fft_rows_r2c<T>(const FftBatch&, const FftDim&, int, hipStream_t);                    \
+  template void launch_fft_rows_c2r<T>(const FftBatch&, const FftDim&, int, double, hipStream_t);
 INST(float)
 INST(double)
 #undef INST

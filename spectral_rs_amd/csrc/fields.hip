@@ -297,7 +297,7 @@ template <class T>
 __device__ __forceinline__ void spectral_mode(const SpecPtrs& sp, int64_t i, int row, int col, int nx, int ny, T cdt,
                                               T ckc, T twopi_x, T twopi_y, Cx<T> jx, Cx<T> jy, Cx<T> jz, Cx<T> rho,
                                               Cx<T> ex, Cx<T> ey, Cx<T> ez, double& g_pre, double& g_post,
-                                              double& g_rho) {
+                                              double& g_rho, bool eb = true) {
   const bool nyq = col == nx / 2 || row == ny / 2;
   T kx = T(col);
   if (col >= nx / 2 + 1) kx -= T(nx);
@@ -352,6 +352,7 @@ __device__ __forceinline__ void spectral_mode(const SpecPtrs& sp, int64_t i, int
   st(wxp, i, wx);
   st(wyp, i, wy);
   st(wzp, i, wz);
+  if (!eb) return;  // E and B of this mode are not read back (real-field pipeline)
   st((T*)sp.p[SP_EX], i, ex);
   st((T*)sp.p[SP_EY], i, ey);
   st((T*)sp.p[SP_EZ], i, ez);
@@ -393,18 +394,21 @@ __global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T
   if (gauss) gauss_partials(g_pre, g_post, g_rho, gauss);
 }
 
-// Hermitian projection and update in one pass: one thread per mirrored pair
-// (i, m = -i mod (Ny, Nx)) with m >= i, over rows 0..Ny/2.  The seven
-// forward spectra (Jx Jy Jz rho Ex Ey Ez) are transforms of real fields;
-// they become X[i] <- (X[i] + conj(X[m])) / 2, X[m] <- conj(X[i]) (the
-// projected J and rho are stored back), and both modes are updated by
-// spectral_mode.  Exact in exact arithmetic; in floating point it removes the
-// anti-Hermitian rounding component of the FFTs.  That component matters
+// The step's update (the real-field pipeline, even nx and ny): one thread per
+// mirrored pair (i, m = -i mod (Ny, Nx)) with m >= i, over rows 0..Ny/2.  The
+// seven forward spectra (Jx Jy Jz rho Ex Ey Ez) are transforms of real fields,
+// and the forward column pass wrote every column but 0 and nx/2 as the exact
+// conjugate mirror of another; the modes of those two self-mirrored columns
+// are Hermitian-projected here, X[i] <- (X[i] + conj(X[m])) / 2,
+// X[m] <- conj(X[i]) (J and rho stored back), elsewhere mode m is not read.
+// Both modes are updated by spectral_mode.  Exact Hermitian inputs matter
 // because B_wrk persists in spectral space: its anti-Hermitian part evolves as
 // B_a <- (1 - (c dt |k|)^2) B_a each step (the matching E part is dropped by
 // the real-part extraction), which grows for c dt |k| > sqrt(2).  With
 // Hermitian inputs spectral_mode maps mirrored modes to exactly conjugate
-// outputs, so B_wrk stays exactly Hermitian.
+// outputs, so B_wrk stays exactly Hermitian.  E and B are stored on the half
+// plane kx = 0 .. nx/2 only, which is all the inverse pipeline reads; B_wrk on
+// every mode.
 template <class T>
 __global__ void __launch_bounds__(256) k_spectral_pair(SpecPtrs sp, int nx, int ny, T cdt, T ckc, T twopi_x,
                                                        T twopi_y, double* gauss) {
@@ -417,25 +421,31 @@ __global__ void __launch_bounds__(256) k_spectral_pair(SpecPtrs sp, int nx, int 
     if (m >= i) {
       const int ids[7] = {SP_JX, SP_JY, SP_JZ, SP_RHO, SP_EX, SP_EY, SP_EZ};
       Cx<T> a[7];
+      const bool proj = col == 0 || 2 * col == nx;
+      if (proj) {
 #pragma unroll
-      for (int f = 0; f < 7; ++f) {
-        const T* p = (const T*)sp.p[ids[f]];
-        const Cx<T> u = ld(p, i), v = ld(p, m);
-        a[f] = Cx<T>{T(0.5) * (u.re + v.re), T(0.5) * (u.im - v.im)};
-      }
+        for (int f = 0; f < 7; ++f) {
+          const T* p = (const T*)sp.p[ids[f]];
+          const Cx<T> u = ld(p, i), v = ld(p, m);
+          a[f] = Cx<T>{T(0.5) * (u.re + v.re), T(0.5) * (u.im - v.im)};
+        }
 #pragma unroll
-      for (int f = 0; f < 4; ++f) {  // E is rewritten by the update
-        st((T*)sp.p[ids[f]], i, a[f]);
-        if (m != i) st((T*)sp.p[ids[f]], m, Cx<T>{a[f].re, -a[f].im});
+        for (int f = 0; f < 4; ++f) {  // E is rewritten by the update
+          st((T*)sp.p[ids[f]], i, a[f]);
+          if (m != i) st((T*)sp.p[ids[f]], m, Cx<T>{a[f].re, -a[f].im});
+        }
+      } else {
+#pragma unroll
+        for (int f = 0; f < 7; ++f) a[f] = ld((const T*)sp.p[ids[f]], i);
       }
       spectral_mode<T>(sp, i, row, col, nx, ny, cdt, ckc, twopi_x, twopi_y, a[0], a[1], a[2], a[3], a[4], a[5],
-                       a[6], g_pre, g_post, g_rho);
+                       a[6], g_pre, g_post, g_rho, 2 * col <= nx);
       if (m != i) {
         Cx<T> b[7];
 #pragma unroll
         for (int f = 0; f < 7; ++f) b[f] = Cx<T>{a[f].re, -a[f].im};
         spectral_mode<T>(sp, m, mrow, mcol, nx, ny, cdt, ckc, twopi_x, twopi_y, b[0], b[1], b[2], b[3], b[4], b[5],
-                         b[6], g_pre, g_post, g_rho);
+                         b[6], g_pre, g_post, g_rho, 2 * mcol <= nx);
       }
     }
   }
@@ -471,7 +481,7 @@ __global__ void __launch_bounds__(1024) k_gauss_reduce(const double* part, int n
 }
 
 template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
-                                               hipStream_t st, bool project) {
+                                               hipStream_t st, bool step) {
   SpecPtrs sp{};
   for (int i = 0; i < 14; ++i) sp.p[i] = spec[i];
   const T PI = (T)3.14159265358979323846264338327950288;
@@ -479,9 +489,9 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   const T twopi_y = T(2.0) * PI / T(g.ny);
   const T ckc = g.dt / T(dens);  // mod.rs:127
   const T cdt = g.dt * g.c;      // mod.rs:128
-  const int64_t n = project ? (int64_t)(g.ny / 2 + 1) * g.nx : (int64_t)g.nx * g.ny;
+  const int64_t n = step ? (int64_t)(g.ny / 2 + 1) * g.nx : (int64_t)g.nx * g.ny;
   const int nblk = (int)((n + 255) / 256);
-  if (project)
+  if (step)
     hipLaunchKernelGGL(k_spectral_pair<T>, dim3((unsigned)nblk), dim3(256), 0, st, sp, g.nx, g.ny, cdt, ckc, twopi_x,
                        twopi_y, gauss ? gauss + GAUSS_PARTIALS : nullptr);
   else

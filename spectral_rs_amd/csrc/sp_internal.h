@@ -100,10 +100,11 @@ constexpr int MAX_SPECIES = 64;
 constexpr int DIAG_FIXED = 4;
 constexpr int DIAG_DOUBLES = DIAG_FIXED + 4 * MAX_SPECIES;
 inline size_t gauss_doubles(int nx, int ny) { return GAUSS_PARTIALS + 3 * (((size_t)nx * ny + 255) / 256); }
-// `project`: Hermitian-project the seven forward spectra in the same pass (the
-// step's path; see k_spectral_pair).
+// `step`: the step's update on the real-field pipeline's spectra (projection of
+// the self-mirrored columns, E and B on the half plane; see k_spectral_pair);
+// otherwise every mode as given (sp_spectral_update).
 template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
-                                               hipStream_t st, bool project);
+                                               hipStream_t st, bool step);
 // FP64 diagnostics: out[0] += sum over the interior of a^2 + b^2 + c^2;
 // kinetic: out[0] += sum (gamma - 1), out[1..3] += sum p.
 // sum32: accumulate in FP32 (precision study), the total still stored as FP64.
@@ -134,8 +135,16 @@ struct FftBatch {
 // `store_scale` multiplies every stored value (the 1/(Nx Ny) of an inverse).
 template <class T> void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, int load, int store,
                                         double store_scale, hipStream_t st);
+// Column transforms of the first ncols columns (0: all nx); mirror (forward):
+// also write column nx - c as the conjugate mirror of column c.
 template <class T> void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, double store_scale,
-                                        hipStream_t st);
+                                        hipStream_t st, int ncols = 0, bool mirror = false);
+// The step's real-field pipeline (even nx and ny): forward rows, two real rows
+// per complex transform, into the halves kx = 0 .. nx/2 of the spectrum rows;
+// inverse rows from those halves back to the padded real grids.
+template <class T> void launch_fft_rows_r2c(const FftBatch& b, const FftDim& dx, int ny, hipStream_t st);
+template <class T> void launch_fft_rows_c2r(const FftBatch& b, const FftDim& dx, int ny, double store_scale,
+                                            hipStream_t st);
 // Largest transform length one block holds in LDS.
 template <class T> constexpr int fft_max_len() { return (int)(65536 / (2 * sizeof(T))); }
 
