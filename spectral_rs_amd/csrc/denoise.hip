@@ -396,12 +396,13 @@ __global__ void __launch_bounds__(256) k_dn_hp(Grids4 in, Grids4 out, const floa
 // stay in LDS as BF16 (the precision the next layer's MFMA reads anyway),
 // pixel-major with the channels of a pixel contiguous (stride FCS, padded so
 // that the 16 lanes of a group read distinct banks), in two ping-pong
-// buffers.  The K order is tap-major (kk = tap * CIN + ci), so a lane's A
-// operand -- 8 consecutive channels of one tap of one pixel (two taps of 4
-// in the first layer) -- is two 8-byte LDS reads at compile-time offsets
-// from the lane's pixel, and each wave
-// keeps the layer's B operands (the weights) in registers for all its pixel
-// tiles.  The weights come pre-converted (BF16, tap-major, zero-padded:
+// buffers.  The K order is tap-major (kk = tap * CIN + ci), so a lane's
+// activation operand -- 8 consecutive channels of one tap of one pixel (two
+// taps of 4 in the first layer) -- is two 8-byte LDS reads at compile-time
+// offsets from the lane's pixel, and each wave keeps the layer's weight
+// operands in registers for all its pixel tiles.  The MFMA computes
+// W x A (output channels x pixels), so a lane's results are four consecutive
+// channels of one pixel, written to LDS as one 8-byte store.  The weights come pre-converted (BF16, tap-major, zero-padded:
 // dn_pack_weights) and are copied to LDS with 16-byte loads.  Activations of
 // pixels outside the image are zero, as in the layer-by-layer path (columns
 // outside; rows wrap periodically); with the same K order and k-step grouping
@@ -492,6 +493,12 @@ __device__ __forceinline__ void fused_layer(const short* s_src, short* s_w, cons
       const short* row = s_w + (j * 16 + m) * lds_kp(L) + st * 32 + k8;
       bw[st][j] = ld8(row, row + 4);
     }
+  // this lane's output-channel biases, 16 j + 4 g + i (zero past COUT)
+  float bias_l[NTL][4];
+#pragma unroll
+  for (int j = 0; j < NTL; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bias_l[j][i] = j * 16 + kq + i < COUT ? bias[j * 16 + kq + i] : 0.0f;
   uint2 wnext[WREG];
   if constexpr (L + 1 < DN_LAYERS) {
     __syncthreads();  // every wave holds its B operands: s_w is free
@@ -503,11 +510,11 @@ __device__ __forceinline__ void fused_layer(const short* s_src, short* s_w, cons
   // tap st, 8 per lane group (added to the lane's base once here).
   const int lane_off = CIN == 4 ? 0 : k8;
   for (int t = wave; t < NTILE; t += FNT / 64) {
-    const int q = min(t * 16 + m, NPX - 1);  // this lane's A-row pixel (clamped; its row is discarded)
+    const int q = min(t * 16 + m, NPX - 1);  // this lane's pixel (clamped; past NPX its column is discarded)
     const int qy = q / OW, qx = q - qy * OW;
     const short* a0 = s_src + (qy * IW + qx) * SSTR + lane_off;
-    // all A operands of the tile first, so the LDS reads are in flight
-    // together instead of one wait per k-step
+    // all activation operands of the tile first, so the LDS reads are in
+    // flight together instead of one wait per k-step
     bf16x8 av[NS];
 #pragma unroll
     for (int st = 0; st < NS; ++st) {
@@ -524,6 +531,12 @@ __device__ __forceinline__ void fused_layer(const short* s_src, short* s_w, cons
     // keep the scheduler from sinking each read next to its MFMA (which
     // would put a full LDS latency before every k-step)
     __builtin_amdgcn_sched_barrier(0);
+    // D = W x A: the weights are the MFMA's A operand (row = output channel)
+    // and the activations its B operand (column = pixel), so lane (m, g) gets
+    // output channels 16 j + 4 g + i (i < 4) of pixel t * 16 + m: four
+    // consecutive channels of one pixel, stored as one 8-byte LDS write.  The
+    // operand registers are the same as for A x W (the two fragments share
+    // their lane layout); only the roles, hence the output's transpose, differ.
     f32x4 acc[NTL];
 #pragma unroll
     for (int j = 0; j < NTL; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -531,48 +544,38 @@ __device__ __forceinline__ void fused_layer(const short* s_src, short* s_w, cons
     for (int st = 0; st < NS; ++st)
 #pragma unroll
       for (int j = 0; j < NTL; ++j)
-        acc[j] = mfma32(av[st], bw[st][j], acc[j]);
-    // this lane's 4 output pixels p0 .. p0 + 3 (OW > 4: at most one row wrap)
-    const int p0 = t * 16 + kq;
-    int py[4], px[4];
-    py[0] = p0 / OW;
-    px[0] = p0 - py[0] * OW;
-#pragma unroll
-    for (int i = 1; i < 4; ++i) {
-      const bool wrap = px[i - 1] + 1 == OW;
-      px[i] = wrap ? 0 : px[i - 1] + 1;
-      py[i] = py[i - 1] + (wrap ? 1 : 0);
-    }
+        acc[j] = mfma32(bw[st][j], av[st], acc[j]);
+    const int p = t * 16 + m;  // this lane's output pixel (= q when p < NPX)
+    if (p >= NPX) continue;
+    const int x = x0 + qx - RING;
+    const bool xin = x >= 0 && x < W;  // columns outside the image are zero; rows wrap
+    // as a factor, so the epilogue has no branch (GELU of a finite value is finite)
+    const float xm = xin ? 1.0f : 0.0f;
 #pragma unroll
     for (int j = 0; j < NTL; ++j) {
-      const int n = j * 16 + m;
-      if (n >= COUT) continue;
-      const float bn = bias[n];
+      float v[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int p = p0 + i;
-        if (p >= NPX) continue;
-        const int x = x0 + px[i] - RING;
-        float v = acc[j][i] + bn;
-        if (LAST) {
-          const int y = y0 + py[i] - RING;
-          if (y < H && x >= 0 && x < W) {
-            out[((size_t)n * H + y) * W + x] = v;
-            *rsq += isfinite(v) ? (double)v * v : __builtin_inf();
-          }
-        } else {
-          v = (x < 0 || x >= W) ? 0.0f : gelu(v);  // outside the image (columns); rows wrap
-          // channels n and n + 1 sit in lanes m and m ^ 1 (same pixel, same
-          // active state): the even lane converts both and writes one dword
-          const float w = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF,
-                                                                             0xF, false));  // quad_perm [1,0,3,2]
-          if ((m & 1) == 0) {
-            bf16x2 pk;
-            pk[0] = (__bf16)v;
-            pk[1] = (__bf16)w;
-            *(bf16x2*)(s_dst + p * FCS + n) = pk;
+      for (int i = 0; i < 4; ++i) v[i] = acc[j][i] + bias_l[j][i];
+      if constexpr (LAST) {
+        const int y = y0 + qy - RING;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int n = j * 16 + kq + i;
+          if (n < COUT && y < H && xin) {
+            out[((size_t)n * H + y) * W + x] = v[i];
+            *rsq += isfinite(v[i]) ? (double)v[i] * v[i] : __builtin_inf();
           }
         }
+      } else {
+        bf16x2 lo, hi;
+        lo[0] = (__bf16)(gelu(v[0]) * xm);
+        lo[1] = (__bf16)(gelu(v[1]) * xm);
+        hi[0] = (__bf16)(gelu(v[2]) * xm);
+        hi[1] = (__bf16)(gelu(v[3]) * xm);
+        uint2 w2;
+        w2.x = __builtin_bit_cast(unsigned, lo);
+        w2.y = __builtin_bit_cast(unsigned, hi);
+        *(uint2*)(s_dst + p * FCS + j * 16 + kq) = w2;
       }
     }
   }
