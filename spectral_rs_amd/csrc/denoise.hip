@@ -621,33 +621,49 @@ __global__ void __launch_bounds__(FNT) k_dn_fused(FusedIn in, DnParams P, const 
   __shared__ __attribute__((aligned(16))) short s_w[DN_CH * (DN_CH * 9 + 4)];
   __shared__ double s_red[FNT / 64];
   const int y0 = blockIdx.y * FY, x0 = blockIdx.x * FX;
-  double xsq = 0.0;
-  for (int i = threadIdx.x; i < DN_IN * H0 * W0; i += FNT) {
-    const int pix = i / DN_IN, c = i - pix * DN_IN;  // pixel-major, channel fastest
-    const int ry = pix / W0, rx = pix - ry * W0;
-    const int yu = y0 + ry - FRING;
-    int y = yu;
-    while (y < 0) y += H;  // rows wrap (once, unless H < FY + FRING)
-    while (y >= H) y -= H;
-    const int xx = x0 + rx - FRING;
-    float v = 0.0f;
-    if (xx >= 0 && xx < W) {
-      if constexpr (GRIDS) {
-        v = in.gr.g[c][(size_t)(y + 1) * (W + 2) + xx + 1] * in.inv_dens / in.scale[c];
-        const bool own = ry >= FRING && ry < FRING + FY && rx >= FRING && rx < FRING + FX && yu < H;
-        if (own) xsq += (double)v * v;
-      } else {
-        v = in.x[((size_t)c * H + y) * W + xx];
+  // The input region (4 channels, pixel-major) in two passes: every load of
+  // the thread first (with layer 0's weights in flight beside them), then the
+  // conversions and LDS stores, so that one workgroup per CU does not wait out
+  // each load in turn.
+  constexpr int NIN = DN_IN * H0 * W0, NSTAGE = (NIN + FNT - 1) / FNT;
+  uint2 w0[WREG];
+  weights_fetch<0>(w0, packed);
+  float sv[NSTAGE];
+#pragma unroll
+  for (int k = 0; k < NSTAGE; ++k) {
+    const int i = threadIdx.x + k * FNT;
+    sv[k] = 0.0f;
+    if (i < NIN) {
+      const int pix = i / DN_IN, c = i - pix * DN_IN;  // pixel-major, channel fastest
+      const int ry = pix / W0, rx = pix - ry * W0;
+      int y = y0 + ry - FRING;
+      while (y < 0) y += H;  // rows wrap (once, unless H < FY + FRING)
+      while (y >= H) y -= H;
+      const int xx = x0 + rx - FRING;
+      if (xx >= 0 && xx < W) {
+        if constexpr (GRIDS) sv[k] = in.gr.g[c][(size_t)(y + 1) * (W + 2) + xx + 1];
+        else sv[k] = in.x[((size_t)c * H + y) * W + xx];
       }
     }
-    s_x[i] = bf16_bits(v);
+  }
+  double xsq = 0.0;
+#pragma unroll
+  for (int k = 0; k < NSTAGE; ++k) {
+    const int i = threadIdx.x + k * FNT;
+    if (i < NIN) {
+      float v = sv[k];
+      if constexpr (GRIDS) {
+        const int pix = i / DN_IN, c = i - pix * DN_IN;
+        const int ry = pix / W0, rx = pix - ry * W0;
+        v = v * in.inv_dens / in.scale[c];
+        const bool own = ry >= FRING && ry < FRING + FY && rx >= FRING && rx < FRING + FX && y0 + ry - FRING < H;
+        if (own && x0 + rx - FRING < W) xsq += (double)v * v;
+      }
+      s_x[i] = bf16_bits(v);
+    }
   }
   if constexpr (GRIDS) fused_block_add(xsq, s_red, in.sums);
-  {
-    uint2 w0[WREG];
-    weights_fetch<0>(w0, packed);
-    weights_store<0>(w0, s_w);
-  }
+  weights_store<0>(w0, s_w);
   __syncthreads();
   fused_layer<0, DN_IN, FY + 8, FX + 8, 4, false>(s_x, s_w, P.b[0], s_a, nullptr, y0, x0, H, W, packed);
   __syncthreads();  // layer 0 is done: s_a written, layer 1's weights in s_w
