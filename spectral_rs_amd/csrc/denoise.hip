@@ -161,9 +161,15 @@ __global__ void __launch_bounds__(DN_NT) k_conv3x3(const float* __restrict__ in,
   for (int t = wave; t < (DN_TY * DN_TX) / 16; t += DN_NT / 64) {
     const int py = t / (DN_TX / 16), px = (t % (DN_TX / 16)) * 16;
     const int pbase = py * S::LX + px + m;  // window corner of pixel (py, px + m) in the staged tile
+    // the bias is the accumulator's start (as in the fused kernel, so that
+    // both paths round the same sums)
     f32x4 acc[NTL];
 #pragma unroll
-    for (int j = 0; j < NTL; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < NTL; ++j) {
+      const int n = j * 16 + m;
+      const float bn = (bias && n < COUT) ? bias[n] : 0.0f;
+      acc[j] = f32x4{bn, bn, bn, bn};
+    }
 #pragma unroll
     for (int kc = 0; kc < S::KF; kc += 32) {
       bf16x8 a, bv[NTL];
@@ -181,13 +187,12 @@ __global__ void __launch_bounds__(DN_NT) k_conv3x3(const float* __restrict__ in,
     for (int j = 0; j < NTL; ++j) {
       const int n = j * 16 + (lane & 15);
       if (n >= COUT) continue;
-      const float bn = bias ? bias[n] : 0.0f;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int y = y0 + py, x = x0 + px + kq + i;
         if (y < H && x < W) {
           const size_t o = (((size_t)b * COUT + n) * H + y) * W + x;
-          float v = acc[j][i] + bn;
+          float v = acc[j][i];
           if (POST == 2) v *= gelu_grad(mask[o]);
           out[o] = v;
         }
@@ -537,9 +542,9 @@ __device__ __forceinline__ void fused_layer(const short* s_src, short* s_w, cons
     // consecutive channels of one pixel, stored as one 8-byte LDS write.  The
     // operand registers are the same as for A x W (the two fragments share
     // their lane layout); only the roles, hence the output's transpose, differ.
-    f32x4 acc[NTL];
+    f32x4 acc[NTL];  // starting at the bias
 #pragma unroll
-    for (int j = 0; j < NTL; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < NTL; ++j) acc[j] = f32x4{bias_l[j][0], bias_l[j][1], bias_l[j][2], bias_l[j][3]};
 #pragma unroll
     for (int st = 0; st < NS; ++st)
 #pragma unroll
@@ -548,34 +553,46 @@ __device__ __forceinline__ void fused_layer(const short* s_src, short* s_w, cons
     const int p = t * 16 + m;  // this lane's output pixel (= q when p < NPX)
     if (p >= NPX) continue;
     const int x = x0 + qx - RING;
-    const bool xin = x >= 0 && x < W;  // columns outside the image are zero; rows wrap
-    // as a factor, so the epilogue has no branch (GELU of a finite value is finite)
-    const float xm = xin ? 1.0f : 0.0f;
 #pragma unroll
     for (int j = 0; j < NTL; ++j) {
       float v[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = acc[j][i] + bias_l[j][i];
+      for (int i = 0; i < 4; ++i) v[i] = acc[j][i];
       if constexpr (LAST) {
         const int y = y0 + qy - RING;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int n = j * 16 + kq + i;
-          if (n < COUT && y < H && xin) {
+          if (n < COUT && y < H && x >= 0 && x < W) {
             out[((size_t)n * H + y) * W + x] = v[i];
             *rsq += isfinite(v[i]) ? (double)v[i] * v[i] : __builtin_inf();
           }
         }
       } else {
         bf16x2 lo, hi;
-        lo[0] = (__bf16)(gelu(v[0]) * xm);
-        lo[1] = (__bf16)(gelu(v[1]) * xm);
-        hi[0] = (__bf16)(gelu(v[2]) * xm);
-        hi[1] = (__bf16)(gelu(v[3]) * xm);
+        lo[0] = (__bf16)gelu(v[0]);
+        lo[1] = (__bf16)gelu(v[1]);
+        hi[0] = (__bf16)gelu(v[2]);
+        hi[1] = (__bf16)gelu(v[3]);
         uint2 w2;
         w2.x = __builtin_bit_cast(unsigned, lo);
         w2.y = __builtin_bit_cast(unsigned, hi);
         *(uint2*)(s_dst + p * FCS + j * 16 + kq) = w2;
+      }
+    }
+  }
+  // activations of columns outside the image are zero (rows wrap): only the
+  // workgroups at the image's left and right edges have such columns, and
+  // clear them once every wave has stored its tiles
+  if constexpr (!LAST) {
+    if (x0 - RING < 0 || x0 + OW - RING > W) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < NPX; i += FNT) {
+        const int qy = i / OW, qx = i - qy * OW, x = x0 + qx - RING;
+        if (x < 0 || x >= W) {
+#pragma unroll
+          for (int c = 0; c < DN_CH; c += 4) *(uint2*)(s_dst + i * FCS + c) = uint2{0u, 0u};
+        }
       }
     }
   }
