@@ -11,6 +11,7 @@
 #pragma clang fp contract(off)
 
 #include <climits>
+#include <limits>
 #include <type_traits>
 
 #include "sp_internal.h"
@@ -83,7 +84,7 @@ __device__ __forceinline__ void move_half(int& ix, int& iy, T& dx, T& dy, T& px,
   // -0.5, d - 0 = d otherwise (all exact, as the reference's branches)
   dx += c1 * px;
   dy += c1 * py;
-  if (sizeof(T) == 4 && g.abs_pos32) {
+  if (!Q && sizeof(T) == 4 && g.abs_pos32) {
     // A wave-uniform branch taken only by the precision study: the empty
     // volatile asm keeps the compiler from computing the rounding on every
     // position update and selecting it away (16 VALU per particle).
@@ -482,6 +483,9 @@ template <bool Q, class T>
 __device__ __forceinline__ void move_and_deposit(int& ix, int& iy, T& dx, T& dy, T& px, T py, T pz, T q,
                                                  const Geo<T>& g, const Sink<T>& k, int& bad) {
   const T psa = gamma_of(px, py, pz, g.csqinv);
+  // a non-finite momentum (or one whose square overflows) gives a non-finite
+  // gamma: one test instead of three on the momenta
+  if (g.checked && !(psa <= std::numeric_limits<T>::max())) bad |= F_FINITE;
   const T inv = rcp_exact(psa);
   const T c1 = g.half_dt * inv;
   move_half<Q>(ix, iy, dx, dy, px, py, c1, g, bad);
@@ -738,8 +742,9 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
   // and, for periodic x, the x ends; the box is whole and both windows are in
   // LDS unclipped.  Then no boundary or assert can act, every stencil lies in
   // the windows, and the particles take a path without those tests (the
-  // arithmetic is the same).
-  bool quiet = use_lds && whole_box && s_box[2] >= 2 && s_box[3] <= g.ny - 1;
+  // arithmetic is the same).  The precision study's absolute-FP32 positions
+  // always take the general path.
+  bool quiet = use_lds && whole_box && s_box[2] >= 2 && s_box[3] <= g.ny - 1 && !g.abs_pos32;
   if constexpr (PUSH && sizeof(T) == 4) quiet = quiet && fw.use;
   {
     int xlo = 2, xhi = g.nx - 1;
@@ -766,7 +771,6 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
         T dx = pdx[i], dy = pdy[i], px = ppx[i], py = ppy[i], pz = ppz[i];
         if (PUSH) {
           boris<Q>(ix, iy, dx, dy, px, py, pz, gs, fw, g, s.alpha, s.beta, bad);
-          if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
           ppy[i] = py;
           ppz[i] = pz;
         }
@@ -784,7 +788,6 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
         T dx = s_dx[p], dy = s_dy[p], px = s_px[p], py = s_py[p], pz = s_pz[p];
         if (PUSH) {
           boris<Q>(ix, iy, dx, dy, px, py, pz, gs, fw, g, s.alpha, s.beta, bad);
-          if (g.checked && !finite3(px, py, pz)) bad |= F_FINITE;
           s_py[p] = py;
           s_pz[p] = pz;
         }
