@@ -205,6 +205,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--size", default=None, help="override grid, e.g. 1024x256 (not the metric workload)")
     ap.add_argument("--sort-interval", type=int, default=20, help="steps between tile sorts of the particles")
+    ap.add_argument("--sort-policy", default="5,50,0.05",
+                    help="adaptive tile-sort schedule min,max,rise (sp_set_sort_policy), or 'fixed' for every "
+                         "--sort-interval steps")
     ap.add_argument("--trace", type=int, default=0, help="print phase times every N steps (diagnostic)")
     ap.add_argument("--autotune", action="store_true", help="shock workload: autotune the launch shapes first")
     ap.add_argument("--branches", type=int, default=4, help="ensemble workload: concurrent branches")
@@ -236,6 +239,11 @@ def main():
     # ix <= delta (src/prtls.rs:163-170), which the rear edge of the slab does
     # in long runs; the bench uses the mirrored-wall left edge (SP_LEFT_REFLECT).
     sim = capi.GpuSim(prec=4, sort_interval=args.sort_interval, left_edge=capi.SP_LEFT_REFLECT, **cfg)
+    policy = None
+    if args.sort_policy != "fixed":
+        lo, hi, rise = args.sort_policy.split(",")
+        policy = (int(lo), int(hi), float(rise))
+        sim.set_sort_policy(*policy)
     for k, (q, m, vth) in enumerate(SPECIES):
         sim.init_species(q, m, vth, seed=0)
     tuning = None
@@ -248,6 +256,7 @@ def main():
     sim.sync()
     sim.set_timing(True)
     sim.timers(reset=True)
+    sorts0 = sim.diag()["sorts"]
     t0 = time.perf_counter()
     if args.trace:  # phase times every `trace` steps (adds a sync per block of steps)
         tm = {"dep": 0.0, "solve": 0.0, "push": 0.0, "particles": 0.0}
@@ -268,6 +277,7 @@ def main():
     el = time.perf_counter() - t0
     if not args.trace:
         tm = sim.timers(reset=True)
+    nsorts = sim.diag()["sorts"] - sorts0
     value = n * args.steps / el
     print(f"bench: {el:.3f} s, {value:.4g} particle-steps/s, phases ms {tm}", file=sys.stderr, flush=True)
 
@@ -306,12 +316,16 @@ def main():
         "config": {"workload": f"plain FP32 unmagnetized shock {cfg['size_x']}x{cfg['size_y']}, 32 ppc/species, "
                                f"delta 16, gamma0 10, mass ratio 16, 4 binomial passes",
                    "particles": n, "grid": [cfg["size_x"], cfg["size_y"]],
-                   "tuning": tuning or sim.tuning()},
+                   "tuning": tuning or sim.tuning(),
+                   "sort_schedule": ({"adaptive": {"min": policy[0], "max": policy[1], "rise": policy[2]}} if policy
+                                     else {"every": args.sort_interval})},
         "phases_ms_per_step": {k: v / args.steps for k, v in tm.items()},
-        "window": (f"{args.steps} steps: {args.steps // args.sort_interval if args.sort_interval else 0} tile sorts "
-                   "(one per sort interval, as in a long run), and one stand-alone Boris push at the end (the "
-                   "last step's push, otherwise fused into the next step's particle pass); the roofline "
-                   "duration is the fused particle pass alone"),
+        "window": (f"{args.steps} steps: {nsorts} tile sorts ("
+                   + (f"adaptive schedule: {policy[0]} to {policy[1]} steps apart, sooner when the share of "
+                      f"particles on the particle pass's general path rose by {policy[2]}" if policy else
+                      f"one per {args.sort_interval} steps") +
+                   "), and one stand-alone Boris push at the end (the last step's push, otherwise fused into "
+                   "the next step's particle pass); the roofline duration is the fused particle pass alone"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_particles<float, true> (fused push + move + deposit, one species per launch)",

@@ -100,6 +100,7 @@ typedef struct sp_diag {
   double gauss_post;
   double gauss_post_max; /* largest gauss_post over every solve so far */
   int64_t steps;         /* loop iterations run so far */
+  int64_t sorts;         /* tile sorts of the particles run so far (sp_prestep and sp_step) */
 } sp_diag;
 
 typedef struct sp_ctx sp_ctx;
@@ -208,6 +209,20 @@ int sp_get_timers(sp_ctx* ctx, double* ms4, int reset);
  * captured again when any buffer changed (species added, a stand-alone filter
  * call).  Results are those of the eager steps. */
 int sp_set_graph(sp_ctx* ctx, int on);
+/* Adaptive tile-sort schedule (no reference counterpart: the reference keeps
+ * its particles in creation order).  The particle pass counts the particles
+ * it takes through its general path (chunks whose cells spread past the LDS
+ * windows, or that lie near a boundary).  With max_interval > 0 a step sorts
+ * when at least min_interval steps have run since the last sort and either
+ * max_interval have, or that count, as a fraction of all particles and read
+ * two steps late, has risen by more than max_slow_rise since the step after
+ * the last sort.  The counts depend on the particle state only, so the
+ * schedule is the same in every run.  max_interval = 0 restores the fixed
+ * schedule (every sort_interval steps, which must be > 0 either way).  The
+ * fixed schedule applies while the step-loop graph is on and inside an
+ * ensemble; after sp_checkpoint_load the adaptive count starts at the loaded
+ * step. */
+int sp_set_sort_policy(sp_ctx* ctx, int min_interval, int max_interval, double max_slow_rise);
 /* FP64 diagnostics (no reference counterpart): energies, momentum, Gauss
  * residuals. */
 int sp_get_diag(sp_ctx* ctx, sp_diag* out);

@@ -25,7 +25,7 @@ SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_spe
            "sp_species_count", "sp_get_species", "sp_get_species_raw", "sp_get_grid", "sp_set_grid", "sp_get_spectrum",
            "sp_set_spectrum", "sp_zero_currents", "sp_move_and_deposit", "sp_deposit_ghosts",
            "sp_fields_update", "sp_boris_push", "sp_filter", "sp_filter_noalloc", "sp_fft2d", "sp_spectral_update",
-           "sp_prestep", "sp_step", "sp_sync", "sp_set_timing", "sp_get_timers", "sp_set_graph",
+           "sp_prestep", "sp_step", "sp_sync", "sp_set_timing", "sp_get_timers", "sp_set_graph", "sp_set_sort_policy",
            "sp_get_diag", "sp_checkpoint_save", "sp_checkpoint_load", "sp_set_tuning", "sp_get_tuning", "sp_autotune", "sp_ensemble_create",
            "sp_ensemble_destroy", "sp_ensemble_last_error", "sp_ensemble_set_graph", "sp_ensemble_step",
            "sp_ensemble_sync", "sp_ensemble_stats", "sp_dn_create", "sp_dn_destroy", "sp_dn_last_error", "sp_dn_set_params",
@@ -47,7 +47,8 @@ class SpParams(ct.Structure):
 class SpDiag(ct.Structure):
     _fields_ = [("energy_e", ct.c_double), ("energy_b", ct.c_double), ("energy_kin", ct.c_double * 8),
                 ("momentum", ct.c_double * 3), ("gauss_pre", ct.c_double), ("gauss_post", ct.c_double),
-                ("gauss_post_max", ct.c_double), ("steps", ct.c_int64)]
+                ("gauss_post_max", ct.c_double), ("steps", ct.c_int64),
+                ("sorts", ct.c_int64)]
 
 
 class SpTuning(ct.Structure):
@@ -96,6 +97,7 @@ def load(path=LIB_PATH):
     for n in ["sp_move_and_deposit", "sp_boris_push", "sp_step", "sp_set_timing", "sp_set_graph"]:
         getattr(L, n).argtypes = [vp, i32]
     L.sp_filter.argtypes = [vp, i32, i32]
+    L.sp_set_sort_policy.argtypes = [vp, i32, i32, d]
     L.sp_filter_noalloc.argtypes = [vp, ct.c_int32, ct.c_int32, i32, i32, vp]
     L.sp_fft2d.argtypes = [vp, i32, i32]
     L.sp_get_timers.argtypes = [vp, vp, i32]
@@ -261,6 +263,10 @@ class GpuSim:
 
     def set_graph(self, on=True): self._chk(load().sp_set_graph(self.h, int(on)))
 
+    def set_sort_policy(self, min_interval=0, max_interval=0, max_slow_rise=0.0):
+        """Adaptive tile-sort schedule (max_interval 0: every sort_interval steps)."""
+        self._chk(load().sp_set_sort_policy(self.h, int(min_interval), int(max_interval), float(max_slow_rise)))
+
     def timers(self, reset=False):
         t = np.zeros(4, np.float64)
         self._chk(load().sp_get_timers(self.h, _p(t), int(reset)))
@@ -275,7 +281,7 @@ class GpuSim:
                 "energy_kin": [d.energy_kin[k] for k in range(ns)],
                 "momentum": [d.momentum[k] for k in range(3)],
                 "gauss_pre": d.gauss_pre, "gauss_post": d.gauss_post, "gauss_post_max": d.gauss_post_max,
-                "steps": d.steps}
+                "steps": d.steps, "sorts": d.sorts}
 
     def set_denoiser(self, params, max_rel=0.5, max_gauss=1.0):
         """Apply a trained denoiser after the binomial filter (params None disables)."""

@@ -63,8 +63,17 @@ struct sp_ctx {
   void* dsp = nullptr;  // per-species density scratch grid (GridSet::dsp)
   void* spec[14] = {};
   FftDim fx{}, fy{};
-  int* flags_d = nullptr;
+  int* flags_d = nullptr;  // [0] invariant flags; [2..3] the slow-path particle counter (SLOW_COUNTER)
   int* flags_h = nullptr;
+  // adaptive tile-sort schedule (sp_set_sort_policy): off while sort_max == 0
+  int sort_min = 0, sort_max = 0;
+  double sort_rise = 0;
+  int64_t last_sort = -1;            // step whose particle pass ran right after the last sort
+  int64_t nsorts = 0;                // tile sorts run (sp_diag.sorts)
+  uint64_t* slow_h = nullptr;        // pinned snapshots of the running slow-path counter, one per step
+  hipEvent_t slow_ev[SLOW_RING] = {};
+  int64_t slow_step[SLOW_RING];     // step of each snapshot (-2: none)
+  double slow_f0 = -1;               // slow fraction of step last_sort, once known
   int* tile_count = nullptr;   // per-tile particle counts of the tile sort
   int* tile_cursor = nullptr;  // per-tile write positions
   double* gauss_d = nullptr;   // Gauss-law sums of the last solve + block partials
@@ -409,6 +418,59 @@ template <class T> void sort_species(sp_ctx* c) {
 
 bool sort_due(const sp_ctx* c) { return c->p.sort_interval > 0 && c->steps % c->p.sort_interval == 0; }
 
+// The adaptive schedule applies to a context stepped on its own, without the
+// step-loop graph (captured periods and ensembles replay a fixed sort phase).
+bool adaptive_sort(const sp_ctx* c) { return c->sort_max > 0 && !c->ens && !c->graph_on; }
+
+// Snapshot of the slow-path counter after step `step`'s particle passes
+// (stream-ordered copy into pinned memory; read back two steps later).
+void slow_snapshot(sp_ctx* c, int64_t step) {
+  const int k = (int)(((step % SLOW_RING) + SLOW_RING) % SLOW_RING);
+  hipMemcpyAsync(&c->slow_h[k], c->flags_d + SLOW_COUNTER, sizeof(uint64_t), hipMemcpyDeviceToHost, c->st);
+  hipEventRecord(c->slow_ev[k], c->st);
+  c->slow_step[k] = step;
+}
+
+// Particles of step `step` that took the general path, as a fraction of all
+// particles; -1 when the snapshots of `step` and `step - 1` are gone.
+double slow_fraction(sp_ctx* c, int64_t step) {
+  const int k1 = (int)(((step % SLOW_RING) + SLOW_RING) % SLOW_RING);
+  const int k0 = (int)((((step - 1) % SLOW_RING) + SLOW_RING) % SLOW_RING);
+  if (c->slow_step[k1] != step || c->slow_step[k0] != step - 1) return -1;
+  hipEventSynchronize(c->slow_ev[k1]);
+  int64_t n = 0;
+  for (const auto& sp : c->species) n += sp.n;
+  if (n <= 0) return 0;
+  return (double)(c->slow_h[k1] - c->slow_h[k0]) / (double)n;
+}
+
+// Whether step c->steps sorts first.  Fixed schedule: every sort_interval
+// steps.  Adaptive: at least sort_min and at most sort_max steps after the
+// last sort, and in between when the fraction of particles the particle pass
+// took through its general path (chunks whose cells spread beyond the LDS
+// windows, or near a boundary) has risen by more than sort_rise since the pass
+// right after the last sort.  The fraction is read two steps late so that the
+// host does not wait for the step it is about to issue; it is a function of
+// the particle state only, so the schedule is the same in every run.
+bool sort_now(sp_ctx* c) {
+  if (!adaptive_sort(c) || c->p.sort_interval <= 0) return sort_due(c);
+  const int64_t s = c->steps;
+  if (c->last_sort < 0) return true;
+  const int64_t k = s - c->last_sort;
+  if (k < c->sort_min) return false;
+  if (k >= c->sort_max) return true;
+  if (c->slow_f0 < 0) c->slow_f0 = slow_fraction(c, c->last_sort);
+  if (c->slow_f0 < 0 || s - 2 <= c->last_sort) return false;
+  const double f = slow_fraction(c, s - 2);
+  return f >= 0 && f - c->slow_f0 > c->sort_rise;
+}
+
+void note_sort(sp_ctx* c) {
+  c->nsorts += 1;
+  c->last_sort = c->steps;
+  c->slow_f0 = -1;
+}
+
 // One iteration of the loop body (src/lib.rs:155-199), with the push left
 // pending (see the file comment).
 template <class T> void one_step(sp_ctx* c) {
@@ -416,7 +478,10 @@ template <class T> void one_step(sp_ctx* c) {
   const GridSet gs = gset(c);
   {
     PhaseScope ps(c, 0);
-    if (sort_due(c)) sort_species<T>(c);
+    if (sort_now(c)) {
+      sort_species<T>(c);
+      note_sort(c);
+    }
     zero_currents<T>(c);
     {
       PhaseScope pp(c, 3);
@@ -425,6 +490,7 @@ template <class T> void one_step(sp_ctx* c) {
         else launch_move_deposit<T>(sdev<T>(c, s), g, gs, c->flags_d, c->st);
       }
     }
+    if (adaptive_sort(c)) slow_snapshot(c, c->steps);
     c->push_pending = false;
     fold<T>(c);
   }
@@ -876,7 +942,7 @@ int sp_create(const sp_params* params, sp_ctx** out) {
     CK(hipMalloc(&c->tile_count, ntiles * sizeof(int)));
     CK(hipMalloc(&c->tile_cursor, ntiles * sizeof(int)));
   }
-  CK(hipMalloc(&c->flags_d, sizeof(int)));
+  CK(hipMalloc(&c->flags_d, 4 * sizeof(int)));
   {
     const size_t gn = gauss_doubles(p.size_x, p.size_y);
     CK(hipMalloc(&c->gauss_d, gn * sizeof(double)));
@@ -884,9 +950,10 @@ int sp_create(const sp_params* params, sp_ctx** out) {
     CK(hipMalloc(&c->diag_d, DIAG_DOUBLES * sizeof(double)));
   }
   CK(hipHostMalloc((void**)&c->flags_h, sizeof(int), hipHostMallocDefault));
-  CK(hipMemsetAsync(c->flags_d, 0, sizeof(int), c->st));
+  CK(hipMemsetAsync(c->flags_d, 0, 4 * sizeof(int), c->st));
   CK(hipStreamSynchronize(c->st));
   *c->flags_h = 0;
+  std::fill(c->slow_step, c->slow_step + SLOW_RING, (int64_t)-2);
 #undef CK
   *out = c;
   return SP_OK;
@@ -913,6 +980,9 @@ int sp_destroy(sp_ctx* c) {
   hipFree(c->gauss_d);
   hipFree(c->diag_d);
   if (c->flags_h) hipHostFree(c->flags_h);
+  if (c->slow_h) hipHostFree(c->slow_h);
+  for (auto e : c->slow_ev)
+    if (e) hipEventDestroy(e);
   for (auto& tp : c->open_pairs) { hipEventDestroy(tp.a); hipEventDestroy(tp.b); }
   for (auto e : c->free_events) hipEventDestroy(e);
   if (c->own_st) hipStreamDestroy(c->own_st);
@@ -1242,11 +1312,13 @@ int sp_prestep(sp_ctx* c) {
   if (c->p.sort_interval > 0) {
     if (c->f64) sort_species<double>(c);
     else sort_species<float>(c);
+    note_sort(c);
   }
   for (auto& s : c->species) {
     if (c->f64) launch_move_deposit<double>(sdev<double>(c, s), geo<double>(c), gset(c), c->flags_d, c->st);
     else launch_move_deposit<float>(sdev<float>(c, s), geo<float>(c), gset(c), c->flags_d, c->st);
   }
+  if (adaptive_sort(c)) slow_snapshot(c, c->steps - 1);
   if (c->f64) fold<double>(c);
   else fold<float>(c);
   HIPCHK(c, hipGetLastError());
@@ -1258,7 +1330,7 @@ int sp_step(sp_ctx* c, int n) {
   if (n < 0) return fail(c, SP_EINVAL, "negative step count");
   for (int i = 0; i < n;) {
     const int P = graph_period(c);
-    if (c->graph_on && !c->timing && c->push_pending && c->steps % P == 0 && n - i >= P) {
+    if (c->graph_on && !c->timing && c->push_pending && c->steps % P == 0 && n - i >= P) {  // fixed sort phase
       const int rc = c->f64 ? graph_period_run<double>(c) : graph_period_run<float>(c);
       if (rc) return rc;
       i += P;
@@ -1338,6 +1410,27 @@ int sp_set_graph(sp_ctx* c, int on) {
   return SP_OK;
 }
 
+int sp_set_sort_policy(sp_ctx* c, int min_interval, int max_interval, double max_slow_rise) {
+  if (!c) return SP_EINVAL;
+  if (max_interval == 0) {
+    c->sort_max = 0;
+    return SP_OK;
+  }
+  if (c->p.sort_interval <= 0) return fail(c, SP_EINVAL, "the adaptive sort schedule needs sort_interval > 0");
+  if (min_interval < 1 || max_interval < min_interval || !(max_slow_rise >= 0) || !(max_slow_rise <= 1))
+    return fail(c, SP_EINVAL, "need 1 <= min_interval <= max_interval and 0 <= max_slow_rise <= 1");
+  if (!c->slow_h) {
+    HIPCHK(c, hipHostMalloc((void**)&c->slow_h, SLOW_RING * sizeof(uint64_t), hipHostMallocDefault));
+    for (auto& e : c->slow_ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  c->sort_min = min_interval;
+  c->sort_max = max_interval;
+  c->sort_rise = max_slow_rise;
+  std::fill(c->slow_step, c->slow_step + SLOW_RING, (int64_t)-2);
+  c->slow_f0 = -1;
+  return SP_OK;
+}
+
 int sp_sync(sp_ctx* c) {
   if (!c) return SP_EINVAL;
   int rc = flush(c);
@@ -1392,6 +1485,7 @@ int sp_get_diag(sp_ctx* c, sp_diag* out) {
   out->gauss_post = rho2 > 0 ? std::sqrt(g[1] / rho2) : std::sqrt(g[1]);
   out->gauss_post_max = g[3];
   out->steps = c->steps;
+  out->sorts = c->nsorts;
   return SP_OK;
 }
 
@@ -1563,6 +1657,11 @@ int sp_checkpoint_load(sp_ctx* c, const char* path) {
   apply_tuning(c, h.tuning);
   c->steps = h.steps;
   c->push_pending = false;
+  // the adaptive sort schedule counts from here (the file's order is the
+  // device order it was saved in); a fixed schedule follows from the step
+  c->last_sort = c->steps;
+  c->slow_f0 = -1;
+  std::fill(c->slow_step, c->slow_step + SLOW_RING, (int64_t)-2);
   drop_graph(c);
   HIPCHK(c, hipMemcpy(c->gauss_d + 3, &h.gauss_post_max, sizeof(double), hipMemcpyHostToDevice));
   return SP_OK;

@@ -755,6 +755,10 @@ k_particles(SpeciesDev<T> s, Geo<T> g, GridSet gs, int* flags) {
     }
     quiet = quiet && s_box[0] >= xlo && s_box[1] <= xhi;
   }
+  // slow-path particles, for the adaptive sort schedule (one atomic per
+  // chunk that is not quiet)
+  if (tid == 0 && !quiet && cnt > 0)
+    atomicAdd((unsigned long long*)(flags + SLOW_COUNTER), (unsigned long long)cnt);
   const auto particle = [&](auto q, int j) {
     constexpr bool Q = decltype(q)::value;
     if constexpr (DIRECT) {

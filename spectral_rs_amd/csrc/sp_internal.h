@@ -20,8 +20,15 @@ enum Flags : int {
   F_CELL = 1,   // ix or iy outside the interior (src/prtls.rs:100-101, src/lib.rs:324-325)
   F_WALL = 2,   // max ix > wall + 1 (src/prtls.rs:155-162)
   F_LEFT = 4,   // min ix <= delta (src/prtls.rs:163-170)
-  F_FINITE = 8  // non-finite momentum (no reference assert; reported as invariant)
+  F_FINITE = 8  // non-finite momentum or gamma (no reference assert; reported as invariant)
 };
+
+// The flags buffer of a context holds 4 ints: [0] the flags above, [2..3] a
+// running 64-bit count of the particles the particle pass took through its
+// general path (chunks that were not quiet), which drives the adaptive
+// tile-sort schedule; SLOW_RING snapshots of it are kept on the host.
+constexpr int SLOW_COUNTER = 2;
+constexpr int SLOW_RING = 8;
 
 template <class T> struct Geo {
   int nx, ny, pitch;       // pitch = nx + 2

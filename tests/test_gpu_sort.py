@@ -89,3 +89,48 @@ def test_sorted_and_unsorted_fp64_runs_agree():
     # the unsorted run kept the order of addition, the sorted one did not
     assert np.array_equal(a.species_raw(1)["tag"], np.arange(a.count(1)))
     assert not np.array_equal(b.species_raw(1)["tag"], np.arange(b.count(1)))
+
+
+def test_adaptive_sort_schedule_changes_only_the_order():
+    """sp_set_sort_policy: a schedule that sorts at its longest interval (3
+    steps; the rise of the slow-path fraction can never exceed 1) runs exactly
+    the sorts it states, and neither it nor one driven by any rise of the
+    slow-path fraction changes the arithmetic (FP64: cells bit for bit, the
+    rest to the deposit's summation order)."""
+    kw = dict(size_x=256, size_y=32, delta=8, dens=8, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0, left_edge=1)
+    runs = {}
+    for mode in ("fixed", "every3", "rise"):
+        g = capi.GpuSim(prec=8, sort_interval=20, **kw)
+        if mode == "every3":
+            g.set_sort_policy(1, 3, 1.0)
+        elif mode == "rise":
+            g.set_sort_policy(1, 100, 0.0)
+        for q, m, vth, sd in [(1.0, 16.0, 0.0025, 3), (-1.0, 1.0, 0.01, 4)]:
+            g.add_species(q, m, **shock_species(256, 32, 8, 8, 10.0, 10.0, vth, seed=sd, ft=np.float64))
+        g.prestep()
+        g.step(10)
+        runs[mode] = g
+    # fixed: the pre-step and step 0; every3: the pre-step and steps 3, 6, 9
+    assert runs["fixed"].diag()["sorts"] == 2
+    assert runs["every3"].diag()["sorts"] == 4
+    assert runs["rise"].diag()["sorts"] >= 1
+    a = runs["fixed"]
+    for b in (runs["every3"], runs["rise"]):
+        for sid in range(2):
+            sa, sb = a.species(sid), b.species(sid)
+            for k in ("ix", "iy"):
+                assert np.array_equal(sa[k], sb[k]), (sid, k)
+            for k in ("dx", "dy", "px", "py", "pz"):
+                assert np.max(np.abs(sa[k] - sb[k])) <= 1e-11 * max(np.max(np.abs(sa[k])), 1e-30), (sid, k)
+        for name in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz", "rho", "dens"):
+            x, y = a.grid(name), b.grid(name)
+            assert np.max(np.abs(x - y)) <= 1e-11 * max(np.max(np.abs(x)), 1e-30), name
+    # the arguments are validated; a context that never sorts has no schedule to adapt
+    g = runs["fixed"]
+    for bad in [(0, 3, 0.1), (4, 3, 0.1), (1, 3, -0.1), (1, 3, 1.5)]:
+        with pytest.raises(capi.SpError):
+            g.set_sort_policy(*bad)
+    g.set_sort_policy(0, 0, 0.0)  # back to the fixed schedule
+    u = capi.GpuSim(prec=8, sort_interval=0, **kw)
+    with pytest.raises(capi.SpError):
+        u.set_sort_policy(1, 3, 0.1)
