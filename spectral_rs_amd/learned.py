@@ -42,10 +42,18 @@ def shock_kw(size_x, size_y, ppc):
     return dict(SHOCK, size_x=size_x, size_y=size_y, delta=delta_for(size_x), dens=ppc)
 
 
-def shock_sim(size_x, size_y, ppc, seed=0, sort_interval=20, gamma=10.0):
+# the tile-sort schedule of every run here (the bench's): at least 5 and at
+# most 50 steps apart, sooner when the particles spread out of the particle
+# pass's windows
+SORT_POLICY = (5, 50, 0.05)
+
+
+def shock_sim(size_x, size_y, ppc, seed=0, sort_interval=20, gamma=10.0, sort_policy=SORT_POLICY):
     kw = shock_kw(size_x, size_y, ppc)
     kw["gamma_inj"] = gamma
     sim = capi.GpuSim(prec=4, sort_interval=sort_interval, left_edge=capi.SP_LEFT_REFLECT, **kw)
+    if sort_policy:
+        sim.set_sort_policy(*sort_policy)
     for q, m, vth in SPECIES:
         sim.init_species(q, m, vth, seed=seed)
     return sim
