@@ -344,6 +344,14 @@ int sp_set_denoiser_input(sp_ctx* ctx, int raw, float max_hp);
 int sp_set_denoiser_channels(sp_ctx* ctx, int mask);
 /* Solves that applied the network / fell back since the denoiser was set. */
 int sp_get_denoiser_stats(sp_ctx* ctx, int* applied, int* fallbacks);
+/* Backoff (no reference counterpart): after a solve whose checks failed, the
+ * next `solves` solves keep the binomial filter's result without running the
+ * network (they count as fallbacks); 0 (the default) runs it in every solve.
+ * The checks' verdict holds over long stretches of a run, so this saves the
+ * inference that would only fall back again. */
+int sp_set_denoiser_backoff(sp_ctx* ctx, int solves);
+/* Solves skipped by the backoff since the denoiser was set. */
+int sp_get_denoiser_skipped(sp_ctx* ctx, int* skipped);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,7 @@ SYMBOLS = ["sp_create", "sp_destroy", "sp_last_error", "sp_species_add", "sp_spe
            "sp_ensemble_sync", "sp_ensemble_stats", "sp_dn_create", "sp_dn_destroy", "sp_dn_last_error", "sp_dn_set_params",
            "sp_dn_get_params", "sp_dn_set_loss_weights", "sp_dn_forward", "sp_dn_gradient", "sp_dn_train",
            "sp_set_denoiser", "sp_set_denoiser_input", "sp_set_denoiser_channels",
-           "sp_get_denoiser_stats"]
+           "sp_get_denoiser_stats", "sp_set_denoiser_backoff", "sp_get_denoiser_skipped"]
 DN_NTRAIN, DN_NPARAMS = 30084, 30088
 DN_LAYERS, DN_CH = 5, 32  # the denoiser: 5 convolutions, 32 hidden channels
 
@@ -116,6 +116,8 @@ def load(path=LIB_PATH):
     L.sp_get_denoiser_stats.argtypes = [vp, ct.POINTER(i32), ct.POINTER(i32)]
     L.sp_set_denoiser_channels.argtypes = [vp, i32]
     L.sp_set_denoiser_input.argtypes = [vp, i32, ct.c_float]
+    L.sp_set_denoiser_backoff.argtypes = [vp, i32]
+    L.sp_get_denoiser_skipped.argtypes = [vp, ct.POINTER(i32)]
     L.sp_set_tuning.argtypes = [vp, ct.POINTER(SpTuning)]
     L.sp_get_tuning.argtypes = [vp, ct.POINTER(SpTuning)]
     L.sp_autotune.argtypes = [vp, i32, i32, ct.POINTER(SpTuning)]
@@ -305,6 +307,15 @@ class GpuSim:
         a, f = ct.c_int32(), ct.c_int32()
         self._chk(load().sp_get_denoiser_stats(self.h, ct.byref(a), ct.byref(f)))
         return a.value, f.value
+
+    def set_denoiser_backoff(self, solves):
+        """Solves that keep the filter's result without inference after a failed check."""
+        self._chk(load().sp_set_denoiser_backoff(self.h, int(solves)))
+
+    def denoiser_skipped(self):
+        k = ct.c_int32()
+        self._chk(load().sp_get_denoiser_skipped(self.h, ct.byref(k)))
+        return k.value
 
     # ---- launch shapes ----
     def tuning(self):

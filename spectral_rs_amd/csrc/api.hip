@@ -84,6 +84,7 @@ struct sp_ctx {
   int* dn_stats = nullptr;
   float dn_tol = 0, dn_gtol = 0;
   int dn_mask = 0xF;  // channels the correction is written to (bit c: Jx Jy Jz rho)
+  int dn_backoff = 0;  // solves skipped after a failed check (sp_set_denoiser_backoff)
   bool dn_raw = false;  // the network reads the unfiltered deposit (it replaces the filter)
   float dn_hp = 0;      // spectrum check: high-pass energy bound relative to the filter's (0 = off)
   // step-loop graph (sp_set_graph): the captured period and the buffers it uses
@@ -283,7 +284,7 @@ template <class T> void fields_update(sp_ctx* c) {
       float* w4[4] = {(float*)c->work[0], (float*)c->work[1], (float*)c->work[2], (float*)c->work[3]};
       const bool raw = c->dn_raw && c->p.n_pass > 0;
       dn_apply_sim(dn_params(c->dn_buf), dn_tensors(c), raw ? w4 : g4, g4, nx, ny, c->p.dens, c->dn_sums, c->gauss_d,
-                   c->dn_tol, c->dn_gtol, c->dn_hp, c->dn_mask, c->dn_stats, c->st);
+                   c->dn_tol, c->dn_gtol, c->dn_hp, c->dn_mask, c->dn_backoff, c->dn_stats, c->st);
       launch_update_ghosts<float>(g4, 4, nx, ny, c->st);
     }
   }
@@ -1359,10 +1360,10 @@ int sp_set_denoiser(sp_ctx* c, const float* params, int n, float max_rel, float 
     const size_t planes = DN_IN * c->nspec_elems();
     HIPCHK(c, hipMalloc(&c->dn_buf, (DN_NPARAMS + DN_PACK_SHORTS / 2 + planes) * sizeof(float)));
     HIPCHK(c, hipMalloc(&c->dn_sums, DN_SUMS * sizeof(double)));
-    HIPCHK(c, hipMalloc(&c->dn_stats, 2 * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->dn_stats, DN_STATS * sizeof(int)));
   }
   HIPCHK(c, hipMemcpyAsync(c->dn_buf, params, DN_NPARAMS * sizeof(float), hipMemcpyHostToDevice, c->st));
-  fill<int>(c->dn_stats, 0, 2, c->st);
+  fill<int>(c->dn_stats, 0, DN_STATS, c->st);
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->dn_tol = max_rel;
   c->dn_gtol = max_gauss;
@@ -1399,6 +1400,26 @@ int sp_get_denoiser_stats(sp_ctx* c, int* applied, int* fallbacks) {
   }
   *applied = h[0];
   *fallbacks = h[1];
+  return SP_OK;
+}
+
+int sp_set_denoiser_backoff(sp_ctx* c, int solves) {
+  if (!c) return SP_EINVAL;
+  if (solves < 0) return fail(c, SP_EINVAL, "the backoff must be >= 0 solves");
+  c->dn_backoff = solves;
+  c->tune_gen++;  // captured steps hold it as a kernel argument
+  drop_graph(c);
+  return SP_OK;
+}
+
+int sp_get_denoiser_skipped(sp_ctx* c, int* skipped) {
+  if (!c || !skipped) return SP_EINVAL;
+  int h[DN_STATS] = {};
+  if (c->dn_stats) {
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(h, c->dn_stats, sizeof(h), hipMemcpyDeviceToHost));
+  }
+  *skipped = h[DN_STAT_SKIPPED];
   return SP_OK;
 }
 

@@ -325,6 +325,7 @@ __global__ void k_dn_apply(Grids4 in, Grids4 out, const float* __restrict__ scal
                            const float* __restrict__ r, int nx, int ny, const double* __restrict__ sums,
                            const double* __restrict__ gauss, float tol, float gauss_tol, float hp_tol, int chmask,
                            int* stats) {
+  if (stats[DN_STAT_SKIP] > 0) return;  // no inference ran this solve (k_dn_backoff counts it)
   const double ratio2 = sums[0] > 0 ? sums[1] / sums[0] : 0.0;
   bool ok = ratio2 <= (double)tol * tol;
   if (gauss_tol > 0) {
@@ -336,7 +337,10 @@ __global__ void k_dn_apply(Grids4 in, Grids4 out, const float* __restrict__ scal
     for (int c = 0; c < 4; ++c)
       if ((chmask >> c) & 1) ok = ok && sums[2 + 2 * c] <= (double)hp_tol * sums[3 + 2 * c];
   }
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) atomicAdd(stats + (ok ? 0 : 1), 1);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    atomicAdd(stats + (ok ? DN_STAT_APPLIED : DN_STAT_FALLBACK), 1);
+    stats[DN_STAT_LAST] = ok ? 0 : 1;
+  }
   if (!ok) return;
   const int xx = blockIdx.x * blockDim.x + threadIdx.x;
   if (xx >= nx) return;
@@ -354,8 +358,10 @@ __global__ void k_dn_apply(Grids4 in, Grids4 out, const float* __restrict__ scal
 // periodic neighbours (its symbol 4 sin^2(kx/2) + 4 sin^2(ky/2) weights the
 // highest wavenumbers most).  FP64 sums, one atomic per block and quantity.
 __global__ void __launch_bounds__(256) k_dn_hp(Grids4 in, Grids4 out, const float* __restrict__ scale, float dens,
-                                               const float* __restrict__ r, int nx, int ny, double* sums) {
+                                               const float* __restrict__ r, int nx, int ny, double* sums,
+                                               const int* __restrict__ stats) {
   __shared__ double red[2][4];
+  if (stats[DN_STAT_SKIP] > 0) return;
   const int c = blockIdx.z;
   const int xx = blockIdx.x * blockDim.x + threadIdx.x;
   double hy = 0.0, hf = 0.0;
@@ -609,6 +615,7 @@ struct FusedIn {
   const float* scale;
   float inv_dens;
   double* sums;
+  const int* skip;  // in the simulation: solves left to skip (DnStat::SKIP), else NULL
 };
 
 // FP64 sum over the workgroup (FNT threads), added to *out by one atomic.
@@ -637,6 +644,7 @@ __global__ void __launch_bounds__(FNT) k_dn_fused(FusedIn in, DnParams P, const 
   __shared__ __attribute__((aligned(16))) short s_b[NB * FCS];
   __shared__ __attribute__((aligned(16))) short s_w[DN_CH * (DN_CH * 9 + 4)];
   __shared__ double s_red[FNT / 64];
+  if (GRIDS && in.skip && *in.skip > 0) return;  // backing off after a failed check: the filter's result stands
   const int y0 = blockIdx.y * FY, x0 = blockIdx.x * FX;
   // The input region (4 channels, pixel-major) in two passes: every load of
   // the thread first (with layer 0's weights in flight beside them), then the
@@ -768,9 +776,25 @@ void dn_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, in
   hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, p, g, m1, m2, n, lr, t);
 }
 
+// After a solve: a skipped solve counts as a fallback and shortens the
+// backoff; a failed check starts one of `backoff` solves (0: never skip).
+// The check's verdict holds over long stretches of a run (at 4096 x 1024, 4
+// ppc: about 150-200 consecutive solves each way), so the solves after a
+// failure rarely need the inference that would only fall back again.
+__global__ void k_dn_backoff(int* stats, int backoff) {
+  if (threadIdx.x != 0) return;
+  if (stats[DN_STAT_SKIP] > 0) {
+    stats[DN_STAT_SKIP] -= 1;
+    stats[DN_STAT_FALLBACK] += 1;
+    stats[DN_STAT_SKIPPED] += 1;
+  } else if (stats[DN_STAT_LAST] && backoff > 0) {
+    stats[DN_STAT_SKIP] = backoff;
+  }
+}
+
 void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* inputs, float* const* grids, int nx, int ny,
                   int dens, double* sums, const double* gauss, float tol, float gauss_tol, float hp_tol, int chmask,
-                  int* stats, hipStream_t st) {
+                  int backoff, int* stats, hipStream_t st) {
   FusedIn in{};
   Grids4 out{};
   for (int c = 0; c < 4; ++c) {
@@ -780,6 +804,7 @@ void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* inputs, f
   in.scale = P.scale;
   in.inv_dens = 1.0f / (float)dens;
   in.sums = sums;
+  in.skip = stats + DN_STAT_SKIP;
   fill<double>(sums, 0.0, DN_SUMS, st);
   hipLaunchKernelGGL(k_dn_pack_weights, dim3((DN_PACK_SHORTS + 255) / 256), dim3(256), 0, st, P, T.packed);
   dim3 grid((nx + FX - 1) / FX, (ny + FY - 1) / FY);
@@ -788,9 +813,10 @@ void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* inputs, f
   // (one block per row made them serialise: 0.35 ms at 4096 x 1024)
   if (hp_tol > 0)
     hipLaunchKernelGGL(k_dn_hp, dim3((nx + 255) / 256, (unsigned)std::min(ny, 16), 4), dim3(256), 0, st, in.gr, out,
-                       P.scale, (float)dens, T.r, nx, ny, sums);
+                       P.scale, (float)dens, T.r, nx, ny, sums, stats);
   hipLaunchKernelGGL(k_dn_apply, dim3((nx + 255) / 256, (unsigned)std::min(4 * ny, 4096)), dim3(256), 0, st, in.gr,
                      out, P.scale, (float)dens, T.r, nx, ny, sums, gauss, tol, gauss_tol, hp_tol, chmask, stats);
+  hipLaunchKernelGGL(k_dn_backoff, dim3(1), dim3(64), 0, st, stats, backoff);
 }
 
 }  // namespace sp

@@ -989,6 +989,18 @@ __device__ __forceinline__ void table_add(lds_ptr<int> keys, lds_ptr<int> cnt, i
   }
 }
 
+// x mod n for 0 <= x < 2^23 and 1 <= n <= 2^12, through FP32: x is exact as
+// a float, and the quotient from the rounded reciprocal is off by at most one
+// (|x rn - x / n| < x / n 2^-22 < 1), which the two corrections remove.  About
+// a tenth of the instructions of the integer division it replaces.
+__device__ __forceinline__ int mod_small(int x, int n) {
+  const float rn = __builtin_amdgcn_rcpf((float)n);
+  int r = x - (int)((float)x * rn) * n;
+  r += r < 0 ? n : 0;
+  r -= r >= n ? n : 0;
+  return r;
+}
+
 // Inverse of a modulo n (gcd(a, n) = 1), extended Euclid.
 __device__ __forceinline__ int inv_mod(int a, int n) {
   int t = 0, nt = 1, r = n, nr = a % n;
@@ -1131,7 +1143,7 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
     const int64_t i = base + threadIdx.x + SORT_NT * j;
     if (i >= a.n) continue;
     const int nu = s_n[slot[j]], n = nu & 0xFFF, inv = (nu >> 12) & 0x7FF;
-    const int r = n > 1 ? (rank[j] * inv) % n : 0;
+    const int r = n > 1 ? mod_small(rank[j] * inv, n) : 0;
     const int loc = keys[slot[j]] + r;
     s_gpos[loc] = cnt[slot[j]] + r;
     rank[j] = loc;  // from here on: the local index

@@ -210,11 +210,17 @@ void dn_forward_fused(const DnParams& P, const DnTensors& T, int H, int W, hipSt
 // replaces the filter) and, when the checks pass, writes input + dens s_c r
 // into the filtered grids for the channels set in chmask (bit c = grid c of
 // Jx Jy Jz rho); otherwise the filtered grids stay.  T.packed and T.r are its
-// buffers; sums holds DN_SUMS doubles; stats[0] counts applications,
-// stats[1] fallbacks.  gauss_tol <= 0 and hp_tol <= 0 switch those checks off.
+// buffers; sums holds DN_SUMS doubles; stats (DN_STATS ints, below) counts
+// applications and fallbacks.  gauss_tol <= 0 and hp_tol <= 0 switch those
+// checks off; after a fallback the next `backoff` solves skip the network (0:
+// none skips).
 constexpr int DN_SUMS = 10;  // |x|^2, |r|^2, then per channel the high-pass energies of output and filter
+// device counters of the in-loop denoiser: solves that applied the network,
+// that kept the filter's result (skipped ones included), the solves still to
+// skip, the verdict of the last check (1 = failed), and the skipped solves
+enum DnStat { DN_STAT_APPLIED = 0, DN_STAT_FALLBACK, DN_STAT_SKIP, DN_STAT_LAST, DN_STAT_SKIPPED, DN_STATS };
 void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* inputs, float* const* grids, int nx, int ny,
                   int dens, double* sums, const double* gauss, float tol, float gauss_tol, float hp_tol, int chmask,
-                  int* stats, hipStream_t st);
+                  int backoff, int* stats, hipStream_t st);
 
 }  // namespace sp

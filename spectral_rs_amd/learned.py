@@ -189,7 +189,7 @@ def evaluate(p, X, XF, T):
     return {"channels": CH, "binomial_rel_mse": list(base / n / ms), "learned_rel_mse": list(learned / n / ms)}
 
 
-def run_case(size_x, size_y, ppc, steps, params=None, seed=0, max_rel=1.0, mask=0xF, raw=True, max_hp=0.0):
+def run_case(size_x, size_y, ppc, steps, params=None, seed=0, max_rel=1.0, mask=0xF, raw=True, max_hp=0.0, backoff=0):
     sim = shock_sim(size_x, size_y, ppc, seed)
     if params is not None:
         # the network may remove most of a channel that is almost pure noise
@@ -198,6 +198,7 @@ def run_case(size_x, size_y, ppc, steps, params=None, seed=0, max_rel=1.0, mask=
         sim.set_denoiser(params, max_rel=max_rel, max_gauss=0.0)
         sim.set_denoiser_input(raw, max_hp)
         sim.set_denoiser_channels(mask)
+        sim.set_denoiser_backoff(backoff)
     sim.prestep()
     sim.step(2)
     sim.sync()
@@ -207,7 +208,7 @@ def run_case(size_x, size_y, ppc, steps, params=None, seed=0, max_rel=1.0, mask=
     el = time.perf_counter() - t0
     prof = profile(sim, ppc)
     spec = {n: xspectrum(sim, n) for n in ("ex", "bz")}
-    stats = sim.denoiser_stats() if params is not None else None
+    stats = (*sim.denoiser_stats(), sim.denoiser_skipped()) if params is not None else None
     sim.close()
     return el, prof, stats, spec
 
@@ -281,6 +282,9 @@ def main():
                     help="cases of the time-to-solution study (plain_<n>ppc, learned)")
     ap.add_argument("--dn-masks", default="7",
                     help="channel masks of the learned cases (bit c: Jx Jy Jz rho written back)")
+    ap.add_argument("--dn-backoffs", default="0",
+                    help="backoffs of the learned cases: solves that keep the filter's result without inference after "
+                         "a failed check")
     ap.add_argument("--out", default=None, help="JSON report")
     ap.add_argument("--params-out", default=None, help=".npy of the trained parameters")
     ap.add_argument("--params-in", default=None, help="skip training: evaluate these parameters (.npy)")
@@ -317,17 +321,19 @@ def main():
     d = delta_for(ex)
     eseeds = [int(v) for v in a.eval_seeds.split(",")]
     masks = [int(v) for v in a.dn_masks.split(",")]
+    backoffs = [int(v) for v in a.dn_backoffs.split(",")]
     want = [c for c in a.cases.split(",") if c]
     cases = {}
     for c in want:
         if c.startswith("plain_"):
-            cases[c] = (int(c[len("plain_"):-3]), None, 0)
+            cases[c] = (int(c[len("plain_"):-3]), None, 0, 0)
     if "learned" in want:
         for mk in masks:
-            cases[f"learned_4ppc_mask{mk}"] = (4, p, mk)
+            for bo in backoffs:
+                cases[f"learned_4ppc_mask{mk}" + (f"_backoff{bo}" if bo else "")] = (4, p, mk, bo)
     res = {}
-    for name, (ppc, prm, mk) in cases.items():
-        runs = [run_case(ex, ey, ppc, a.steps, params=prm, seed=sd, mask=mk, raw=raw, max_hp=a.max_hp)
+    for name, (ppc, prm, mk, bo) in cases.items():
+        runs = [run_case(ex, ey, ppc, a.steps, params=prm, seed=sd, mask=mk, raw=raw, max_hp=a.max_hp, backoff=bo)
                 for sd in eseeds]
         res[name] = dict(t=float(np.mean([r[0] for r in runs])), prof=np.mean([r[1] for r in runs], axis=0),
                          stats=[r[2] for r in runs],

@@ -106,7 +106,7 @@ def test_training_reduces_loss_on_a_denoising_task():
     dn.close()
 
 
-def shock(denoise=None, max_rel=0.5, raw=False, max_hp=0.0, mask=0xF, max_gauss=1.0):
+def shock(denoise=None, max_rel=0.5, raw=False, max_hp=0.0, mask=0xF, max_gauss=1.0, backoff=0, steps=10):
     kw = dict(size_x=128, size_y=32, delta=8, dens=4, n_pass=4, dt=0.045, c=10.0, gamma_inj=10.0, left_edge=1)
     g = capi.GpuSim(prec=4, sort_interval=5, **kw)
     for q, m, vth, sd in [(1.0, 16.0, 0.0025, 3), (-1.0, 1.0, 0.01, 4)]:
@@ -115,8 +115,10 @@ def shock(denoise=None, max_rel=0.5, raw=False, max_hp=0.0, mask=0xF, max_gauss=
         g.set_denoiser(denoise, max_rel=max_rel, max_gauss=max_gauss)
         g.set_denoiser_input(raw, max_hp)
         g.set_denoiser_channels(mask)
+        if backoff:
+            g.set_denoiser_backoff(backoff)
     g.prestep()
-    g.step(10)
+    g.step(steps)
     g.sync()
     return g
 
@@ -143,6 +145,33 @@ def test_in_loop_identity_network_and_fallback():
     for name, f in floor.items():
         assert rel(fb.grid(name), plain.grid(name)) <= 3 * f + 1e-6, (name, f)
     for s in (plain, plain2, ident, fb):
+        s.close()
+
+
+def test_backoff_after_a_failed_check():
+    # After a failed check the next 3 solves keep the filter's result without
+    # inference: of 12 solves with a network that always fails, solves 0, 4, 8
+    # run it and fall back, the other 9 are skipped (fallbacks as well), and
+    # the run is the plain one either way.  A network that always passes is
+    # never skipped.
+    plain, plain2 = shock(steps=12), shock(steps=12)
+    floor = {n: rel(plain2.grid(n), plain.grid(n)) for n in ["jx", "rho", "ex", "bz"]}
+    p0 = np.zeros(D.NPARAMS, np.float32)
+    p0[-4:] = 1.0
+    pb = p0.copy()
+    pb[D.NTRAIN - 4:D.NTRAIN] = 1e3
+    fb = shock(pb, max_rel=0.5, backoff=3, steps=12)
+    assert fb.denoiser_stats() == (0, 12)
+    assert fb.denoiser_skipped() == 9
+    for name, f in floor.items():
+        assert rel(fb.grid(name), plain.grid(name)) <= 3 * f + 1e-6, (name, f)
+    ok = shock(p0, backoff=3, steps=12)
+    assert ok.denoiser_stats() == (12, 0) and ok.denoiser_skipped() == 0
+    for name, f in floor.items():
+        assert rel(ok.grid(name), plain.grid(name)) <= 3 * f + 1e-6, (name, f)
+    with pytest.raises(capi.SpError):
+        ok.set_denoiser_backoff(-1)
+    for s in (plain, plain2, fb, ok):
         s.close()
 
 
