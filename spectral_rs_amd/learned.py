@@ -282,7 +282,7 @@ def main():
                     help="cases of the time-to-solution study (plain_<n>ppc, learned)")
     ap.add_argument("--dn-masks", default="7",
                     help="channel masks of the learned cases (bit c: Jx Jy Jz rho written back)")
-    ap.add_argument("--dn-backoffs", default="0",
+    ap.add_argument("--dn-backoffs", default="0,16",
                     help="backoffs of the learned cases: solves that keep the filter's result without inference after "
                          "a failed check")
     ap.add_argument("--out", default=None, help="JSON report")
