@@ -940,7 +940,7 @@ int sp_create(const sp_params* params, sp_ctx** out) {
     // room for the smallest sort tile a tuning may pick
     const int tmin = 1 << TILE_LOG_MIN;
     const size_t ntiles = (size_t)((p.size_x + tmin - 1) / tmin) * ((p.size_y + tmin - 1) / tmin);
-    CK(hipMalloc(&c->tile_count, ntiles * sizeof(int)));
+    CK(hipMalloc(&c->tile_count, tile_count_size(ntiles) * sizeof(int)));
     CK(hipMalloc(&c->tile_cursor, ntiles * sizeof(int)));
   }
   CK(hipMalloc(&c->flags_d, 4 * sizeof(int)));

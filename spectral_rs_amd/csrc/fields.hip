@@ -138,90 +138,158 @@ template <class T> void launch_update_ghosts(T* const* grids, int ngrids, int nx
 
 // Field::binomial_filter_2_d (src/flds/field.rs:385-420) applied NP times,
 // each pass followed by the ghost refresh of field.rs:422 -- in one launch.
-// A workgroup owns FTY x FTX output cells and keeps an extended region of
-// (FTY + 2H) x (FTX + 2H) cells, H = NP - 1, in LDS:
-//  * pass 1 of every extended cell reads its 3x3 padded neighbourhood from
-//    global memory, so cells next to the domain edge see the input's ghost
-//    ring as it stands (stale after the deposit fold, as in the reference);
-//    extended cells beyond the edge are the periodic images of interior
-//    cells, which is what the refreshed ghosts of passes 2.. hold;
-//  * passes 2..NP run in LDS on a region that shrinks by one cell a side;
-//  * the last pass writes the interior and, for edge cells, their periodic
-//    copies in the output's ghost ring.
-// Each pass sums its 9 taps in the reference's order, so the result is
+// A workgroup owns FTY x FTX output cells.  Two LDS buffers share one cell
+// frame, (EH + 2) x (EW + 2) with EH = FTY + 2H, EW = FTX + 2H, H = NP - 1:
+// frame cell (r, c) is padded cell (r0 + r, c0 + c) of the input, and pass k
+// leaves its values at frame rows and columns k .. E + 1 - k in buffer k & 1.
+//  * Pass 1 of a workgroup whose frame lies inside the padded grid (all but
+//    the edge workgroups) first copies the frame from global memory into
+//    buffer 0 and evaluates from LDS.  The frame includes the ghost ring where
+//    it touches it, stale after the deposit fold, as the reference reads it.
+//  * Pass 1 of an edge workgroup reads each extended cell's 3x3 padded
+//    neighbourhood from global memory at its periodic image, so cells next to
+//    the domain edge see the input's ghost ring as it stands and cells beyond
+//    the edge are the periodic images of interior cells, which is what the
+//    refreshed ghosts of passes 2.. hold.
+//  * Passes 2..NP run in LDS; the last one writes the interior from registers
+//    and, for edge cells, their periodic copies in the output's ghost ring.
+// A thread evaluates a strip of SR cells of one column: 3 (SR + 2) LDS reads
+// for SR cells instead of 9 per cell, lanes on consecutive columns.  Each
+// cell sums its 9 taps in the reference's order, so the result is
 // bit-identical to NP separate passes.
-constexpr int FTX = 64, FTY = 16, FMAX = 8;
+constexpr int FTX = 64, FMAX = 8, FSR = 4;
+template <class T> constexpr int filter_tile_rows() { return sizeof(T) == 4 ? 32 : 16; }
+
+// The SR cells of frame column c, rows r .. r + SR - 1, from `f` (pitch LW).
+template <class T, int LW>
+__device__ __forceinline__ void binomial_strip(const T* f, int r, int c, T (&o)[FSR]) {
+  const T q = T(0.0625), e = T(0.125), m = T(0.25);
+  T a[FSR + 2][3];
+  const T* p = f + (r - 1) * LW + (c - 1);
+#pragma unroll
+  for (int j = 0; j < FSR + 2; ++j) {
+    a[j][0] = p[j * LW];
+    a[j][1] = p[j * LW + 1];
+    a[j][2] = p[j * LW + 2];
+  }
+#pragma unroll
+  for (int s = 0; s < FSR; ++s) {
+    T v = q * a[s][0];
+    v += e * a[s][1];
+    v += q * a[s][2];
+    v += e * a[s + 1][0];
+    v += m * a[s + 1][1];
+    v += e * a[s + 1][2];
+    v += q * a[s + 2][0];
+    v += e * a[s + 2][1];
+    v += q * a[s + 2][2];
+    o[s] = v;
+  }
+}
+
 template <class T, int NP>
 __global__ void __launch_bounds__(256) k_binomial_fused(Ptrs8<T> in, Ptrs8<T> out, int nx, int ny) {
-  constexpr int H = NP - 1, EH = FTY + 2 * H, EW = FTX + 2 * H, LW = EW + 1;
-  __shared__ T buf[2][EH * LW];
+  constexpr int FTY = filter_tile_rows<T>();
+  constexpr int H = NP - 1, EH = FTY + 2 * H, EW = FTX + 2 * H, LW = EW + 2;
+  // frame rows plus the rows the last strip of a pass reads past its end
+  constexpr int ROWS = EH + 2 + FSR;
+  __shared__ T buf[2][ROWS * LW];
   const T* __restrict__ src = in.p[blockIdx.z];
   T* __restrict__ dst = out.p[blockIdx.z];
   const int P = nx + 2;
-  const int r0 = blockIdx.y * FTY - H, c0 = blockIdx.x * FTX - H;  // 0-based interior coords of cell (0, 0)
-  const T q = T(0.0625), e = T(0.125), m = T(0.25);
-  for (int i = threadIdx.x; i < EH * EW; i += 256) {
-    const int lr = i / EW, lc = i - lr * EW;
-    int ir = (r0 + lr) % ny, ic = (c0 + lc) % nx;
-    if (ir < 0) ir += ny;
-    if (ic < 0) ic += nx;
-    const T* a = src + (int64_t)ir * P + ic;  // padded (ir+1-1, ic+1-1)
-    T v = q * a[0];
-    v += e * a[1];
-    v += q * a[2];
-    v += e * a[P];
-    v += m * a[P + 1];
-    v += e * a[P + 2];
-    v += q * a[2 * P];
-    v += e * a[2 * P + 1];
-    v += q * a[2 * P + 2];
-    buf[0][lr * LW + lc] = v;
-  }
+  const int r0 = blockIdx.y * FTY - H, c0 = blockIdx.x * FTX - H;  // 0-based interior coords of extended cell (0, 0)
+  const bool inner = r0 >= 0 && c0 >= 0 && r0 + EH <= ny && c0 + EW <= nx;
+  if (inner) {
+    // all of a thread's frame loads in flight before the first LDS store
+    constexpr int FR = (EH + 2) * (EW + 2), NL = (FR + 255) / 256;
+    T x[NL];
 #pragma unroll
-  for (int k = 1; k < NP; ++k) {
-    __syncthreads();
-    const T* f = buf[(k - 1) & 1];
-    T* o = buf[k & 1];
-    const int w = EW - 2 * k, h = EH - 2 * k;
-    for (int i = threadIdx.x; i < w * h; i += 256) {
-      const int lr = k + i / w, lc = k + i % w;
-      const T* a = f + (lr - 1) * LW + (lc - 1);
+    for (int j = 0; j < NL; ++j) {
+      const int i = threadIdx.x + 256 * j;
+      const int r = i / (EW + 2), c = i - r * (EW + 2);
+      if (i < FR) x[j] = src[(int64_t)(r0 + r) * P + c0 + c];
+    }
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      const int i = threadIdx.x + 256 * j;
+      const int r = i / (EW + 2), c = i - r * (EW + 2);
+      if (i < FR) buf[0][r * LW + c] = x[j];
+    }
+  } else {
+    const T q = T(0.0625), e = T(0.125), m = T(0.25);
+    for (int i = threadIdx.x; i < EH * EW; i += 256) {
+      const int lr = i / EW, lc = i - lr * EW;
+      int ir = (r0 + lr) % ny, ic = (c0 + lc) % nx;
+      if (ir < 0) ir += ny;
+      if (ic < 0) ic += nx;
+      const T* a = src + (int64_t)ir * P + ic;  // padded (ir+1-1, ic+1-1)
       T v = q * a[0];
       v += e * a[1];
       v += q * a[2];
-      v += e * a[LW];
-      v += m * a[LW + 1];
-      v += e * a[LW + 2];
-      v += q * a[2 * LW];
-      v += e * a[2 * LW + 1];
-      v += q * a[2 * LW + 2];
-      o[lr * LW + lc] = v;
+      v += e * a[P];
+      v += m * a[P + 1];
+      v += e * a[P + 2];
+      v += q * a[2 * P];
+      v += e * a[2 * P + 1];
+      v += q * a[2 * P + 2];
+      buf[1][(lr + 1) * LW + lc + 1] = v;
+    }
+  }
+  // passes 1 (inner workgroups) or 2 (edge workgroups) .. NP - 1 in LDS
+#pragma unroll
+  for (int k = 1; k < NP; ++k) {
+    if (k == 1 && !inner) continue;
+    __syncthreads();
+    const T* f = buf[(k - 1) & 1];
+    T* o = buf[k & 1];
+    const int w = EW + 2 - 2 * k, h = EH + 2 - 2 * k, ns = (h + FSR - 1) / FSR;
+    for (int i = threadIdx.x; i < w * ns; i += 256) {
+      const int sr = i / w, cc = i - sr * w;
+      const int r = k + sr * FSR, c = k + cc;
+      T v[FSR];
+      binomial_strip<T, LW>(f, r, c, v);
+#pragma unroll
+      for (int s = 0; s < FSR; ++s)
+        if (r + s < k + h) o[(r + s) * LW + c] = v[s];
     }
   }
   __syncthreads();
-  const T* f = buf[(NP - 1) & 1];
-  for (int i = threadIdx.x; i < FTY * FTX; i += 256) {
-    const int tr = i / FTX, tc = i - tr * FTX;
-    const int ir = r0 + H + tr, ic = c0 + H + tc;
-    if (ir >= ny || ic >= nx) continue;
-    const T v = f[(H + tr) * LW + H + tc];
-    const int r = ir + 1, c = ic + 1;
-    dst[(int64_t)r * P + c] = v;
-    // ghost ring = periodic copies (field.rs:141-257)
-    const bool top = r == 1, bot = r == ny, lft = c == 1, rgt = c == nx;
-    if (top) dst[(int64_t)(ny + 1) * P + c] = v;
-    if (bot) dst[c] = v;
-    if (lft) dst[(int64_t)r * P + nx + 1] = v;
-    if (rgt) dst[(int64_t)r * P] = v;
-    if (top && lft) dst[(int64_t)(ny + 1) * P + nx + 1] = v;
-    if (top && rgt) dst[(int64_t)(ny + 1) * P] = v;
-    if (bot && lft) dst[nx + 1] = v;
-    if (bot && rgt) dst[0] = v;
+  // pass NP: frame rows and columns NP .. NP + FT - 1 (H + 1 = NP)
+  for (int i = threadIdx.x; i < FTX * (FTY / FSR); i += 256) {
+    const int sr = i / FTX, cc = i - sr * FTX;
+    const int r = NP + sr * FSR, c = NP + cc;
+    T v[FSR];
+    if (NP == 1 && !inner) {
+#pragma unroll
+      for (int s = 0; s < FSR; ++s) v[s] = buf[1][(r + s) * LW + c];
+    } else {
+      binomial_strip<T, LW>(buf[(NP - 1) & 1], r, c, v);
+    }
+    const int ic = c0 + c - 1;
+    if (ic >= nx) continue;
+#pragma unroll
+    for (int s = 0; s < FSR; ++s) {
+      const int ir = r0 + r + s - 1;
+      if (ir >= ny) break;
+      const int pr = ir + 1, pc = ic + 1;
+      dst[(int64_t)pr * P + pc] = v[s];
+      // ghost ring = periodic copies (field.rs:141-257)
+      const bool top = pr == 1, bot = pr == ny, lft = pc == 1, rgt = pc == nx;
+      if (top) dst[(int64_t)(ny + 1) * P + pc] = v[s];
+      if (bot) dst[pc] = v[s];
+      if (lft) dst[(int64_t)pr * P + nx + 1] = v[s];
+      if (rgt) dst[(int64_t)pr * P] = v[s];
+      if (top && lft) dst[(int64_t)(ny + 1) * P + nx + 1] = v[s];
+      if (top && rgt) dst[(int64_t)(ny + 1) * P] = v[s];
+      if (bot && lft) dst[nx + 1] = v[s];
+      if (bot && rgt) dst[0] = v[s];
+    }
   }
 }
 
 template <class T, int NP>
 static void launch_fused(const Ptrs8<T>& a, const Ptrs8<T>& b, int ngrids, int nx, int ny, hipStream_t st) {
+  constexpr int FTY = filter_tile_rows<T>();
   dim3 grid((nx + FTX - 1) / FTX, (ny + FTY - 1) / FTY, ngrids);
   hipLaunchKernelGGL((k_binomial_fused<T, NP>), grid, dim3(256), 0, st, a, b, nx, ny);
 }

@@ -83,8 +83,10 @@ template <class T> void launch_push_move_deposit(const SpeciesDev<T>& s, const G
                                                  int* flags, hipStream_t st);
 // Cell-tile sort keeping each particle chunk spatially compact (see
 // k_particles): tiles of 2^twl x 2^thl cells, tile rows in serpentine order.
-// `count` and `cursor` hold one int per tile.
-constexpr int TILE_LOG_MIN = 1, TILE_LOG_MAX = 5;
+// `cursor` holds one int per tile, `count` one per tile and then one per
+// block of SCAN_BLK tiles (tile_count_size).
+constexpr int TILE_LOG_MIN = 1, TILE_LOG_MAX = 5, SCAN_BLK = 1024;
+inline size_t tile_count_size(size_t ntiles) { return ntiles + (ntiles + SCAN_BLK - 1) / SCAN_BLK; }
 template <class T> void launch_tile_sort(const SpeciesDev<T>& src, const SpeciesDev<T>& dst, int nx, int ny,
                                          int twl, int thl, int* count, int* cursor, hipStream_t st);
 template <class T> void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, double gamma_inj,

@@ -127,10 +127,13 @@ def test_deposit_ghosts_bit_exact(prec):
 
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("shape,passes", [((12, 24), 4), ((64, 256), 4), ((48, 96), 4), ((96, 384), 1),
-                                          ((96, 384), 9)])
+                                          ((96, 384), 9), ((192, 576), 2), ((192, 576), 3), ((144, 288), 6)])
 def test_binomial_filter_bit_exact_with_stale_ring(prec, shape, passes):
-    # tiles of 16 x 64 cells: the larger grids have tiles whose first pass
-    # wraps around an edge and tiles whose first pass stays inside
+    # tiles of 32 x 64 cells (FP64: 16 x 64): the larger grids have tiles
+    # whose first pass wraps around an edge and tiles whose frame (tile + the
+    # passes' halo + the 1-cell ring) lies inside the padded grid, read into
+    # LDS first, including frames that touch the ghost ring (one pass); grids
+    # that are not multiples of the tile have partial tiles
     ny, nx = shape
     g, r = pair(prec, size_x=nx, size_y=ny, delta=2)
     a = np.random.default_rng(11).standard_normal((ny + 2, nx + 2)).astype(g.ft)  # non-periodic ring
