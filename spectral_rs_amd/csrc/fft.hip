@@ -248,6 +248,29 @@ __device__ void fft_lds(V2<T>* lds, const FftPlan& pl, const Div& fseq, int seqs
   }
 }
 
+// Copies element i = threadIdx.x + k nt < E through `ld(i)` and `st(i, v)`,
+// eight elements at a time with all their loads issued before the first
+// store: a plain strided loop waits for each load before the next (the
+// compiler does not hoist loads across iterations of a runtime-length loop),
+// one memory round trip per element.
+template <class V, class LD, class ST>
+__device__ __forceinline__ void staged_copy(int E, int nt, LD ld, ST st) {
+  constexpr int B = 8;
+  for (int i0 = threadIdx.x; i0 < E; i0 += B * nt) {
+    V v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int i = i0 + u * nt;
+      if (i < E) v[u] = ld(i);
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int i = i0 + u * nt;
+      if (i < E) st(i, v[u]);
+    }
+  }
+}
+
 // Row transforms: block (bx, field) handles rows [bx*rpb, bx*rpb + rpb).
 // `dn`, `dP`: division by the row length and by the padded row length.
 template <class T, bool INV, bool POW2>
@@ -266,14 +289,19 @@ __global__ void __launch_bounds__(ROWS_NT_MAX) k_fft_rows(FftBatch b, FftPlan pl
   if (load == LOAD_REAL_PADDED) {
     const T sc = (T)b.scale[f];
     const bool scaled = b.scale[f] != 1.0;
-    for (int i = threadIdx.x; i < E; i += nt) {
-      const int s = (int)qdiv<POW2>(dn, i), n = i - s * nx;
-      T v = grid[(r0 + s + 1) * P + n + 1];
-      if (scaled) v = v * sc;
-      lds[lp<true>(i)] = V2<T>{v, T(0.0)};
-    }
+    staged_copy<T>(
+        E, nt,
+        [&](int i) {
+          const int s = (int)qdiv<POW2>(dn, i), n = i - s * nx;
+          return grid[(r0 + s + 1) * P + n + 1];
+        },
+        [&](int i, T v) {
+          if (scaled) v = v * sc;
+          lds[lp<true>(i)] = V2<T>{v, T(0.0)};
+        });
   } else {
-    for (int i = threadIdx.x; i < E; i += nt) lds[lp<true>(i)] = spec[(int64_t)r0 * nx + i];
+    staged_copy<V2<T>>(
+        E, nt, [&](int i) { return spec[(int64_t)r0 * nx + i]; }, [&](int i, V2<T> v) { lds[lp<true>(i)] = v; });
   }
   __syncthreads();
   fft_lds<T, INV, false, POW2, EPT_ROWS>(lds, pl, dn, rows, nx, 1, nt);
@@ -291,11 +319,12 @@ __global__ void __launch_bounds__(ROWS_NT_MAX) k_fft_rows(FftBatch b, FftPlan pl
     }
   } else {
     const bool scaled = scale_store != T(1);
-    for (int i = threadIdx.x; i < E; i += nt) {
-      V2<T> v = lds[lp<true>(i)];
-      if (scaled) v = V2<T>{v.x * scale_store, v.y * scale_store};
-      spec[(int64_t)r0 * nx + i] = v;
-    }
+    staged_copy<V2<T>>(
+        E, nt, [&](int i) { return lds[lp<true>(i)]; },
+        [&](int i, V2<T> v) {
+          if (scaled) v = V2<T>{v.x * scale_store, v.y * scale_store};
+          spec[(int64_t)r0 * nx + i] = v;
+        });
   }
 }
 
@@ -320,16 +349,17 @@ __global__ void __launch_bounds__(ROWS_NT_MAX) k_fft_rows_r2c(FftBatch b, FftPla
   const T* grid = (const T*)b.grid[f];
   const T sc = (T)b.scale[f];
   const bool scaled = b.scale[f] != 1.0;
-  for (int i = threadIdx.x; i < E; i += nt) {
-    const int s = (int)qdiv<POW2>(dn, i), n = i - s * nx;
-    const int64_t o = (int64_t)(2 * (q0 + s) + 1) * P + n + 1;  // padded row 2q + 1 = interior row 2q
-    T va = grid[o], vb = grid[o + P];
-    if (scaled) {
-      va = va * sc;
-      vb = vb * sc;
-    }
-    lds[lp<true>(i)] = V2<T>{va, vb};
-  }
+  staged_copy<V2<T>>(
+      E, nt,
+      [&](int i) {
+        const int s = (int)qdiv<POW2>(dn, i), n = i - s * nx;
+        const int64_t o = (int64_t)(2 * (q0 + s) + 1) * P + n + 1;  // padded row 2q + 1 = interior row 2q
+        return V2<T>{grid[o], grid[o + P]};
+      },
+      [&](int i, V2<T> v) {
+        if (scaled) v = V2<T>{v.x * sc, v.y * sc};
+        lds[lp<true>(i)] = v;
+      });
   __syncthreads();
   fft_lds<T, false, false, POW2, EPT_ROWS>(lds, pl, dn, rows, nx, 1, nt);
   const T h = T(0.5);
@@ -363,20 +393,27 @@ __global__ void __launch_bounds__(ROWS_NT_MAX) k_fft_rows_c2r(FftBatch b, FftPla
   const int nt = blockDim.x;
   const V2<T>* spec = (const V2<T>*)b.spec[f];
   T* grid = (T*)b.grid[f];
-  for (int i = threadIdx.x; i < E; i += nt) {
-    const int s = (int)qdiv<POW2>(dn, i), k = i - s * nx;
-    const int64_t o = (int64_t)(2 * (q0 + s)) * nx;
-    V2<T> z;
-    if (k <= half) {
-      V2<T> a = spec[o + k], c = spec[o + nx + k];
-      if (k == 0 || k == half) a.y = c.y = T(0);
-      z = V2<T>{a.x - c.y, a.y + c.x};
-    } else {  // A[k] = conj(A[nx - k]), B[k] = conj(B[nx - k])
-      const V2<T> a = spec[o + nx - k], c = spec[o + nx + nx - k];
-      z = V2<T>{a.x + c.y, c.x - a.y};
-    }
-    lds[lp<true>(i)] = z;
-  }
+  struct AC {
+    V2<T> a, c;
+  };
+  staged_copy<AC>(
+      E, nt,
+      [&](int i) {
+        const int s = (int)qdiv<POW2>(dn, i), k = i - s * nx;
+        const int64_t o = (int64_t)(2 * (q0 + s)) * nx + (k <= half ? k : nx - k);
+        return AC{spec[o], spec[o + nx]};
+      },
+      [&](int i, AC v) {
+        const int s = (int)qdiv<POW2>(dn, i), k = i - s * nx;
+        V2<T> a = v.a, c = v.c, z;
+        if (k <= half) {
+          if (k == 0 || k == half) a.y = c.y = T(0);
+          z = V2<T>{a.x - c.y, a.y + c.x};
+        } else {  // A[k] = conj(A[nx - k]), B[k] = conj(B[nx - k])
+          z = V2<T>{a.x + c.y, c.x - a.y};
+        }
+        lds[lp<true>(i)] = z;
+      });
   __syncthreads();
   fft_lds<T, true, false, POW2, EPT_ROWS>(lds, pl, dn, rows, nx, 1, nt);
   for (int i = threadIdx.x; i < rows * P; i += nt) {
@@ -415,10 +452,13 @@ __global__ void __launch_bounds__(COLS_NT_MAX) k_fft_cols(FftBatch b, FftPlan pl
   const int nt = blockDim.x;
   V2<T>* spec = (V2<T>*)b.spec[f];
   const Div fc = make_div((uint32_t)cols);
-  for (int i = threadIdx.x; i < E; i += nt) {
-    const int n = (int)qdiv<POW2>(fc, i), c = i - n * cols;
-    lds[i] = spec[(int64_t)n * nx + c0 + c];
-  }
+  staged_copy<V2<T>>(
+      E, nt,
+      [&](int i) {
+        const int n = (int)qdiv<POW2>(fc, i), c = i - n * cols;
+        return spec[(int64_t)n * nx + c0 + c];
+      },
+      [&](int i, V2<T> v) { lds[i] = v; });
   __syncthreads();
   fft_lds<T, INV, true, POW2, EPT_COLS>(lds, pl, fc, cols, 1, cols, nt);
   const bool scaled = scale_store != T(1);
