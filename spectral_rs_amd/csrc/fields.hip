@@ -526,8 +526,22 @@ __global__ void __launch_bounds__(256) k_spectral_pair(SpecPtrs sp, int nx, int 
 __global__ void __launch_bounds__(1024) k_gauss_reduce(const double* part, int nblk, double* out) {
   __shared__ double red[3][16];
   double s[3] = {0, 0, 0};
-  for (int b = threadIdx.x; b < nblk; b += 1024)
-    for (int k = 0; k < 3; ++k) s[k] += part[(int64_t)b * 3 + k];
+  // U partials per thread in flight at a time, added in the same order
+  constexpr int U = 8;
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * 1024) {
+    double v[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * 1024;
+      if (b < nblk)
+        for (int k = 0; k < 3; ++k) v[u][k] = part[(int64_t)b * 3 + k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (b0 + u * 1024 < nblk)
+        for (int k = 0; k < 3; ++k) s[k] += v[u][k];
+    }
+  }
   const int w = threadIdx.x >> 6;
   for (int k = 0; k < 3; ++k) {
     s[k] = wave_sum_d(s[k]);

@@ -1324,7 +1324,9 @@ __global__ void __launch_bounds__(256) k_fold_species_density(T* __restrict__ rh
 template <class T>
 static void fold_species_density(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, hipStream_t st) {
   const int64_t n = (int64_t)(g.ny + 2) * g.pitch;
-  const int64_t blocks = std::min<int64_t>((n / (16 / (int64_t)sizeof(T)) + 255) / 256, 2048) + 1;
+  // one 16-byte vector per thread up to 2^22 of them (4096 x 1024 cells: a
+  // grid-stride loop waited for each iteration's loads)
+  const int64_t blocks = std::min<int64_t>((n / (16 / (int64_t)sizeof(T)) + 255) / 256, 16384) + 1;
   hipLaunchKernelGGL(k_fold_species_density<T>, dim3((unsigned)blocks), dim3(256), 0, st, (T*)gs.g[SP_RHO],
                      (T*)gs.g[SP_DENS], (T*)gs.dsp, s.charge, n);
 }
