@@ -1033,43 +1033,45 @@ __device__ __forceinline__ int gcd_int(int a, int b) {
 // tile (round robin over its cells).  Lanes then rarely hit the same deposit
 // address, and neighbouring lanes read neighbouring field values.
 template <class T, bool HIST>
-__global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty,
+__global__ void __launch_bounds__(SORT_NT) __attribute__((amdgpu_waves_per_eu(5))) k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty,
                                                        int twl, int thl, int* ctr) {
   constexpr int CH = SORT_NT * SORT_PPT;
-  constexpr int OUT = HIST ? 1 : CH;
+  constexpr int OUT = HIST ? 2 : CH;
   static_assert(CH <= SORT_HT, "the staging pairs reuse the hash table");
   // hash keys | counts; once the output positions are known, the same LDS
   // stages pairs of 32-bit values by local index
   __shared__ __attribute__((aligned(16))) int s_tab[2 * SORT_HT];
   __shared__ int s_n[HIST ? 1 : SORT_HT];
-  __shared__ int s_gpos[OUT];  // placement: output position by local index
-  __shared__ int s_total, s_groups;
+  // placement: output position by local index.  Until the positions are
+  // written, its last two words count the workgroup's particles and groups
+  // (separate words would take the LDS 8 bytes past 32 KB, and with it the
+  // workgroups per CU from 5 to 4)
+  __shared__ int s_gpos[OUT];
+  int* const s_total = s_gpos + OUT - 2;
+  int* const s_groups = s_gpos + OUT - 1;
   lds_ptr<int> keys = (lds_ptr<int>)s_tab, cnt = (lds_ptr<int>)(s_tab + SORT_HT);
   for (int i = threadIdx.x; i < SORT_HT; i += SORT_NT) {
     keys[i] = -1;
     cnt[i] = 0;
   }
   if (threadIdx.x == 0) {
-    s_total = 0;
-    s_groups = 0;
+    *s_total = 0;
+    *s_groups = 0;
   }
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * CH;
-  // all loads first, so that they are in flight together
-  int32_t c[SORT_PPT], tg[SORT_PPT];
-  T v[5][SORT_PPT];
+  // the cells first (all in flight together); placement loads the other
+  // arrays only once the output positions are known, so that they do not
+  // hold registers through the ranking (with them loaded here the kernel
+  // took 138 VGPRs, 3 workgroups per CU)
+  int32_t c[SORT_PPT];
+  // the workgroup's part of each array from a uniform base, per-lane 32-bit
+  // offsets (compile-time steps of SORT_NT elements)
+  const int32_t* const ac = a.cell + base;
 #pragma unroll
   for (int j = 0; j < SORT_PPT; ++j) {
     const int64_t i = base + threadIdx.x + SORT_NT * j;
-    c[j] = i < a.n ? a.cell[i] : 0;
-    if (!HIST && i < a.n) {
-      v[0][j] = a.dx[i];
-      v[1][j] = a.dy[i];
-      v[2][j] = a.px[i];
-      v[3][j] = a.py[i];
-      v[4][j] = a.pz[i];
-      tg[j] = a.tag[i];
-    }
+    c[j] = i < a.n ? ac[threadIdx.x + SORT_NT * j] : 0;
   }
   int slot[SORT_PPT], rank[SORT_PPT];
 #pragma unroll
@@ -1089,8 +1091,8 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
       atomicAdd(ctr + key, n);
     } else {
       cnt[sl] = atomicAdd(ctr + key, n);  // the group's first output position
-      keys[sl] = atomicAdd(&s_total, n);  // the group's first local index
-      const int d = atomicAdd(&s_groups, 1);
+      keys[sl] = atomicAdd(s_total, n);  // the group's first local index
+      const int d = atomicAdd(s_groups, 1);
       // stride: the group's mean particle count per cell of the tile (n <=
       // 2048 < 2^12, u^-1 < n, group index d < 512 whenever it is used)
       int u = max(n >> (twl + thl), 1);
@@ -1100,27 +1102,32 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
   }
   if (HIST) return;
   __syncthreads();
+  const int ngroups = *s_groups;
+  __syncthreads();  // every thread has the count before s_gpos is rewritten
   // rank of each particle in its group's cell order (cell index inside the
   // tile, then arrival): per-group cell histograms in s_gpos, exclusive scans,
   // and the arrival rank inside the cell from the histogram's atomics.  With
   // more groups than the histograms fit (input far from sorted) the rank stays
   // the arrival order.
   const int C = 1 << (twl + thl);
-  if (s_groups * C <= CH) {
+  if (ngroups * C <= CH) {
     lds_ptr<int> hist = (lds_ptr<int>)s_gpos;
-    for (int i = threadIdx.x; i < s_groups * C; i += SORT_NT) hist[i] = 0;
+    for (int i = threadIdx.x; i < ngroups * C; i += SORT_NT) hist[i] = 0;
     __syncthreads();
-    int hb[SORT_PPT];
+    // histogram slot of particle j: its group's C counters, its cell's one
+    // (recomputed where needed rather than held in registers)
+    const auto hbin = [&](int j) {
+      const int cl = ((cell_ix(c[j]) - 1) & ((1 << twl) - 1)) | ((cell_iy(c[j]) - 1) & ((1 << thl) - 1)) << twl;
+      return (int)((unsigned)s_n[slot[j]] >> 23) * C + cl;
+    };
 #pragma unroll
     for (int j = 0; j < SORT_PPT; ++j) {
       const int64_t i = base + threadIdx.x + SORT_NT * j;
       if (i >= a.n) continue;
-      const int cl = ((cell_ix(c[j]) - 1) & ((1 << twl) - 1)) | ((cell_iy(c[j]) - 1) & ((1 << thl) - 1)) << twl;
-      hb[j] = (int)((unsigned)s_n[slot[j]] >> 23) * C + cl;
-      rank[j] = __hip_atomic_fetch_add(hist + hb[j], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      rank[j] = __hip_atomic_fetch_add(hist + hbin(j), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < s_groups; d += SORT_NT) {
+    for (int d = threadIdx.x; d < ngroups; d += SORT_NT) {
       int sum = 0;
       for (int q = 0; q < C; ++q) {
         const int t = hist[d * C + q];
@@ -1133,7 +1140,7 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
     for (int j = 0; j < SORT_PPT; ++j) {
       const int64_t i = base + threadIdx.x + SORT_NT * j;
       if (i >= a.n) continue;
-      rank[j] += hist[hb[j]];
+      rank[j] += hist[hbin(j)];
     }
     __syncthreads();  // s_gpos is rewritten below
   }
@@ -1148,54 +1155,73 @@ __global__ void __launch_bounds__(SORT_NT) k_tile_pass(SpeciesDev<T> a, SpeciesD
     s_gpos[loc] = cnt[slot[j]] + r;
     rank[j] = loc;  // from here on: the local index
   }
-  const int m = (int)min((int64_t)CH, a.n - base);
-  // the value arrays go through LDS two at a time, so that the global writes
+  // The value arrays go through LDS two at a time, so that the global writes
   // follow the output order (contiguous within a group): (cell, tag), then for
-  // FP32 (dx, dy), (px, py), (pz, -)
+  // FP32 (dx, dy), (px, py), (pz, -).  Each pair's loads are issued one round
+  // ahead, so that they are in flight while the previous pair is staged and
+  // only two pairs hold registers (all seven arrays loaded at once, or at the
+  // start, took 116-138 VGPRs: 3-4 workgroups per CU instead of 5).
+  const int m = (int)min((int64_t)CH, a.n - base);
+  const unsigned nl = (unsigned)min((int64_t)CH, a.n - base);  // this workgroup's particles
   int2* s_pair = (int2*)s_tab;
-  constexpr int ROUNDS = sizeof(T) == 4 ? 4 : 1;
-#pragma unroll
-  for (int f = 0; f < ROUNDS; ++f) {
-    __syncthreads();  // round 0: every thread is done with the hash table
+  const auto load32 = [&](const void* arr, int esz, int (&x)[SORT_PPT]) {
 #pragma unroll
     for (int j = 0; j < SORT_PPT; ++j) {
-      const int64_t i = base + threadIdx.x + SORT_NT * j;
-      if (i >= a.n) continue;
-      int2 x = {c[j], tg[j]};
-      if constexpr (sizeof(T) == 4) {
-        if (f > 0) {
-          x.x = __builtin_bit_cast(int, (float)v[2 * f - 2][j]);
-          x.y = f < 3 ? __builtin_bit_cast(int, (float)v[2 * f - 1][j]) : 0;
-        }
-      }
-      s_pair[rank[j]] = x;
+      const unsigned o = threadIdx.x + SORT_NT * j;
+      if (o < nl) x[j] = ((const int*)((const char*)arr + (int64_t)base * esz))[o];
     }
+  };
+  const auto stage = [&](const int (&x0)[SORT_PPT], const int (&x1)[SORT_PPT]) {
+    __syncthreads();  // the previous round's readers (first round: the hash table's) are done
+#pragma unroll
+    for (int j = 0; j < SORT_PPT; ++j)
+      if (threadIdx.x + SORT_NT * j < nl) s_pair[rank[j]] = int2{x0[j], x1[j]};
     __syncthreads();
+  };
+  int p0[SORT_PPT], p1[SORT_PPT], q0[SORT_PPT], q1[SORT_PPT];
+  load32(a.tag, 4, p1);
+  if constexpr (sizeof(T) == 4) {
+    load32(a.dx, 4, q0);
+    load32(a.dy, 4, q1);
+  }
+  stage(c, p1);
+  for (int l = threadIdx.x; l < m; l += SORT_NT) {
+    const int g = s_gpos[l];
+    const int2 x = s_pair[l];
+    b.cell[g] = x.x;
+    b.tag[g] = x.y;
+  }
+  if constexpr (sizeof(T) == 4) {
+    load32(a.px, 4, p0);
+    load32(a.py, 4, p1);
+    stage(q0, q1);
     for (int l = threadIdx.x; l < m; l += SORT_NT) {
       const int g = s_gpos[l];
       const int2 x = s_pair[l];
-      if (f == 0) {
-        b.cell[g] = x.x;
-        b.tag[g] = x.y;
-      } else if constexpr (sizeof(T) == 4) {
-        T* d0 = f == 1 ? b.dx : f == 2 ? b.px : b.pz;
-        d0[g] = __builtin_bit_cast(float, x.x);
-        if (f < 3) (f == 1 ? b.dy : b.py)[g] = __builtin_bit_cast(float, x.y);
-      }
+      b.dx[g] = __builtin_bit_cast(float, x.x);
+      b.dy[g] = __builtin_bit_cast(float, x.y);
     }
-  }
-  if constexpr (sizeof(T) != 4) {
+    load32(a.pz, 4, q0);
+    stage(p0, p1);
+    for (int l = threadIdx.x; l < m; l += SORT_NT) {
+      const int g = s_gpos[l];
+      const int2 x = s_pair[l];
+      b.px[g] = __builtin_bit_cast(float, x.x);
+      b.py[g] = __builtin_bit_cast(float, x.y);
+    }
+    stage(q0, q0);
+    for (int l = threadIdx.x; l < m; l += SORT_NT) b.pz[s_gpos[l]] = __builtin_bit_cast(float, s_pair[l].x);
+  } else {
     // FP64 values do not fit the 32-bit staging: written directly
+    const T* const src[5] = {a.dx + base, a.dy + base, a.px + base, a.py + base, a.pz + base};
+    T* const dst[5] = {b.dx, b.dy, b.px, b.py, b.pz};
 #pragma unroll
     for (int j = 0; j < SORT_PPT; ++j) {
-      const int64_t i = base + threadIdx.x + SORT_NT * j;
-      if (i >= a.n) continue;
+      const unsigned o = threadIdx.x + SORT_NT * j;
+      if (o >= nl) continue;
       const int g = s_gpos[rank[j]];
-      b.dx[g] = v[0][j];
-      b.dy[g] = v[1][j];
-      b.px[g] = v[2][j];
-      b.py[g] = v[3][j];
-      b.pz[g] = v[4][j];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) dst[q][g] = src[q][o];
     }
   }
 }
