@@ -39,6 +39,9 @@ struct SpeciesBuf {
   int64_t n = 0;
   double charge = 0, mass = 0;
   SoA cur, alt;  // alt is the target of the tile sort; the two swap after it
+  // the species' own density scratch grid for the step, whose passes all
+  // deposit before one fold adds every species' density to rho and dens
+  void* dsp = nullptr;
 };
 
 struct TimerPair {
@@ -485,11 +488,24 @@ template <class T> void one_step(sp_ctx* c) {
     }
     zero_currents<T>(c);
     {
+      // each species deposits its density into its own scratch grid; one
+      // fold then adds them to rho and dens in species order (the same
+      // arithmetic as a fold after each pass)
       PhaseScope pp(c, 3);
       for (auto& s : c->species) {
-        if (c->push_pending) launch_push_move_deposit<T>(sdev<T>(c, s), g, gs, c->flags_d, c->st);
-        else launch_move_deposit<T>(sdev<T>(c, s), g, gs, c->flags_d, c->st);
+        GridSet gss = gs;
+        gss.dsp = s.dsp;
+        if (c->push_pending) launch_push_move_deposit<T>(sdev<T>(c, s), g, gss, c->flags_d, c->st, false);
+        else launch_move_deposit<T>(sdev<T>(c, s), g, gss, c->flags_d, c->st, false);
       }
+      void* d[MAX_SPECIES];
+      double q[MAX_SPECIES];
+      int ns = 0;
+      for (auto& s : c->species) {
+        d[ns] = s.dsp;
+        q[ns++] = s.charge;
+      }
+      fold_species_densities<T>(gs, g, d, q, ns, c->st);
     }
     if (adaptive_sort(c)) slow_snapshot(c, c->steps);
     c->push_pending = false;
@@ -540,6 +556,9 @@ int valid_species(sp_ctx* c, int id) {
 int alloc_species(sp_ctx* c, SpeciesBuf& s, int64_t n) {
   s.n = n;
   const size_t nb = (size_t)(n > 0 ? n : 1);
+  const size_t gb = c->ngrid_elems() * c->esz;
+  HIPCHK(c, hipMalloc(&s.dsp, gb));
+  HIPCHK(c, hipMemsetAsync(s.dsp, 0, gb, c->st));
   for (SoA* a : {&s.cur, &s.alt}) {
     HIPCHK(c, hipMalloc(&a->cell, nb * 4));
     HIPCHK(c, hipMalloc(&a->tag, nb * 4));
@@ -550,6 +569,7 @@ int alloc_species(sp_ctx* c, SpeciesBuf& s, int64_t n) {
 }
 
 void free_species(SpeciesBuf& s) {
+  hipFree(s.dsp);
   for (SoA* a : {&s.cur, &s.alt}) {
     hipFree(a->cell);
     hipFree(a->tag);

@@ -1317,52 +1317,92 @@ static void launch_particles(const SpeciesDev<T>& s, const Geo<T>& g, const Grid
   }
 }
 
-// rho += q dsp, dens += dsp, dsp = 0 over the padded grid, after one
-// species' pass (ghosts included: the fold of the ring comes later).
+// rho += q_k dsp_k, dens += dsp_k, dsp_k = 0 over the padded grid for the NS
+// species k in order (ghosts included: the fold of the ring comes later).
 // 16-byte accesses over the body (the grids are separate allocations, so
 // 16-byte aligned), scalar ones over a tail.
-template <class T>
+template <class T, int NS> struct DspSet {
+  T* d[NS];
+  T q[NS];
+};
+template <class T, int NS>
 __global__ void __launch_bounds__(256) k_fold_species_density(T* __restrict__ rho, T* __restrict__ dens,
-                                                              T* __restrict__ dsp, T q, int64_t n) {
+                                                              DspSet<T, NS> ds, int64_t n) {
   constexpr int V = 16 / (int)sizeof(T);
   typedef T vec __attribute__((ext_vector_type(V)));
   const int64_t stride = (int64_t)gridDim.x * blockDim.x, t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t nv = n / V;
   for (int64_t i = t; i < nv; i += stride) {
-    const vec d = ((const vec*)dsp)[i];
+    vec d[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) d[k] = ((const vec*)ds.d[k])[i];
     vec r = ((vec*)rho)[i], e = ((vec*)dens)[i];
 #pragma unroll
-    for (int k = 0; k < V; ++k) {
-      r[k] += q * d[k];
-      e[k] += d[k];
+    for (int k = 0; k < NS; ++k) {
+#pragma unroll
+      for (int l = 0; l < V; ++l) {
+        r[l] += ds.q[k] * d[k][l];
+        e[l] += d[k][l];
+      }
     }
     ((vec*)rho)[i] = r;
     ((vec*)dens)[i] = e;
-    ((vec*)dsp)[i] = vec{};
+#pragma unroll
+    for (int k = 0; k < NS; ++k) ((vec*)ds.d[k])[i] = vec{};
   }
   for (int64_t i = nv * V + t; i < n; i += stride) {
-    const T d = dsp[i];
-    rho[i] += q * d;
-    dens[i] += d;
-    dsp[i] = T(0);
+    T r = rho[i], e = dens[i];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const T d = ds.d[k][i];
+      r += ds.q[k] * d;
+      e += d;
+      ds.d[k][i] = T(0);
+    }
+    rho[i] = r;
+    dens[i] = e;
+  }
+}
+template <class T, int NS>
+static void fold_group(const GridSet& gs, int64_t n, void* const* dsp, const double* q, hipStream_t st) {
+  DspSet<T, NS> ds;
+  for (int k = 0; k < NS; ++k) {
+    ds.d[k] = (T*)dsp[k];
+    ds.q[k] = (T)q[k];
+  }
+  // one 16-byte vector per thread up to 2^22 of them (4096 x 1024 cells: a
+  // grid-stride loop waited for each iteration's loads)
+  const int64_t blocks = std::min<int64_t>((n / (16 / (int64_t)sizeof(T)) + 255) / 256, 16384) + 1;
+  hipLaunchKernelGGL((k_fold_species_density<T, NS>), dim3((unsigned)blocks), dim3(256), 0, st, (T*)gs.g[SP_RHO],
+                     (T*)gs.g[SP_DENS], ds, n);
+}
+template <class T>
+void fold_species_densities(const GridSet& gs, const Geo<T>& g, void* const* dsp, const double* q, int n,
+                            hipStream_t st) {
+  const int64_t cells = (int64_t)(g.ny + 2) * g.pitch;
+  for (int k = 0; k < n; k += 4) {
+    switch (std::min(4, n - k)) {
+      case 1: fold_group<T, 1>(gs, cells, dsp + k, q + k, st); break;
+      case 2: fold_group<T, 2>(gs, cells, dsp + k, q + k, st); break;
+      case 3: fold_group<T, 3>(gs, cells, dsp + k, q + k, st); break;
+      default: fold_group<T, 4>(gs, cells, dsp + k, q + k, st); break;
+    }
   }
 }
 template <class T>
 static void fold_species_density(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, hipStream_t st) {
-  const int64_t n = (int64_t)(g.ny + 2) * g.pitch;
-  // one 16-byte vector per thread up to 2^22 of them (4096 x 1024 cells: a
-  // grid-stride loop waited for each iteration's loads)
-  const int64_t blocks = std::min<int64_t>((n / (16 / (int64_t)sizeof(T)) + 255) / 256, 16384) + 1;
-  hipLaunchKernelGGL(k_fold_species_density<T>, dim3((unsigned)blocks), dim3(256), 0, st, (T*)gs.g[SP_RHO],
-                     (T*)gs.g[SP_DENS], (T*)gs.dsp, s.charge, n);
+  void* d[1] = {gs.dsp};
+  const double q[1] = {(double)s.charge};
+  fold_species_densities<T>(gs, g, d, q, 1, st);
 }
 
 template <class T>
-void launch_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
+void launch_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st,
+                         bool fold) {
   if (s.n == 0) return;
   if (g.direct) launch_particles<T, false, true>(s, g, gs, flags, st);
   else launch_particles<T, false, false>(s, g, gs, flags, st);
-  fold_species_density<T>(s, g, gs, st);
+  if (fold) fold_species_density<T>(s, g, gs, st);
 }
 template <class T>
 void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags, hipStream_t st) {
@@ -1371,11 +1411,11 @@ void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int
 }
 template <class T>
 void launch_push_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags,
-                              hipStream_t st) {
+                              hipStream_t st, bool fold) {
   if (s.n == 0) return;
   if (g.direct) launch_particles<T, true, true>(s, g, gs, flags, st);
   else launch_particles<T, true, false>(s, g, gs, flags, st);
-  fold_species_density<T>(s, g, gs, st);
+  if (fold) fold_species_density<T>(s, g, gs, st);
 }
 template <class T>
 void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, double gamma_inj, uint64_t seed,
@@ -1389,14 +1429,16 @@ void launch_init_species(const SpeciesDev<T>& s, const Geo<T>& g, double vth, do
                      key);
 }
 
-#define INST(T)                                                                                               \
-  template void launch_move_deposit<T>(const SpeciesDev<T>&, const Geo<T>&, const GridSet&, int*, hipStream_t); \
-  template void launch_push<T>(const SpeciesDev<T>&, const Geo<T>&, const GridSet&, int*, hipStream_t);         \
-  template void launch_push_move_deposit<T>(const SpeciesDev<T>&, const Geo<T>&, const GridSet&, int*,          \
-                                            hipStream_t);                                                     \
-  template void launch_init_species<T>(const SpeciesDev<T>&, const Geo<T>&, double, double, uint64_t, int,      \
-                                       hipStream_t);                                                          \
-  template void launch_tile_sort<T>(const SpeciesDev<T>&, const SpeciesDev<T>&, int, int, int, int, int*, int*,   \
+#define INST(T)                                                                                                   \
+  template void launch_move_deposit<T>(const SpeciesDev<T>&, const Geo<T>&, const GridSet&, int*, hipStream_t, bool); \
+  template void launch_push<T>(const SpeciesDev<T>&, const Geo<T>&, const GridSet&, int*, hipStream_t);             \
+  template void launch_push_move_deposit<T>(const SpeciesDev<T>&, const Geo<T>&, const GridSet&, int*, hipStream_t, \
+                                            bool);                                                                  \
+  template void fold_species_densities<T>(const GridSet&, const Geo<T>&, void* const*, const double*, int,         \
+                                          hipStream_t);                                                             \
+  template void launch_init_species<T>(const SpeciesDev<T>&, const Geo<T>&, double, double, uint64_t, int,          \
+                                       hipStream_t);                                                                \
+  template void launch_tile_sort<T>(const SpeciesDev<T>&, const SpeciesDev<T>&, int, int, int, int, int*, int*,       \
                                     hipStream_t);
 INST(float)
 INST(double)

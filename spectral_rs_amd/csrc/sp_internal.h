@@ -75,12 +75,19 @@ __host__ __device__ inline int32_t make_cell(int ix, int iy) { return (iy << 16)
 template <class T> void fill(T* p, T v, int64_t n, hipStream_t st);
 // fill of up to 8 arrays of n elements in one launch
 template <class T> void fill_many(T* const* ptrs, int count, T v, int64_t n, hipStream_t st);
+// The particle pass of one species; its density goes to gs.dsp, which
+// `fold` then adds to rho (times the charge) and dens and clears.
 template <class T> void launch_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags,
-                                            hipStream_t st);
+                                            hipStream_t st, bool fold = true);
 template <class T> void launch_push(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs, int* flags,
                                    hipStream_t st);
 template <class T> void launch_push_move_deposit(const SpeciesDev<T>& s, const Geo<T>& g, const GridSet& gs,
-                                                 int* flags, hipStream_t st);
+                                                 int* flags, hipStream_t st, bool fold = true);
+// rho += q_k dsp_k, dens += dsp_k, dsp_k = 0 for k = 0 .. n - 1 in order, over
+// the padded grids (the folds of n passes run with fold = false, one launch
+// per 4 species).
+template <class T> void fold_species_densities(const GridSet& gs, const Geo<T>& g, void* const* dsp,
+                                               const double* q, int n, hipStream_t st);
 // Cell-tile sort keeping each particle chunk spatially compact (see
 // k_particles): tiles of 2^twl x 2^thl cells, tile rows in serpentine order.
 // `cursor` holds one int per tile, `count` one per tile and then one per
