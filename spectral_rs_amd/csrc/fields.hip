@@ -216,23 +216,42 @@ __global__ void __launch_bounds__(256) k_binomial_fused(Ptrs8<T> in, Ptrs8<T> ou
       if (i < FR) buf[0][r * LW + c] = x[j];
     }
   } else {
+    // U extended cells per thread at a time, all 9 U loads in flight before
+    // the sums (one cell per iteration waited for its loads each time; the
+    // small grids of an ensemble have only edge workgroups)
+    constexpr int U = 3;
     const T q = T(0.0625), e = T(0.125), m = T(0.25);
-    for (int i = threadIdx.x; i < EH * EW; i += 256) {
-      const int lr = i / EW, lc = i - lr * EW;
-      int ir = (r0 + lr) % ny, ic = (c0 + lc) % nx;
-      if (ir < 0) ir += ny;
-      if (ic < 0) ic += nx;
-      const T* a = src + (int64_t)ir * P + ic;  // padded (ir+1-1, ic+1-1)
-      T v = q * a[0];
-      v += e * a[1];
-      v += q * a[2];
-      v += e * a[P];
-      v += m * a[P + 1];
-      v += e * a[P + 2];
-      v += q * a[2 * P];
-      v += e * a[2 * P + 1];
-      v += q * a[2 * P + 2];
-      buf[1][(lr + 1) * LW + lc + 1] = v;
+    for (int i0 = threadIdx.x; i0 < EH * EW; i0 += U * 256) {
+      T a9[U][9];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * 256;
+        if (i >= EH * EW) break;
+        const int lr = i / EW, lc = i - lr * EW;
+        int ir = r0 + lr, ic = c0 + lc;
+        if (ir < 0 || ir >= ny) ir = (ir % ny + ny) % ny;
+        if (ic < 0 || ic >= nx) ic = (ic % nx + nx) % nx;
+        const T* a = src + (int64_t)ir * P + ic;  // padded (ir+1-1, ic+1-1)
+#pragma unroll
+        for (int t = 0; t < 9; ++t) a9[u][t] = a[(t / 3) * P + t % 3];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * 256;
+        if (i >= EH * EW) break;
+        const int lr = i / EW, lc = i - lr * EW;
+        const T* a = a9[u];
+        T v = q * a[0];
+        v += e * a[1];
+        v += q * a[2];
+        v += e * a[3];
+        v += m * a[4];
+        v += e * a[5];
+        v += q * a[6];
+        v += e * a[7];
+        v += q * a[8];
+        buf[1][(lr + 1) * LW + lc + 1] = v;
+      }
     }
   }
   // passes 1 (inner workgroups) or 2 (edge workgroups) .. NP - 1 in LDS
