@@ -1033,8 +1033,8 @@ __device__ __forceinline__ int gcd_int(int a, int b) {
 // tile (round robin over its cells).  Lanes then rarely hit the same deposit
 // address, and neighbouring lanes read neighbouring field values.
 template <class T, bool HIST>
-__global__ void __launch_bounds__(SORT_NT) __attribute__((amdgpu_waves_per_eu(5))) k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty,
-                                                       int twl, int thl, int* ctr) {
+__global__ void __launch_bounds__(SORT_NT) __attribute__((amdgpu_waves_per_eu(5)))
+k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty, int twl, int thl, int* ctr) {
   constexpr int CH = SORT_NT * SORT_PPT;
   constexpr int OUT = HIST ? 2 : CH;
   static_assert(CH <= SORT_HT, "the staging pairs reuse the hash table");
