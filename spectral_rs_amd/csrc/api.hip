@@ -598,8 +598,16 @@ int check_tuning(sp_ctx* c, const sp_tuning& t) {
 
 void apply_tuning(sp_ctx* c, const sp_tuning& t) {
   c->tune = t;
-  plan_dim(c->fx, c->p.size_x, t.fft_radix_x);  // keeps the twiddle table: it depends on n only
+  // the twiddle table depends on n only; its pass-order copy on the plan
+  plan_dim(c->fx, c->p.size_x, t.fft_radix_x);
   plan_dim(c->fy, c->p.size_y, t.fft_radix_y);
+  if (c->f64) {
+    fft_pass_twiddles<double>(c->fx, c->st);
+    fft_pass_twiddles<double>(c->fy, c->st);
+  } else {
+    fft_pass_twiddles<float>(c->fx, c->st);
+    fft_pass_twiddles<float>(c->fy, c->st);
+  }
   c->fx.per_block = t.fft_rows;
   c->fy.per_block = t.fft_cols;
   c->tune_gen++;
@@ -948,6 +956,8 @@ int sp_create(const sp_params* params, sp_ctx** out) {
   for (auto& s : c->spec) CK(hipMemsetAsync(s, 0, sb, c->st));
   CK(hipMalloc(&c->fx.tw, (size_t)fx.n * 2 * c->esz));
   CK(hipMalloc(&c->fy.tw, (size_t)fy.n * 2 * c->esz));
+  CK(hipMalloc(&c->fx.twp, (size_t)fx.n * 2 * c->esz));
+  CK(hipMalloc(&c->fy.twp, (size_t)fy.n * 2 * c->esz));
   if (c->f64) {
     fft_make_twiddles<double>(c->fx, c->st);
     fft_make_twiddles<double>(c->fy, c->st);
@@ -995,6 +1005,8 @@ int sp_destroy(sp_ctx* c) {
   for (auto s : c->spec) hipFree(s);
   hipFree(c->fx.tw);
   hipFree(c->fy.tw);
+  hipFree(c->fx.twp);
+  hipFree(c->fy.twp);
   hipFree(c->flags_d);
   hipFree(c->tile_count);
   hipFree(c->tile_cursor);

@@ -144,16 +144,20 @@ template <bool POW2> __device__ __forceinline__ int qmul(const Div& v, int x) {
 
 // Per-pass constants of one transform length, built on the host.
 struct FftPlan {
-  const void* tw;  // n complex Float: exp(-2 pi i k / n)
+  const void* twp;  // the plan's twiddles in pass order (FftDim::twp)
   int n, nrad;
   int R[16];          // radix of pass p
   Div nb[16], ns[16];  // n / R_p and the stride Ns_p = R_0 ... R_{p-1}
   Div nsr[16];         // Ns_p R_p
-  Div twstep[16];      // n / (Ns_p R_p)
 };
 
 // Twiddle k: a 32-bit byte offset from the table's (uniform) base, so the
-// load takes the scalar-base form with no 64-bit address arithmetic.
+// load takes the scalar-base form with no 64-bit address arithmetic.  The
+// pass-order table makes a pass's loads unit-stride in k: the lanes of a wave
+// read consecutive entries (rows) or a few (columns), where the strided reads
+// of tw[r k n / (Ns R)] touched up to one cache line per lane and made the
+// twiddles about half of the row transforms' time (4096-point rows: 87 us
+// with the passes, 42 without).
 template <class T> __device__ __forceinline__ V2<T> tw_load(const V2<T>* __restrict__ tw, int k) {
   const uint32_t off = (uint32_t)k * (uint32_t)sizeof(V2<T>);
   return *(const V2<T>*)((const char*)tw + off);
@@ -167,9 +171,8 @@ template <class T> __device__ __forceinline__ V2<T> tw_load(const V2<T>* __restr
 // overlaps the LDS reads.
 template <class T, int R, bool INV, bool SEQ_FAST, bool POW2, int EPTV>
 __device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, const Div& fnb, const Div& fns,
-                                              const Div& fnsr, const Div& fseq, const Div& twstep, int seqs,
-                                              int sstride, int estride, const V2<T>* __restrict__ tw,
-                                              int nthreads) {
+                                              const Div& fnsr, const Div& fseq, int seqs, int sstride, int estride,
+                                              const V2<T>* __restrict__ twp, int nthreads) {
   constexpr int BPT = (EPTV + R - 1) / R;
   const int nb = N / R;
   const int total = nb * seqs;
@@ -188,9 +191,9 @@ __device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, const D
       const int jq = (int)qdiv<POW2>(fns, (uint32_t)j);
       const int k = j - qmul<POW2>(fns, jq);
       if (Ns > 1) {
-        const int kt = qmul<POW2>(twstep, k);
+        const V2<T>* twb = twp + (Ns - 1);  // this pass's entries
 #pragma unroll
-        for (int r = 1; r < R; ++r) w[m][r] = tw_load<T>(tw, r * kt);
+        for (int r = 1; r < R; ++r) w[m][r] = tw_load<T>(twb, (r - 1) * Ns + k);
       }
       src[m] = s * sstride + j * estride;
       dst[m] = s * sstride + (qmul<POW2>(fnsr, jq) + k) * estride;
@@ -231,13 +234,13 @@ __device__ __forceinline__ void stockham_pass(V2<T>* lds, int N, int Ns, const D
 template <class T, bool INV, bool SEQ_FAST, bool POW2, int EPTV>
 __device__ void fft_lds(V2<T>* lds, const FftPlan& pl, const Div& fseq, int seqs, int sstride, int estride,
                         int nthreads) {
-  const V2<T>* tw = (const V2<T>*)pl.tw;
+  const V2<T>* twp = (const V2<T>*)pl.twp;
   int Ns = 1;
   for (int p = 0; p < pl.nrad; ++p) {
     const int R = pl.R[p];
-    const Div fnb = pl.nb[p], fns = pl.ns[p], fnsr = pl.nsr[p], ts = pl.twstep[p];
+    const Div fnb = pl.nb[p], fns = pl.ns[p], fnsr = pl.nsr[p];
 #define SP_PASS(RR, P2) \
-  stockham_pass<T, RR, INV, SEQ_FAST, P2, EPTV>(lds, pl.n, Ns, fnb, fns, fnsr, fseq, ts, seqs, sstride, estride, tw, \
+  stockham_pass<T, RR, INV, SEQ_FAST, P2, EPTV>(lds, pl.n, Ns, fnb, fns, fnsr, fseq, seqs, sstride, estride, twp, \
                                                  nthreads)
     if (R == 8) SP_PASS(8, POW2);
     else if (R == 4) SP_PASS(4, POW2);
@@ -513,15 +516,42 @@ template <class T> static void allow_big_lds() {
   });
 }
 
+// twp[e] for e < n - 1: the entry's pass p (Ns_p - 1 <= e < Ns_{p+1} - 1),
+// then r and k; the values are copies of tw.  twp[n - 1] is unused (1).
+template <class T>
+__global__ void k_twiddles_pass(V2<T>* __restrict__ twp, const V2<T>* __restrict__ tw, int n, uint64_t code,
+                                int nrad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  V2<T> v{T(1), T(0)};
+  int ns = 1;
+  for (int p = 0; p < nrad && p < 16; ++p) {
+    const int R = (int)((code >> (4 * p)) & 15u);
+    if (e < ns * R - 1) {
+      const int loc = e - (ns - 1), r = loc / ns + 1, k = loc - (r - 1) * ns;
+      v = tw[(int64_t)r * k * (n / (ns * R))];
+      break;
+    }
+    ns *= R;
+  }
+  twp[e] = v;
+}
+
+template <class T> void fft_pass_twiddles(const FftDim& d, hipStream_t st) {
+  hipLaunchKernelGGL(k_twiddles_pass<T>, dim3((d.n + 255) / 256), dim3(256), 0, st, (V2<T>*)d.twp,
+                     (const V2<T>*)d.tw, d.n, d.radix_code, d.nrad);
+}
+
 template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st, bool fp32_gen) {
   allow_big_lds<T>();  // at set-up, outside any stream capture
   hipLaunchKernelGGL(k_twiddles<T>, dim3((d.n + 255) / 256), dim3(256), 0, st, (V2<T>*)d.tw, d.n, (int)fp32_gen);
+  fft_pass_twiddles<T>(d, st);
 }
 
 // The pass constants of a plan; `pow2`: every radix is 2, 4 or 8.
 static FftPlan make_plan(const FftDim& d, bool& pow2) {
   FftPlan pl{};
-  pl.tw = d.tw;
+  pl.twp = d.twp;
   pl.n = d.n;
   pl.nrad = d.nrad;
   pow2 = true;
@@ -532,7 +562,6 @@ static FftPlan make_plan(const FftDim& d, bool& pow2) {
     pl.nb[p] = make_div((uint32_t)(d.n / R));
     pl.ns[p] = make_div((uint32_t)Ns);
     pl.nsr[p] = make_div((uint32_t)(Ns * R));
-    pl.twstep[p] = make_div((uint32_t)(d.n / (Ns * R)));
     if (R == 3) pow2 = false;
     Ns *= R;
   }
@@ -631,6 +660,7 @@ void launch_fft_cols(const FftBatch& b, const FftDim& dy, int nx, int inverse, d
 
 #define INST(T)                                                                                 \
   template void fft_make_twiddles<T>(FftDim&, hipStream_t, bool);                               \
+  template void fft_pass_twiddles<T>(const FftDim&, hipStream_t);                               \
   template void launch_fft_rows<T>(const FftBatch&, const FftDim&, int, int, int, int, double, hipStream_t); \
   template void launch_fft_cols<T>(const FftBatch&, const FftDim&, int, int, double, hipStream_t, int, bool); \
   template void launch_fft_rows_r2c<T>(const FftBatch&, const FftDim&, int, hipStream_t);                    \

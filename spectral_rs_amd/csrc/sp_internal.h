@@ -136,9 +136,16 @@ struct FftDim {
   int nrad;
   uint64_t radix_code;  // radix of pass p in bits [4p, 4p+4)
   void* tw;             // n complex Float: exp(-2 pi i k / n)
+  // the twiddles of the current plan in the order its passes read them: pass p
+  // (Ns_p = R_0 ... R_{p-1}) holds tw[r k n / (Ns_p R_p)] at Ns_p - 1 + (r - 1)
+  // Ns_p + k, r = 1 .. R_p - 1, k < Ns_p (n - 1 entries for any plan)
+  void* twp;
 };
-// fp32_gen: generate in FP32 (precision study) instead of FP64.
+// fp32_gen: generate in FP32 (precision study) instead of FP64; also lays
+// out twp for the current plan.
 template <class T> void fft_make_twiddles(FftDim& d, hipStream_t st, bool fp32_gen = false);
+// Lays out twp from tw for the plan in d (after a plan change).
+template <class T> void fft_pass_twiddles(const FftDim& d, hipStream_t st);
 // Row (x) and column (y) passes over up to 8 spectra at once.
 enum FftLoad { LOAD_COMPLEX = 0, LOAD_REAL_PADDED = 1 };
 enum FftStore { STORE_COMPLEX = 0, STORE_REAL_PADDED = 1 };
