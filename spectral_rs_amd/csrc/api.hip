@@ -66,6 +66,9 @@ struct sp_ctx {
   void* dsp = nullptr;  // per-species density scratch grid (GridSet::dsp)
   void* spec[14] = {};
   FftDim fx{}, fy{};
+  // spectra (bit = grid id) of which the step's real-field pipeline left only
+  // the half plane kx = 0 .. nx/2 valid; full_spectrum() fills the rest
+  uint32_t half_spec = 0;
   int* flags_d = nullptr;  // [0] invariant flags; [2..3] the slow-path particle counter (SLOW_COUNTER)
   int* flags_h = nullptr;
   // adaptive tile-sort schedule (sp_set_sort_policy): off while sort_max == 0
@@ -263,6 +266,26 @@ template <class T> void fold(sp_ctx* c) {
   launch_fold_ghosts<T>(g, 5, c->p.size_x, c->p.size_y, c->st);
 }
 
+// The spectra the step's pipeline leaves on the half plane kx = 0 .. nx/2:
+// the forward transforms of J and rho (the forward columns write that half
+// only).  E and B are transformed back in place after the update, so their
+// buffers hold the inverse pipeline's intermediate afterwards, as the
+// reference's in-place inverse leaves its spectral arrays.
+constexpr uint32_t HALF_SPECTRA = 1u << SP_JX | 1u << SP_JY | 1u << SP_JZ | 1u << SP_RHO;
+
+// Makes spectrum `id` whole again (the other half as conjugate mirrors of the
+// half plane) if the pipeline left only the half plane.
+void full_spectrum(sp_ctx* c, int id) {
+  if (!((c->half_spec >> id) & 1u)) return;
+  if (c->f64) launch_mirror_fill<double>(c->spec[id], c->p.size_x, c->p.size_y, c->st);
+  else launch_mirror_fill<float>(c->spec[id], c->p.size_x, c->p.size_y, c->st);
+  c->half_spec &= ~(1u << id);
+}
+void full_spectra(sp_ctx* c, uint32_t mask) {
+  for (int id = 0; id < 14; ++id)
+    if ((mask >> id) & 1u) full_spectrum(c, id);
+}
+
 template <class T> void flush_push(sp_ctx* c) {
   if (!c->push_pending) return;
   PhaseScope ps(c, 2);
@@ -293,10 +316,11 @@ template <class T> void fields_update(sp_ctx* c) {
   }
   // Every field transformed here is real and nx, ny are even (sp_create), so
   // the step runs a real-field pipeline: rows two at a time into half spectra,
-  // columns kx = 0 .. nx/2 (forward: plus their conjugate mirrors, so the
-  // stored J, rho spectra are full and exactly Hermitian), the update on those,
-  // and back.  (With even sizes the Nyquist column nx/2 and row ny/2 that the
-  // update zeroes, mod.rs:230-266, are their own mirrors.)
+  // columns kx = 0 .. nx/2, the update on that half plane (and B_wrk on both
+  // halves), and back.  The other half of the stored spectra is filled, as
+  // exact conjugate mirrors, only when read through the API (full_spectrum).
+  // (With even sizes the Nyquist column nx/2 and row ny/2 that the update
+  // zeroes, mod.rs:230-266, are their own mirrors.)
   {  // mod.rs:97-103, 122-124: 7 forward transforms, spatial->spectral fused
     FftBatch b{};
     const int ids[7] = {SP_JX, SP_JY, SP_JZ, SP_EX, SP_EY, SP_EZ, SP_RHO};
@@ -308,8 +332,9 @@ template <class T> void fields_update(sp_ctx* c) {
     }
     b.scale[6] = (double)(T(1.0) / (T)c->p.dens);  // mod.rs:80-84
     launch_fft_rows_r2c<T>(b, c->fx, ny, c->st);
-    launch_fft_cols<T>(b, c->fy, nx, 0, 1.0, c->st, nx / 2 + 1, true);
+    launch_fft_cols<T>(b, c->fy, nx, 0, 1.0, c->st, nx / 2 + 1, false);
   }
+  c->half_spec |= HALF_SPECTRA;
   // mod.rs:105-278, with the forward spectra (transforms of real fields) kept
   // exactly Hermitian (fields.hip)
   launch_spectral_update<T>(c->spec, geo<T>(c), c->p.dens, c->gauss_d, c->st, true);
@@ -643,6 +668,7 @@ int clone_ctx(sp_ctx* c, sp_ctx** out) {
     hipMemcpyAsync(t->grid[i], c->grid[i], c->ngrid_elems() * c->esz, hipMemcpyDeviceToDevice, c->st);
   for (int i = 0; i < 14; ++i)
     hipMemcpyAsync(t->spec[i], c->spec[i], c->nspec_elems() * 2 * c->esz, hipMemcpyDeviceToDevice, c->st);
+  t->half_spec = c->half_spec;
   const hipError_t e = hipStreamSynchronize(c->st);
   if (e != hipSuccess) return bail(hipGetErrorString(e));
   *out = t;
@@ -1171,6 +1197,11 @@ static int grid_io(sp_ctx* c, int grid, void* host, size_t count, bool to_dev, b
   rc = sync_check(c);
   if (rc) return rc;
   void* dev = spectrum ? c->spec[grid] : c->grid[grid];
+  if (spectrum && to_dev) c->half_spec &= ~(1u << grid);  // the caller's values are whole
+  if (spectrum && !to_dev) {
+    full_spectrum(c, grid);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
   if (to_dev) HIPCHK(c, hipMemcpy(dev, host, need * c->esz, hipMemcpyHostToDevice));
   else HIPCHK(c, hipMemcpy(host, dev, need * c->esz, hipMemcpyDeviceToHost));
   return SP_OK;
@@ -1312,6 +1343,7 @@ int sp_filter_noalloc(void* grid, int32_t size_x, int32_t size_y, int precision,
 int sp_fft2d(sp_ctx* c, int spec, int inverse) {
   ENTRY_PRELUDE(c);
   if (spec < 0 || spec >= 14) return fail(c, SP_EINVAL, "unknown spectrum id");
+  full_spectrum(c, spec);  // the transform reads (inverse) or rewrites (forward) all of it
   FftBatch b{};
   b.count = 1;
   b.spec[0] = c->spec[spec];
@@ -1333,6 +1365,8 @@ int sp_fft2d(sp_ctx* c, int spec, int inverse) {
 
 int sp_spectral_update(sp_ctx* c) {
   ENTRY_PRELUDE(c);
+  // the per-mode update reads every mode of J, rho and E
+  full_spectra(c, HALF_SPECTRA);
   if (c->f64) launch_spectral_update<double>(c->spec, geo<double>(c), c->p.dens, c->gauss_d, c->st, false);
   else launch_spectral_update<float>(c->spec, geo<float>(c), c->p.dens, c->gauss_d, c->st, false);
   HIPCHK(c, hipGetLastError());
@@ -1373,6 +1407,7 @@ int sp_step(sp_ctx* c, int n) {
       ++i;
     }
   }
+  if (n > 0) c->half_spec |= HALF_SPECTRA;  // graph replays skip the host side of fields_update
   HIPCHK(c, hipGetLastError());
   return SP_OK;
 }
@@ -1603,6 +1638,7 @@ int sp_checkpoint_save(sp_ctx* c, const char* path) {
   if (!c || !path) return SP_EINVAL;
   int rc = flush(c);
   if (rc) return rc;
+  full_spectra(c, HALF_SPECTRA);  // the file holds whole spectra
   rc = sync_check(c);
   if (rc) return rc;
   CkptHeader h{};
@@ -1702,6 +1738,7 @@ int sp_checkpoint_load(sp_ctx* c, const char* path) {
   }
   for (int g = 0; g < 11 && !rc; ++g) rc = file_to_dev(c, fc.f, c->grid[g], c->ngrid_elems() * c->esz, stage);
   for (int g = 0; g < 14 && !rc; ++g) rc = file_to_dev(c, fc.f, c->spec[g], c->nspec_elems() * 2 * c->esz, stage);
+  c->half_spec = 0;  // the file holds whole spectra
   if (rc) {
     for (auto& t : c->species) free_species(t);
     c->species.clear();
@@ -1877,6 +1914,8 @@ int sp_ensemble_step(sp_ensemble* e, int n) {
       ++i;
     }
   }
+  if (n > 0)
+    for (sp_ctx* c : e->m) c->half_spec |= HALF_SPECTRA;  // graph replays skip the host side of fields_update
   const hipError_t r = hipGetLastError();
   if (r != hipSuccess) return efail(e, SP_EHIP, hipGetErrorString(r));
   return SP_OK;

@@ -482,33 +482,36 @@ __global__ void __launch_bounds__(256) k_spectral(SpecPtrs sp, int nx, int ny, T
 }
 
 // The step's update (the real-field pipeline, even nx and ny): one thread per
-// mirrored pair (i, m = -i mod (Ny, Nx)) with m >= i, over rows 0..Ny/2.  The
-// seven forward spectra (Jx Jy Jz rho Ex Ey Ez) are transforms of real fields,
-// and the forward column pass wrote every column but 0 and nx/2 as the exact
-// conjugate mirror of another; the modes of those two self-mirrored columns
-// are Hermitian-projected here, X[i] <- (X[i] + conj(X[m])) / 2,
-// X[m] <- conj(X[i]) (J and rho stored back), elsewhere mode m is not read.
-// Both modes are updated by spectral_mode.  Exact Hermitian inputs matter
-// because B_wrk persists in spectral space: its anti-Hermitian part evolves as
-// B_a <- (1 - (c dt |k|)^2) B_a each step (the matching E part is dropped by
-// the real-part extraction), which grows for c dt |k| > sqrt(2).  With
-// Hermitian inputs spectral_mode maps mirrored modes to exactly conjugate
-// outputs, so B_wrk stays exactly Hermitian.  E and B are stored on the half
-// plane kx = 0 .. nx/2 only, which is all the inverse pipeline reads; B_wrk on
-// every mode.
+// mirrored pair (i, m = -i mod (Ny, Nx)) with i on the half plane kx = 0 ..
+// nx/2 (in the self-mirrored columns 0 and nx/2, m >= i).  The seven forward
+// spectra (Jx Jy Jz rho Ex Ey Ez) are transforms of real fields, of which the
+// forward pipeline wrote the half plane only; mode m of a pair in the other
+// columns is the conjugate of mode i and is not read.  The modes of the
+// self-mirrored columns are Hermitian-projected here, X[i] <- (X[i] +
+// conj(X[m])) / 2, X[m] <- conj(X[i]) (J and rho stored back).  Both modes are
+// updated by spectral_mode.  Exact Hermitian inputs matter because B_wrk
+// persists in spectral space: its anti-Hermitian part evolves as B_a <- (1 -
+// (c dt |k|)^2) B_a each step (the matching E part is dropped by the real-part
+// extraction), which grows for c dt |k| > sqrt(2).  With Hermitian inputs
+// spectral_mode maps mirrored modes to exactly conjugate outputs, so B_wrk
+// stays exactly Hermitian.  E and B are stored on the half plane only, which is
+// all the inverse pipeline reads; B_wrk on every mode.  (The context fills the
+// other half of the stored J and rho spectra as conjugate mirrors when they
+// are read through the API, k_mirror_fill.)
 template <class T>
 __global__ void __launch_bounds__(256) k_spectral_pair(SpecPtrs sp, int nx, int ny, T cdt, T ckc, T twopi_x,
                                                        T twopi_y, double* gauss) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int H = nx / 2 + 1;
   double g_pre = 0, g_post = 0, g_rho = 0;
-  if (i < (int64_t)(ny / 2 + 1) * nx) {
-    const int row = (int)(i / nx), col = (int)(i - (int64_t)row * nx);
+  if (t < (int64_t)ny * H) {
+    const int row = (int)(t / H), col = (int)(t - (int64_t)row * H);
     const int mrow = (ny - row) % ny, mcol = (nx - col) % nx;
-    const int64_t m = (int64_t)mrow * nx + mcol;
-    if (m >= i) {
+    const int64_t i = (int64_t)row * nx + col, m = (int64_t)mrow * nx + mcol;
+    const bool proj = col == 0 || 2 * col == nx;
+    if (!proj || m >= i) {
       const int ids[7] = {SP_JX, SP_JY, SP_JZ, SP_RHO, SP_EX, SP_EY, SP_EZ};
       Cx<T> a[7];
-      const bool proj = col == 0 || 2 * col == nx;
       if (proj) {
 #pragma unroll
         for (int f = 0; f < 7; ++f) {
@@ -526,7 +529,7 @@ __global__ void __launch_bounds__(256) k_spectral_pair(SpecPtrs sp, int nx, int 
         for (int f = 0; f < 7; ++f) a[f] = ld((const T*)sp.p[ids[f]], i);
       }
       spectral_mode<T>(sp, i, row, col, nx, ny, cdt, ckc, twopi_x, twopi_y, a[0], a[1], a[2], a[3], a[4], a[5],
-                       a[6], g_pre, g_post, g_rho, 2 * col <= nx);
+                       a[6], g_pre, g_post, g_rho, true);
       if (m != i) {
         Cx<T> b[7];
 #pragma unroll
@@ -581,6 +584,24 @@ __global__ void __launch_bounds__(1024) k_gauss_reduce(const double* part, int n
   }
 }
 
+// Columns nx/2 + 1 .. nx - 1 of a spectrum of a real field from the half
+// plane: X[r][c] = conj(X[-r][nx - c]) (nx even; the sources lie in columns
+// 1 .. nx/2 - 1, which are not written).
+template <class T> __global__ void k_mirror_fill(T* spec, int nx, int ny) {
+  const int W = nx / 2 - 1;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (W <= 0 || e >= (int64_t)ny * W) return;
+  const int row = (int)(e / W), col = nx / 2 + 1 + (int)(e - (int64_t)row * W);
+  const int64_t src = (int64_t)((ny - row) % ny) * nx + (nx - col), dst = (int64_t)row * nx + col;
+  spec[2 * dst] = spec[2 * src];
+  spec[2 * dst + 1] = -spec[2 * src + 1];
+}
+template <class T> void launch_mirror_fill(void* spec, int nx, int ny, hipStream_t st) {
+  const int64_t n = (int64_t)ny * (nx / 2 - 1);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mirror_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (T*)spec, nx, ny);
+}
+
 template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
                                                hipStream_t st, bool step) {
   SpecPtrs sp{};
@@ -590,7 +611,7 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   const T twopi_y = T(2.0) * PI / T(g.ny);
   const T ckc = g.dt / T(dens);  // mod.rs:127
   const T cdt = g.dt * g.c;      // mod.rs:128
-  const int64_t n = step ? (int64_t)(g.ny / 2 + 1) * g.nx : (int64_t)g.nx * g.ny;
+  const int64_t n = step ? (int64_t)g.ny * (g.nx / 2 + 1) : (int64_t)g.nx * g.ny;
   const int nblk = (int)((n + 255) / 256);
   if (step)
     hipLaunchKernelGGL(k_spectral_pair<T>, dim3((unsigned)nblk), dim3(256), 0, st, sp, g.nx, g.ny, cdt, ckc, twopi_x,
@@ -607,6 +628,7 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   template void launch_fold_ghosts<T>(T* const*, int, int, int, hipStream_t);                   \
   template void launch_update_ghosts<T>(T* const*, int, int, int, hipStream_t);                 \
   template void launch_binomial<T>(void**, void**, int, int, int, int, hipStream_t);              \
+  template void launch_mirror_fill<T>(void*, int, int, hipStream_t);                            \
   template void launch_spectral_update<T>(void* const*, const Geo<T>&, int, double*, hipStream_t, bool);
 INST(float)
 INST(double)

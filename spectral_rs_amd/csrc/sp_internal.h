@@ -104,6 +104,9 @@ template <class T> void launch_update_ghosts(T* const* grids, int ngrids, int nx
 // that grids[i] holds the result.
 template <class T> void launch_binomial(void** grids, void** work, int ngrids, int nx, int ny, int passes,
                                         hipStream_t st);
+// Columns nx/2 + 1 .. nx - 1 of the spectrum of a real field as the conjugate
+// mirrors of the half plane (nx even).
+template <class T> void launch_mirror_fill(void* spec, int nx, int ny, hipStream_t st);
 // `gauss` (may be null) receives the FP64 Gauss-law sums of this solve:
 // gauss[0..2] = sum |k.E + i rho|^2 before / after the correction and
 // sum |rho|^2; gauss[3] keeps the largest relative post-correction residual

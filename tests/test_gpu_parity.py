@@ -223,10 +223,11 @@ def rel(a, b):
 @pytest.mark.parametrize("prec", [4, 8])
 def test_step_real_field_pipeline_spectra_and_update_match_oracle(prec):
     # With even sizes the step transforms its real fields two rows at a time
-    # into half spectra, transforms columns kx = 0..Nx/2 and writes the other
-    # columns as their conjugate mirrors, projects only the self-mirrored
-    # columns 0 and Nx/2, and updates.  Checked here:
-    #  * the stored J, rho spectra are exactly Hermitian and within FP rounding
+    # into half spectra, transforms columns kx = 0..Nx/2 (the getter fills the
+    # other columns as their conjugate mirrors), projects only the
+    # self-mirrored columns 0 and Nx/2, and updates the half plane (B_wrk on
+    # both halves).  Checked here:
+    #  * the J, rho spectra read back are exactly Hermitian and within FP rounding
     #    of numpy's FP64 fft2 of the interiors (rho / dens);
     #  * B_wrk equals, bit for bit, the oracle's update run on the spectra the
     #    step used: J, rho as stored, E rebuilt through the same pipeline in a
@@ -275,6 +276,38 @@ def test_step_real_field_pipeline_spectra_and_update_match_oracle(prec):
         s = g.spectrum(name)
         assert np.array_equal(s, r.spectrum(name)), name
         assert np.array_equal(s[mi][:, mj], np.conj(s)), name
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_half_plane_spectra_read_back_whole(prec):
+    # The step's pipeline stores the J and rho spectra on the half plane
+    # kx = 0..Nx/2 only; the getter fills the other columns as exact conjugate
+    # mirrors, so what it returns is exactly Hermitian and the transform of the
+    # (filtered) grid.  A spectrum set through the API is returned as set (not
+    # mirrored), until the next update.
+    nx, ny, dens = 64, 32, 3
+    g, _ = pair(prec, size_x=nx, size_y=ny, delta=2, dens=dens, n_pass=2)
+    rng = np.random.default_rng(21)
+    for k in ["ex", "ey", "ez", "jx", "jy", "jz", "rho"]:
+        g.set_grid(k, rng.standard_normal((ny + 2, nx + 2)).astype(g.ft))
+    g.fields_update()
+    mi, mj = (-np.arange(ny)) % ny, (-np.arange(nx)) % nx
+    tol = 2e-6 if prec == 4 else 1e-14
+    for k in ["jz", "jx", "jy", "rho"]:
+        sp = g.spectrum(k)
+        assert np.array_equal(sp[mi][:, mj], np.conj(sp)), k
+        x = g.grid(k)[1:-1, 1:-1].astype(np.float64)
+        ref = np.fft.fft2(x / dens if k == "rho" else x)
+        if k in ("jx", "jy"):
+            ref[0, 0] = 0.0  # mod.rs:106-108
+        assert np.max(np.abs(sp - ref)) <= tol * np.max(np.abs(ref)), k
+    x = (rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))).astype(g.ct)
+    g.set_spectrum("jz", x)
+    assert np.array_equal(g.spectrum("jz"), x)
+    g.fields_update()
+    sp = g.spectrum("jz")
+    assert np.array_equal(sp[mi][:, mj], np.conj(sp))
+    g.close()
 
 
 @pytest.mark.parametrize("prec", [4, 8])
