@@ -143,7 +143,12 @@ int sp_get_species_raw(sp_ctx* ctx, int id, int64_t n, int32_t* cell, int32_t* t
 /* Padded real grids: count = (size_x+2)*(size_y+2) elements. */
 int sp_get_grid(sp_ctx* ctx, int grid, void* out, size_t count);
 int sp_set_grid(sp_ctx* ctx, int grid, const void* in, size_t count);
-/* Spectra: interleaved (re, im), count = 2*size_x*size_y elements. */
+/* Spectra: interleaved (re, im), count = 2*size_x*size_y elements.  After a
+ * step the J and rho spectra read back whole (the step keeps only their half
+ * plane kx = 0..Nx/2; the getter fills the rest as conjugate mirrors); the E
+ * and B buffers have been transformed back in place and hold no spectra, as
+ * the reference's arrays after its in-place inverse (src/flds/mod.rs:280-290);
+ * SP_BXW..SP_BZW persist. */
 int sp_get_spectrum(sp_ctx* ctx, int spec, void* out, size_t count);
 int sp_set_spectrum(sp_ctx* ctx, int spec, const void* in, size_t count);
 

@@ -269,8 +269,9 @@ template <class T> void fold(sp_ctx* c) {
 // The spectra the step's pipeline leaves on the half plane kx = 0 .. nx/2:
 // the forward transforms of J and rho (the forward columns write that half
 // only).  E and B are transformed back in place after the update, so their
-// buffers hold the inverse pipeline's intermediate afterwards, as the
-// reference's in-place inverse leaves its spectral arrays.
+// buffers no longer hold spectra afterwards (the reference's in-place inverse,
+// src/flds/mod.rs:280-290, leaves its E and B spectral arrays transformed
+// back as well).
 constexpr uint32_t HALF_SPECTRA = 1u << SP_JX | 1u << SP_JY | 1u << SP_JZ | 1u << SP_RHO;
 
 // Makes spectrum `id` whole again (the other half as conjugate mirrors of the
