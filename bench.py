@@ -325,8 +325,8 @@ def main():
                       f"particles on the particle pass's general path rose by {policy[2]}" if policy else
                       f"one per {args.sort_interval} steps") +
                    "), and one stand-alone Boris push at the end (the last step's push, otherwise fused into "
-                   "the next step's particle pass); the roofline duration is the fused particle pass with the "
-                   "species-density fold that follows it (k_fold_species_density, ~18 us)"),
+                   "the next step's particle pass); the roofline duration is the fused particle pass with its "
+                   "share of the species-density fold that follows both passes (k_fold_species_density, ~23 us)"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_particles<float, true> (fused push + move + deposit, one species per launch)",
