@@ -344,11 +344,17 @@ __global__ void k_dn_apply(Grids4 in, Grids4 out, const float* __restrict__ scal
   if (!ok) return;
   const int xx = blockIdx.x * blockDim.x + threadIdx.x;
   if (xx >= nx) return;
-  for (int cr = blockIdx.y; cr < 4 * ny; cr += gridDim.y) {
-    const int c = cr / ny, yy = cr - c * ny;
-    if (!((chmask >> c) & 1)) continue;
+  // rows (z, yy) of the written channels only: z-th set bit of chmask
+  const int nch = __popc((unsigned)chmask & 15u);
+  for (int zr = blockIdx.y; zr < nch * ny; zr += gridDim.y) {
+    const int z = zr / ny, yy = zr - z * ny;
+    int c = -1;
+    for (int k = z; k >= 0; --k) {
+      do ++c;
+      while (c < 3 && !((chmask >> c) & 1));
+    }
     const int64_t o = (int64_t)(yy + 1) * (nx + 2) + xx + 1;
-    out.g[c][o] = in.g[c][o] + dens * scale[c] * r[(int64_t)cr * nx + xx];
+    out.g[c][o] = in.g[c][o] + dens * scale[c] * r[((int64_t)c * ny + yy) * nx + xx];
   }
 }
 
@@ -359,10 +365,16 @@ __global__ void k_dn_apply(Grids4 in, Grids4 out, const float* __restrict__ scal
 // highest wavenumbers most).  FP64 sums, one atomic per block and quantity.
 __global__ void __launch_bounds__(256) k_dn_hp(Grids4 in, Grids4 out, const float* __restrict__ scale, float dens,
                                                const float* __restrict__ r, int nx, int ny, double* sums,
-                                               const int* __restrict__ stats) {
+                                               const int* __restrict__ stats, int chmask) {
   __shared__ double red[2][4];
   if (stats[DN_STAT_SKIP] > 0) return;
-  const int c = blockIdx.z;
+  // blockIdx.z: the z-th channel written back (the check reads only those;
+  // the launch has one z per set bit of chmask)
+  int c = -1;
+  for (int z = (int)blockIdx.z; z >= 0; --z) {
+    do ++c;
+    while (c < 3 && !((chmask >> c) & 1));
+  }
   const int xx = blockIdx.x * blockDim.x + threadIdx.x;
   double hy = 0.0, hf = 0.0;
   const int P = nx + 2;
@@ -811,11 +823,13 @@ void dn_apply_sim(const DnParams& P, const DnTensors& T, float* const* inputs, f
   hipLaunchKernelGGL((k_dn_fused<true>), grid, dim3(FNT), 0, st, in, P, (const short*)T.packed, T.r, ny, nx);
   // a few hundred rows per block: 2 FP64 atomics per block on 8 addresses
   // (one block per row made them serialise: 0.35 ms at 4096 x 1024)
-  if (hp_tol > 0)
-    hipLaunchKernelGGL(k_dn_hp, dim3((nx + 255) / 256, (unsigned)std::min(ny, 16), 4), dim3(256), 0, st, in.gr, out,
-                       P.scale, (float)dens, T.r, nx, ny, sums, stats);
-  hipLaunchKernelGGL(k_dn_apply, dim3((nx + 255) / 256, (unsigned)std::min(4 * ny, 4096)), dim3(256), 0, st, in.gr,
-                     out, P.scale, (float)dens, T.r, nx, ny, sums, gauss, tol, gauss_tol, hp_tol, chmask, stats);
+  const int nch = __builtin_popcount((unsigned)chmask & 15u);
+  if (hp_tol > 0 && nch > 0)
+    hipLaunchKernelGGL(k_dn_hp, dim3((nx + 255) / 256, (unsigned)std::min(ny, 16), (unsigned)nch), dim3(256), 0, st,
+                       in.gr, out, P.scale, (float)dens, T.r, nx, ny, sums, stats, chmask);
+  hipLaunchKernelGGL(k_dn_apply, dim3((nx + 255) / 256, (unsigned)std::max(1, std::min(nch * ny, 4096))), dim3(256),
+                     0, st, in.gr, out, P.scale, (float)dens, T.r, nx, ny, sums, gauss, tol, gauss_tol, hp_tol, chmask,
+                     stats);
   hipLaunchKernelGGL(k_dn_backoff, dim3(1), dim3(64), 0, st, stats, backoff);
 }
 
