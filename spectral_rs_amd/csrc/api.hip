@@ -338,11 +338,13 @@ template <class T> void fields_update(sp_ctx* c) {
   c->half_spec |= HALF_SPECTRA;
   // mod.rs:105-278, with the forward spectra (transforms of real fields) kept
   // exactly Hermitian (fields.hip)
-  launch_spectral_update<T>(c->spec, geo<T>(c), c->p.dens, c->gauss_d, c->st, true);
+  const int gblk = launch_spectral_update<T>(c->spec, geo<T>(c), c->p.dens, c->gauss_d, c->st, true, false);
   {  // mod.rs:280-307: 6 inverse transforms, spectral->spatial + ghosts fused
     FftBatch b{};
     const int ids[6] = {SP_BX, SP_BY, SP_BZ, SP_EX, SP_EY, SP_EZ};
     b.count = 6;
+    b.gauss = c->gauss_d;  // the inverse columns' first block sums the update's Gauss partials
+    b.gauss_nblk = gblk;
     for (int i = 0; i < 6; ++i) {
       b.spec[i] = c->spec[ids[i]];
       b.grid[i] = c->grid[ids[i]];

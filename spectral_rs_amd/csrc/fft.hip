@@ -453,6 +453,7 @@ __global__ void __launch_bounds__(COLS_NT_MAX) k_fft_cols(FftBatch b, FftPlan pl
   const int cols = min(C, ncols - c0);
   const int E = cols * ny;
   const int nt = blockDim.x;
+  if (b.gauss && blockIdx.x == 0 && blockIdx.y == 0) gauss_reduce_block(b.gauss, b.gauss_nblk);
   V2<T>* spec = (V2<T>*)b.spec[f];
   const Div fc = make_div((uint32_t)cols);
   staged_copy<V2<T>>(

@@ -542,47 +542,10 @@ __global__ void __launch_bounds__(256) k_spectral_pair(SpecPtrs sp, int nx, int 
   if (gauss) gauss_partials(g_pre, g_post, g_rho, gauss);
 }
 
-// Sums the block partials of k_spectral (one block, fixed order) into
-// out[0..2] = (pre, post, |rho|^2) of this solve, and keeps in out[3] the
-// largest relative post-correction residual sqrt(post / |rho|^2) seen so far.
-__global__ void __launch_bounds__(1024) k_gauss_reduce(const double* part, int nblk, double* out) {
-  __shared__ double red[3][16];
-  double s[3] = {0, 0, 0};
-  // U partials per thread in flight at a time, added in the same order
-  constexpr int U = 8;
-  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * 1024) {
-    double v[U][3];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int b = b0 + u * 1024;
-      if (b < nblk)
-        for (int k = 0; k < 3; ++k) v[u][k] = part[(int64_t)b * 3 + k];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (b0 + u * 1024 < nblk)
-        for (int k = 0; k < 3; ++k) s[k] += v[u][k];
-    }
-  }
-  const int w = threadIdx.x >> 6;
-  for (int k = 0; k < 3; ++k) {
-    s[k] = wave_sum_d(s[k]);
-    if ((threadIdx.x & 63) == 0) red[k][w] = s[k];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[3];
-    for (int k = 0; k < 3; ++k) {
-      t[k] = 0;
-      for (int i = 0; i < 16; ++i) t[k] += red[k][i];
-    }
-    out[0] = t[0];
-    out[1] = t[1];
-    out[2] = t[2];
-    const double rel = t[2] > 0 ? sqrt(t[1] / t[2]) : sqrt(t[1]);
-    out[3] = fmax(out[3], rel);
-  }
-}
+// Sums the block partials of the spectral update (one workgroup, fixed order):
+// gauss[0..2] = (pre, post, |rho|^2) of this solve, gauss[3] the largest
+// relative post-correction residual so far (gauss_reduce_block).
+__global__ void __launch_bounds__(1024) k_gauss_reduce(double* gauss, int nblk) { gauss_reduce_block(gauss, nblk); }
 
 // Columns nx/2 + 1 .. nx - 1 of a spectrum of a real field from the half
 // plane: X[r][c] = conj(X[-r][nx - c]) (nx even; the sources lie in columns
@@ -602,8 +565,8 @@ template <class T> void launch_mirror_fill(void* spec, int nx, int ny, hipStream
   hipLaunchKernelGGL(k_mirror_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (T*)spec, nx, ny);
 }
 
-template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
-                                               hipStream_t st, bool step) {
+template <class T> int launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
+                                              hipStream_t st, bool step, bool reduce) {
   SpecPtrs sp{};
   for (int i = 0; i < 14; ++i) sp.p[i] = spec[i];
   const T PI = (T)3.14159265358979323846264338327950288;
@@ -619,7 +582,8 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   else
     hipLaunchKernelGGL(k_spectral<T>, dim3((unsigned)nblk), dim3(256), 0, st, sp, g.nx, g.ny, cdt, ckc, twopi_x,
                        twopi_y, gauss ? gauss + GAUSS_PARTIALS : nullptr);
-  if (gauss) hipLaunchKernelGGL(k_gauss_reduce, dim3(1), dim3(1024), 0, st, gauss + GAUSS_PARTIALS, nblk, gauss);
+  if (gauss && reduce) hipLaunchKernelGGL(k_gauss_reduce, dim3(1), dim3(1024), 0, st, gauss, nblk);
+  return nblk;
 }
 
 #define INST(T)                                                                                 \
@@ -629,7 +593,7 @@ template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& 
   template void launch_update_ghosts<T>(T* const*, int, int, int, hipStream_t);                 \
   template void launch_binomial<T>(void**, void**, int, int, int, int, hipStream_t);              \
   template void launch_mirror_fill<T>(void*, int, int, hipStream_t);                            \
-  template void launch_spectral_update<T>(void* const*, const Geo<T>&, int, double*, hipStream_t, bool);
+  template int launch_spectral_update<T>(void* const*, const Geo<T>&, int, double*, hipStream_t, bool, bool);
 INST(float)
 INST(double)
 #undef INST

@@ -119,11 +119,59 @@ constexpr int MAX_SPECIES = 64;
 constexpr int DIAG_FIXED = 4;
 constexpr int DIAG_DOUBLES = DIAG_FIXED + 4 * MAX_SPECIES;
 inline size_t gauss_doubles(int nx, int ny) { return GAUSS_PARTIALS + 3 * (((size_t)nx * ny + 255) / 256); }
+// Sums the three partials of nblk blocks (gauss + GAUSS_PARTIALS) into
+// gauss[0..2] and keeps the largest relative post-correction residual
+// sqrt(post / |rho|^2) seen so far in gauss[3]; run by every thread of one
+// workgroup (a whole number of waves, at most 1024 threads).
+__device__ inline void gauss_reduce_block(double* gauss, int nblk) {
+  __shared__ double red[3][16];
+  const double* part = gauss + GAUSS_PARTIALS;
+  const int nt = (int)blockDim.x;
+  double s[3] = {0, 0, 0};
+  constexpr int U = 8;  // partials per thread in flight at a time, added in order
+  for (int b0 = (int)threadIdx.x; b0 < nblk; b0 += U * nt) {
+    double v[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * nt;
+      if (b < nblk)
+        for (int k = 0; k < 3; ++k) v[u][k] = part[(int64_t)b * 3 + k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (b0 + u * nt < nblk)
+        for (int k = 0; k < 3; ++k) s[k] += v[u][k];
+    }
+  }
+  const int w = (int)threadIdx.x >> 6;
+  for (int k = 0; k < 3; ++k) {
+    double x = s[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0) red[k][w] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[3];
+    for (int k = 0; k < 3; ++k) {
+      t[k] = 0;
+      for (int i = 0; i < nt / 64; ++i) t[k] += red[k][i];
+    }
+    gauss[0] = t[0];
+    gauss[1] = t[1];
+    gauss[2] = t[2];
+    const double rel = t[2] > 0 ? sqrt(t[1] / t[2]) : sqrt(t[1]);
+    gauss[3] = fmax(gauss[3], rel);
+  }
+}
 // `step`: the step's update on the real-field pipeline's spectra (projection of
 // the self-mirrored columns, E and B on the half plane; see k_spectral_pair);
-// otherwise every mode as given (sp_spectral_update).
-template <class T> void launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
-                                               hipStream_t st, bool step);
+// otherwise every mode as given (sp_spectral_update).  Returns the number of
+// blocks whose Gauss partials the update wrote; with `reduce` false the caller
+// has them summed later (gauss_reduce_block, run by the first block of the
+// step's inverse column transforms), else a single-workgroup launch sums them.
+template <class T> int launch_spectral_update(void* const* spec, const Geo<T>& g, int dens, double* gauss,
+                                              hipStream_t st, bool step, bool reduce = true);
 // FP64 diagnostics: out[0] += sum over the interior of a^2 + b^2 + c^2;
 // kinetic: out[0] += sum (gamma - 1), out[1..3] += sum p.
 // sum32: accumulate in FP32 (precision study), the total still stored as FP64.
@@ -157,6 +205,10 @@ struct FftBatch {
   void* spec[8];     // spectra (Ny x Nx complex)
   void* grid[8];     // padded real grids (for the fused load / store)
   double scale[8];   // multiplier applied at the fused real load
+  // column transforms: when set, the first block first sums the spectral
+  // update's Gauss partials (gauss_reduce_block) -- one launch less per step
+  double* gauss;
+  int gauss_nblk;
 };
 // `store_scale` multiplies every stored value (the 1/(Nx Ny) of an inverse).
 template <class T> void launch_fft_rows(const FftBatch& b, const FftDim& dx, int ny, int inverse, int load, int store,
