@@ -933,13 +933,15 @@ __device__ __forceinline__ int tile_of(int32_t c, int ntx, int nty, int twl, int
   return ty * ntx + ((ty & 1) ? ntx - 1 - tx : tx);
 }
 
-// Each workgroup takes SORT_PPT * 256 consecutive particles and counts them
-// per tile in an LDS hash table (tile -> count), one LDS atomic per group of
-// lanes of a wave that share a tile; then one global atomic per distinct tile
-// of the workgroup.  The input is sorted by the previous pass except for the
-// particles that changed tile since, so a workgroup holds few distinct tiles
-// and the global counters see ~1-3 atomics per workgroup instead of one per
-// wave and tile.
+// Each workgroup takes SORT_PPT * 256 consecutive particles.  Placement
+// aggregates them per tile in an LDS hash table (tile -> count), one LDS
+// atomic per group of lanes of a wave that share a tile, then one global
+// atomic per distinct tile of the workgroup (its first output position).  The
+// input is sorted by the previous pass except for the particles that changed
+// tile since, so a workgroup holds few distinct tiles.  Counting adds each
+// wave's group counts to the global counters directly, without a return value
+// (no LDS round trips: 203 -> 186 us per species at 4096 x 1024 after 20
+// steps of motion, against the hash-table aggregation).
 #ifndef SP_SORT_HT_LOG
 #define SP_SORT_HT_LOG 11
 #endif
@@ -1072,6 +1074,26 @@ k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty, int twl, int thl
   for (int j = 0; j < SORT_PPT; ++j) {
     const int64_t i = base + threadIdx.x + SORT_NT * j;
     c[j] = i < a.n ? ac[threadIdx.x + SORT_NT * j] : 0;
+  }
+  if constexpr (HIST) {
+    // counting: each distinct key of a wave's lanes adds its lane count to the
+    // global counter directly (no return value, no LDS)
+    const int lane = __lane_id();
+#pragma unroll
+    for (int j = 0; j < SORT_PPT; ++j) {
+      const int64_t i = base + threadIdx.x + SORT_NT * j;
+      const bool act = i < a.n;
+      const int key = act ? tile_of(c[j], ntx, nty, twl, thl) : 0;
+      uint64_t todo = __ballot(act);
+      while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int lkey = __shfl(key, leader);
+        const uint64_t grp = __ballot(act && key == lkey) & todo;
+        if (lane == leader) atomicAdd(ctr + lkey, __popcll(grp));
+        todo &= ~grp;
+      }
+    }
+    return;
   }
   int slot[SORT_PPT], rank[SORT_PPT];
 #pragma unroll
