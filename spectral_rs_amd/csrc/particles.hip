@@ -933,15 +933,13 @@ __device__ __forceinline__ int tile_of(int32_t c, int ntx, int nty, int twl, int
   return ty * ntx + ((ty & 1) ? ntx - 1 - tx : tx);
 }
 
-// Each workgroup takes SORT_PPT * 256 consecutive particles.  Placement
-// aggregates them per tile in an LDS hash table (tile -> count), one LDS
-// atomic per group of lanes of a wave that share a tile, then one global
-// atomic per distinct tile of the workgroup (its first output position).  The
-// input is sorted by the previous pass except for the particles that changed
-// tile since, so a workgroup holds few distinct tiles.  Counting adds each
-// wave's group counts to the global counters directly, without a return value
-// (no LDS round trips: 203 -> 186 us per species at 4096 x 1024 after 20
-// steps of motion, against the hash-table aggregation).
+// Each workgroup takes SORT_PPT * 256 consecutive particles and counts them
+// per tile in an LDS hash table (tile -> count), one LDS atomic per group of
+// lanes of a wave that share a tile; then one global atomic per distinct tile
+// of the workgroup.  The input is sorted by the previous pass except for the
+// particles that changed tile since, so a workgroup holds few distinct tiles
+// and the global counters see ~1-3 atomics per workgroup instead of one per
+// wave and tile.
 #ifndef SP_SORT_HT_LOG
 #define SP_SORT_HT_LOG 11
 #endif
@@ -1040,30 +1038,6 @@ k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty, int twl, int thl
   constexpr int CH = SORT_NT * SORT_PPT;
   constexpr int OUT = HIST ? 2 : CH;
   static_assert(CH <= SORT_HT, "the staging pairs reuse the hash table");
-  if constexpr (HIST) {
-    // counting: each distinct key of a wave's lanes adds its lane count to the
-    // global counter directly (no return value, no LDS)
-    const int64_t base = (int64_t)blockIdx.x * CH;
-    const int32_t* const ac = a.cell + base;
-    int32_t c[SORT_PPT];
-#pragma unroll
-    for (int j = 0; j < SORT_PPT; ++j) c[j] = base + threadIdx.x + SORT_NT * j < a.n ? ac[threadIdx.x + SORT_NT * j] : 0;
-    const int lane = __lane_id();
-#pragma unroll
-    for (int j = 0; j < SORT_PPT; ++j) {
-      const bool act = base + threadIdx.x + SORT_NT * j < a.n;
-      const int key = act ? tile_of(c[j], ntx, nty, twl, thl) : 0;
-      uint64_t todo = __ballot(act);
-      while (todo) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const int lkey = __shfl(key, leader);
-        const uint64_t grp = __ballot(act && key == lkey) & todo;
-        if (lane == leader) atomicAdd(ctr + lkey, __popcll(grp));
-        todo &= ~grp;
-      }
-    }
-    return;
-  }
   // hash keys | counts; once the output positions are known, the same LDS
   // stages pairs of 32-bit values by local index
   __shared__ __attribute__((aligned(16))) int s_tab[2 * SORT_HT];
@@ -1113,15 +1087,20 @@ k_tile_pass(SpeciesDev<T> a, SpeciesDev<T> b, int ntx, int nty, int twl, int thl
     const int key = keys[sl];
     if (key < 0) continue;
     const int n = cnt[sl];
-    cnt[sl] = atomicAdd(ctr + key, n);  // the group's first output position
-    keys[sl] = atomicAdd(s_total, n);  // the group's first local index
-    const int d = atomicAdd(s_groups, 1);
-    // stride: the group's mean particle count per cell of the tile (n <=
-    // 2048 < 2^12, u^-1 < n, group index d < 512 whenever it is used)
-    int u = max(n >> (twl + thl), 1);
-    while (gcd_int(u, n) != 1) ++u;
-    s_n[sl] = n | (n > 1 ? inv_mod(u % n, n) : 0) << 12 | (d & 511) << 23;
+    if (HIST) {
+      atomicAdd(ctr + key, n);
+    } else {
+      cnt[sl] = atomicAdd(ctr + key, n);  // the group's first output position
+      keys[sl] = atomicAdd(s_total, n);  // the group's first local index
+      const int d = atomicAdd(s_groups, 1);
+      // stride: the group's mean particle count per cell of the tile (n <=
+      // 2048 < 2^12, u^-1 < n, group index d < 512 whenever it is used)
+      int u = max(n >> (twl + thl), 1);
+      while (gcd_int(u, n) != 1) ++u;
+      s_n[sl] = n | (n > 1 ? inv_mod(u % n, n) : 0) << 12 | (d & 511) << 23;
+    }
   }
+  if (HIST) return;
   __syncthreads();
   const int ngroups = *s_groups;
   __syncthreads();  // every thread has the count before s_gpos is rewritten
