@@ -205,7 +205,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--size", default=None, help="override grid, e.g. 1024x256 (not the metric workload)")
     ap.add_argument("--sort-interval", type=int, default=20, help="steps between tile sorts of the particles")
-    ap.add_argument("--sort-policy", default="5,50,0.05",
+    ap.add_argument("--sort-policy", default="3,40,0.03",
                     help="adaptive tile-sort schedule min,max,rise (sp_set_sort_policy), or 'fixed' for every "
                          "--sort-interval steps")
     ap.add_argument("--trace", type=int, default=0, help="print phase times every N steps (diagnostic)")

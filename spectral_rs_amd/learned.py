@@ -42,10 +42,10 @@ def shock_kw(size_x, size_y, ppc):
     return dict(SHOCK, size_x=size_x, size_y=size_y, delta=delta_for(size_x), dens=ppc)
 
 
-# the tile-sort schedule of every run here (the bench's): at least 5 and at
-# most 50 steps apart, sooner when the particles spread out of the particle
+# the tile-sort schedule of every run here (the bench's): at least 3 and at
+# most 40 steps apart, sooner when the particles spread out of the particle
 # pass's windows
-SORT_POLICY = (5, 50, 0.05)
+SORT_POLICY = (3, 40, 0.03)
 
 
 def shock_sim(size_x, size_y, ppc, seed=0, sort_interval=20, gamma=10.0, sort_policy=SORT_POLICY):
