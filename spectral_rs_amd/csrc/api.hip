@@ -113,18 +113,16 @@ struct sp_ctx {
 
 namespace {
 
-sp_tuning default_tuning(bool f64, int dens) {
+sp_tuning default_tuning(bool f64, int /*dens*/) {
   sp_tuning t{};
   t.chunk = f64 ? 1024 : 2048;
-  // FP32 sort tiles by density.  A chunk covers about chunk / dens cells; at
-  // >= 16 particles per cell (<= 128 cells) 16x2 tiles keep its gathers on the
-  // fewest cache lines (4096x1024 shock at 32 ppc, whole 2000-step run: 6.89
-  // ms/step vs 6.95 with 8x4 and 7.30 with 8x8).  Sparser runs use 8x4: a
-  // 2-row footprint 256+ cells wide outgrows the deposit window once particles
-  // spread a few rows (4 ppc: 4.1 vs 0.86 ms per particle pass).
-  const bool flat = !f64 && dens >= 16;
-  t.tile_w_log = flat ? 4 : 3;
-  t.tile_h_log = f64 ? 3 : (flat ? 1 : 2);
+  // FP32 sort tiles: 8x4 at every density.  4096x1024 shock at 32 ppc, whole
+  // 2000-step run on the adaptive sort schedule: 5.15 ms/step with 8x4 against
+  // 5.26 with 16x2, 5.42 with 8x2 and 16x4, 6.15 with 32x2.  At 4 ppc a 2-row
+  // footprint 256+ cells wide would outgrow the deposit window once particles
+  // spread a few rows (4.1 vs 0.86 ms per particle pass with 16x2).
+  t.tile_w_log = 3;
+  t.tile_h_log = f64 ? 3 : 2;
   t.fft_rows = 1;
   t.fft_cols = 8;
   t.fft_radix_x = 8;

@@ -43,10 +43,9 @@ def shock(nx, ny, dens, prec=8, sort_interval=3, delta=4, seed=0):
 
 
 def test_tuning_defaults_and_validation():
-    # FP32 default sort tiles by density: 8x4 below 16 particles per cell,
-    # 16x2 from 16 on (a 2048-particle chunk then covers <= 128 cells)
+    # FP32 default sort tiles: 8x4 at every density
     dense = shock(64, 16, 16, prec=4)
-    assert (dense.tuning()["tile_w_log"], dense.tuning()["tile_h_log"]) == (4, 1)
+    assert (dense.tuning()["tile_w_log"], dense.tuning()["tile_h_log"]) == (3, 2)
     s = shock(64, 16, 2, prec=4)
     t = s.tuning()
     assert t == dict(chunk=2048, tile_w_log=3, tile_h_log=2, fft_rows=1, fft_cols=8, fft_radix_x=8, fft_radix_y=8)
